@@ -1,0 +1,192 @@
+/*
+ * artea_oracle.h -- CPU restatement of the ARTEA search and refine hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle and the
+ * timed CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  The product
+ * library (artea_amd/libartea_hip.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line whose semantics it restates
+ * (paths relative to include/artea/cpu/ of the reference checkout).
+ *
+ * Pinning status (what checks each part, see tests/test_oracle_*.py):
+ *   distance      closed forms + reference KATs (test_euclidean_distance.cpp,
+ *                 test_cosine_distance.cpp) + FP64 brute force; GCC
+ *                 _mm512_reduce_add_ps tree checked against an AVX-512 probe.
+ *   queue/router  reference KATs (test_compact_csr_query.cpp,
+ *                 test_router_early_stop.cpp, test_candidate_queue.cpp) and
+ *                 brute force on tiny graphs.
+ *   HGRA/compact  byte-exact wire format (hierarchical_graph_file_manager.hpp
+ *                 :20-40) round trips and corrupt-file rejection.
+ *   refine        reference KATs (test_propagate_engine.cpp triangle cases,
+ *                 test_graph_build_regressions.cpp reverse-between-rounds),
+ *                 row invariants, RNG property.
+ *   prefill RNG   parity unpinned: the reference draws from MKL MT19937
+ *                 seeded by std::random_device; this oracle and the GPU use
+ *                 the same counter-based SplitMix64 stream instead.
+ */
+#ifndef ARTEA_ORACLE_H
+#define ARTEA_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* DistanceMetricsT values, framework/type_traits/computer_traits.hpp:38-43 */
+enum { ORC_L2SQR = 0, ORC_DOT = 1, ORC_COSINE = 2, ORC_L2 = 3 };
+
+#define ORC_INVALID_VID 0x7FFFFFFFu
+#define ORC_INVALID_LEVEL 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------- distance */
+float orc_distance(int metric, const float* a, const float* b, uint32_t dim_pad);
+/* FP64 reference of the same metric (no FP32 rounding inside). */
+double orc_distance_f64(int metric, const float* a, const float* b, uint32_t dim_pad);
+/* dists[i*nb + j] = orc_distance(metric, a_i, b_j) */
+void orc_distance_matrix(int metric, const float* a, uint32_t na, const float* b, uint32_t nb,
+                         uint32_t dim_pad, float* dists, int nthreads);
+
+/* ------------------------------------------------------------ compact graph */
+typedef struct orc_graph orc_graph;
+
+/* Build from a "dynamic" description: highest[v] (ORC_INVALID_LEVEL =
+ * unassigned) and, per level l in [0, max_level], a CSR over vertices
+ * (row_off[l][v] .. row_off[l][v+1]) of neighbour ids in row order.  Rows of
+ * vertices with highest < l must be empty.  This runs the compactor
+ * (compactor/hierarchical_graph_compactor.hpp:118-252), including top-bucket
+ * trimming and the FP64-centroid entry point computed with `metric`. */
+orc_graph* orc_compact_graph(uint32_t n, uint32_t max_level, uint32_t ul_max, uint32_t bl_max,
+                             const uint32_t* highest, const uint64_t* const* row_off,
+                             const uint32_t* const* row_ids, const float* vecs, uint32_t dim_pad,
+                             int metric, char* err, size_t errlen);
+/* Direct CSR construction, as the builder-only mutators of
+ * compact_structure/hierarchical_graph.hpp:135-162 allow: vertex_info[2v] =
+ * highest level (ORC_INVALID_LEVEL = unassigned), vertex_info[2v+1] = local
+ * id; group_offsets[h] has count[h]*(h+1)+1 entries (count[h] = number of
+ * vertices with highest h).  Copies everything. */
+orc_graph* orc_graph_from_csr(uint32_t n, uint32_t top, uint32_t ul_max, uint32_t bl_max, uint32_t entry,
+                              const uint32_t* vertex_info, const uint64_t* const* group_offsets,
+                              const uint32_t* nbrs, uint64_t n_nbrs, char* err, size_t errlen);
+/* HierarchicalGraphFileManager::snapshot / restore
+ * (persistence/hierarchical_graph_file_manager.hpp:88-145, 152-324). */
+int orc_graph_snapshot(const orc_graph* g, const char* path, char* err, size_t errlen);
+orc_graph* orc_graph_restore(const char* path, char* err, size_t errlen);
+void orc_graph_free(orc_graph* g);
+
+/* Accessors (compact_structure/hierarchical_graph.hpp:83-132). */
+uint32_t orc_graph_num_vertices(const orc_graph* g);
+uint32_t orc_graph_top(const orc_graph* g); /* ORC_INVALID_LEVEL when empty */
+uint32_t orc_graph_entry(const orc_graph* g);
+uint32_t orc_graph_ul_max(const orc_graph* g);
+uint32_t orc_graph_bl_max(const orc_graph* g);
+uint32_t orc_graph_highest(const orc_graph* g, uint32_t vid);
+uint32_t orc_graph_local(const orc_graph* g, uint32_t vid);
+uint64_t orc_graph_group_count(const orc_graph* g, uint32_t h);
+const uint64_t* orc_graph_group_offsets(const orc_graph* g, uint32_t h);
+const uint32_t* orc_graph_nbrs(const orc_graph* g, uint64_t* n_nbrs);
+/* fetch_level_nbrs (compact hierarchical_graph.hpp:99-108) */
+const uint32_t* orc_graph_level_nbrs(const orc_graph* g, uint32_t vid, uint32_t level,
+                                     uint32_t* count);
+
+/* ------------------------------------------------------------------ router */
+enum { ORC_EARLY_STOP = 1u, ORC_L0_ONLY = 2u };
+
+/* HierarchicalGraphRouter::_batch_query / batch_query_l0_only
+ * (router/hierarchical_graph_router.hpp:195-248, 258-286, 322-385).
+ * thresholds: per level (index = level), already in query distance units
+ * (squared for L2SQR), only read with ORC_EARLY_STOP.  Results row-major
+ * nq x topk, padded with (ORC_INVALID_VID, FLT_MAX).  ndc (optional, nq)
+ * counts distance evaluations per query, entry included
+ * (router/profiler/hg_router_profiler.hpp:153-197).
+ * Returns 0, or -1 with err set (queue < topk, bad entry point). */
+int orc_batch_query(const orc_graph* g, const float* base, uint32_t dim_pad, int metric,
+                    const float* queries, uint32_t nq, uint32_t topk, uint32_t queue,
+                    uint32_t flags, const float* thresholds, uint32_t* out_vids,
+                    float* out_dists, uint64_t* ndc, int nthreads, char* err, size_t errlen);
+
+/* One query with an external visited-table state; exposes the greedy
+ * trajectory for KAT tests: per level from top to 1, the cursor after the
+ * greedy walk (cursor_vids[level]). */
+int orc_query_trace(const orc_graph* g, const float* base, uint32_t dim_pad, int metric,
+                    const float* query, uint32_t topk, uint32_t queue, uint32_t flags,
+                    const float* thresholds, uint32_t* out_vids, float* out_dists,
+                    uint32_t* out_count, uint32_t* cursor_vids, uint64_t* ndc);
+
+/* ------------------------------------------------------- candidate queue */
+/* StdCandidateQueue (router/data_structures/std_candidate_queue.hpp:68-595),
+ * exposed for the queue KATs. */
+typedef struct orc_queue orc_queue;
+orc_queue* orc_queue_create(uint32_t capacity);
+void orc_queue_free(orc_queue* q);
+int orc_queue_try_push(orc_queue* q, uint32_t vid, float dist);
+int orc_queue_pop_best(orc_queue* q, uint32_t* vid, float* dist); /* 0 = empty */
+int orc_queue_should_terminate(const orc_queue* q);
+float orc_queue_lower_bound(const orc_queue* q);
+uint32_t orc_queue_result_size(const orc_queue* q);
+uint32_t orc_queue_unexplored_size(const orc_queue* q);
+void orc_queue_reset_exploration(orc_queue* q);
+uint32_t orc_queue_extract(orc_queue* q, uint32_t k, uint32_t* vids, float* dists);
+
+/* ----------------------------------------------------------------- refine */
+/* Neighbor (index/neighbor.hpp:43-202): vid:31 | new:1, FP32 distance. */
+#define ORC_NBR_NEW 0x80000000u
+typedef struct {
+    uint32_t vid_status;
+    float dist;
+} orc_nbr;
+
+/* dynamic::SingleLayerGraph + PropagateEngine + NbrLogTable
+ * (index/dynamic_structure/single_layer_graph.hpp:29-227,
+ *  refiner/propagate_engine.hpp:38-325, refiner/nbr_log_table.hpp:127-182). */
+typedef struct orc_layer orc_layer;
+/* local_to_global NULL = identity (dense L0 mode); rows are indexed by local
+ * id, entries hold global (storage) vids.  cap = layer max_nbr_size. */
+orc_layer* orc_layer_create(const float* vecs, uint32_t n_global, uint32_t dim_pad, int metric,
+                            uint32_t cap, const uint32_t* local_to_global, uint32_t n_local);
+void orc_layer_free(orc_layer* L);
+uint32_t orc_layer_num_rows(const orc_layer* L);
+/* fill_single_layer_graph_from_layer (refiner/refiner_utils.hpp:73-110):
+ * copies entries up to the first invalid, sorts by (dist, vid), truncates
+ * to cap. */
+void orc_layer_fill_row(orc_layer* L, uint32_t local, const orc_nbr* entries, uint32_t count);
+/* Raw setter (no sort / truncate), for KATs that build exact states. */
+void orc_layer_set_row_raw(orc_layer* L, uint32_t local, const orc_nbr* entries, uint32_t count);
+uint32_t orc_layer_row_size(const orc_layer* L, uint32_t local);
+uint32_t orc_layer_get_row(const orc_layer* L, uint32_t local, orc_nbr* out, uint32_t max_out);
+/* Bulk export: counts[n_local], entries written row after row. */
+uint64_t orc_layer_export(const orc_layer* L, uint32_t* counts, orc_nbr* entries, uint64_t max_entries);
+
+/* PropagateEngine::next / next_range with one updater each; every call is
+ * propagate (parallel over rows) followed by merge_logs. */
+void orc_next_prefill(orc_layer* L, uint32_t gen_size, uint32_t start_vid, uint32_t end_vid,
+                      uint32_t row_begin, uint32_t row_end, uint64_t seed, uint64_t stream,
+                      int nthreads);
+void orc_next_triangle(orc_layer* L, float scale_coeffs, float shifted_coeffs, int nthreads);
+void orc_next_truncate(orc_layer* L, int nthreads);
+void orc_next_reverse(orc_layer* L, int nthreads);
+/* The artea_graph::IndexFactory::refine_layer schedule
+ * (index/artea_graph/index_factory.hpp:344-393): optional L0 prefill, then
+ * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */
+void orc_refine_schedule(orc_layer* L, uint32_t build_loops, uint32_t triu_iters, float scale_coeffs,
+                         float effective_shift, int nthreads);
+/* Counters of the last call (logs merged, pair distances evaluated). */
+uint64_t orc_layer_last_logs(const orc_layer* L);
+uint64_t orc_layer_pair_distances(const orc_layer* L);
+
+/* Counter-based stream shared with the GPU prefill: uniform id in [lo, hi). */
+uint32_t orc_rand_draw(uint64_t seed, uint64_t stream, uint32_t row, uint32_t j, uint32_t lo,
+                       uint32_t hi);
+
+/* -------------------------------------------------------------- ground truth */
+/* FP64 brute-force top-k by (distance, vid); metric as above. */
+void orc_bruteforce_knn(int metric, const float* base, uint32_t n, const float* queries,
+                        uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* out_vids,
+                        double* out_dists, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,411 @@
+"""ctypes view of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.  The product package
+``artea_amd`` never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+L2SQR, DOT, COSINE, L2 = 0, 1, 2, 3
+INVALID_VID = 0x7FFFFFFF
+INVALID_LEVEL = 0xFFFFFFFF
+NBR_NEW = 0x80000000
+EARLY_STOP, L0_ONLY = 1, 2
+
+NBR_DTYPE = np.dtype([("vid_status", "<u4"), ("dist", "<f4")])
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+u32, u64, f32, i32 = C.c_uint32, C.c_uint64, C.c_float, C.c_int
+
+
+def _declare(l):
+    def d(name, res, *args):
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    d("orc_distance", f32, i32, P, P, u32)
+    d("orc_distance_f64", C.c_double, i32, P, P, u32)
+    d("orc_distance_matrix", None, i32, P, u32, P, u32, u32, P, i32)
+    d("orc_compact_graph", P, u32, u32, u32, u32, P, P, P, P, u32, i32, C.c_char_p, C.c_size_t)
+    d("orc_graph_from_csr", P, u32, u32, u32, u32, u32, P, P, P, u64, C.c_char_p, C.c_size_t)
+    d("orc_graph_snapshot", i32, P, C.c_char_p, C.c_char_p, C.c_size_t)
+    d("orc_graph_restore", P, C.c_char_p, C.c_char_p, C.c_size_t)
+    d("orc_graph_free", None, P)
+    for n in ("num_vertices", "top", "entry", "ul_max", "bl_max"):
+        d("orc_graph_" + n, u32, P)
+    d("orc_graph_highest", u32, P, u32)
+    d("orc_graph_local", u32, P, u32)
+    d("orc_graph_group_count", u64, P, u32)
+    d("orc_graph_group_offsets", C.POINTER(u64), P, u32)
+    d("orc_graph_nbrs", C.POINTER(u32), P, C.POINTER(u64))
+    d("orc_graph_level_nbrs", C.POINTER(u32), P, u32, u32, C.POINTER(u32))
+    d("orc_batch_query", i32, P, P, u32, i32, P, u32, u32, u32, u32, P, P, P, P, i32, C.c_char_p, C.c_size_t)
+    d("orc_query_trace", i32, P, P, u32, i32, P, u32, u32, u32, P, P, P, P, P, P)
+    d("orc_queue_create", P, u32)
+    d("orc_queue_free", None, P)
+    d("orc_queue_try_push", i32, P, u32, f32)
+    d("orc_queue_pop_best", i32, P, C.POINTER(u32), C.POINTER(f32))
+    d("orc_queue_should_terminate", i32, P)
+    d("orc_queue_lower_bound", f32, P)
+    d("orc_queue_result_size", u32, P)
+    d("orc_queue_unexplored_size", u32, P)
+    d("orc_queue_reset_exploration", None, P)
+    d("orc_queue_extract", u32, P, u32, P, P)
+    d("orc_layer_create", P, P, u32, u32, i32, u32, P, u32)
+    d("orc_layer_free", None, P)
+    d("orc_layer_num_rows", u32, P)
+    d("orc_layer_fill_row", None, P, u32, P, u32)
+    d("orc_layer_set_row_raw", None, P, u32, P, u32)
+    d("orc_layer_row_size", u32, P, u32)
+    d("orc_layer_get_row", u32, P, u32, P, u32)
+    d("orc_layer_export", u64, P, P, P, u64)
+    d("orc_next_prefill", None, P, u32, u32, u32, u32, u32, u64, u64, i32)
+    d("orc_next_triangle", None, P, f32, f32, i32)
+    d("orc_next_truncate", None, P, i32)
+    d("orc_next_reverse", None, P, i32)
+    d("orc_refine_schedule", None, P, u32, u32, f32, f32, i32)
+    d("orc_layer_last_logs", u64, P)
+    d("orc_layer_pair_distances", u64, P)
+    d("orc_rand_draw", u32, u64, u64, u32, u32, u32, u32)
+    d("orc_bruteforce_knn", None, i32, P, u32, P, u32, u32, u32, P, P, i32)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ------------------------------------------------------------------ distance
+def distance(metric: int, a, b) -> np.float32:
+    a, b = _f32(a), _f32(b)
+    return np.float32(lib().orc_distance(metric, _ptr(a), _ptr(b), a.shape[-1]))
+
+
+def distance_f64(metric: int, a, b) -> float:
+    a, b = _f32(a), _f32(b)
+    return lib().orc_distance_f64(metric, _ptr(a), _ptr(b), a.shape[-1])
+
+
+def distance_matrix(metric: int, a, b, nthreads: int = 0) -> np.ndarray:
+    a, b = _f32(a), _f32(b)
+    out = np.empty((a.shape[0], b.shape[0]), np.float32)
+    lib().orc_distance_matrix(metric, _ptr(a), a.shape[0], _ptr(b), b.shape[0], a.shape[1], _ptr(out), nthreads)
+    return out
+
+
+# --------------------------------------------------------------------- graph
+class Graph:
+    """Compact hierarchical graph held by the oracle."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("null graph handle")
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_graph_free(self.h)
+            self.h = None
+
+    @classmethod
+    def compact(cls, highest, levels, vecs, metric, ul_max, bl_max):
+        """levels: list over level l of list over vid of neighbour-id lists."""
+        n = len(highest)
+        max_level = len(levels) - 1
+        hl = np.ascontiguousarray(highest, dtype=np.uint32)
+        offs, ids = [], []
+        for l in range(max_level + 1):
+            rows = levels[l]
+            cnt = np.array([len(rows[v]) if v < len(rows) and rows[v] is not None else 0 for v in range(n)],
+                           dtype=np.uint64)
+            off = np.zeros(n + 1, np.uint64)
+            np.cumsum(cnt, out=off[1:])
+            flat = np.zeros(max(int(off[-1]), 1), np.uint32)
+            for v in range(n):
+                if cnt[v]:
+                    flat[off[v]:off[v + 1]] = rows[v]
+            offs.append(off)
+            ids.append(flat)
+        off_ptrs = (C.c_void_p * (max_level + 1))(*[o.ctypes.data for o in offs])
+        id_ptrs = (C.c_void_p * (max_level + 1))(*[i.ctypes.data for i in ids])
+        v = _f32(vecs)
+        err = C.create_string_buffer(512)
+        h = lib().orc_compact_graph(n, max_level, ul_max, bl_max, _ptr(hl), off_ptrs, id_ptrs, _ptr(v),
+                                    v.shape[1] if v.ndim == 2 else 16, metric, err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, n, top, ul_max, bl_max, entry, vertex_info, group_offsets, nbrs):
+        vi = np.ascontiguousarray(vertex_info, np.uint32).reshape(-1)
+        gos = [np.ascontiguousarray(g, np.uint64) for g in group_offsets]
+        ptrs = (C.c_void_p * len(gos))(*[g.ctypes.data for g in gos])
+        nb = np.ascontiguousarray(nbrs, np.uint32)
+        if nb.size == 0:
+            nb = np.zeros(1, np.uint32)
+            nn = 0
+        else:
+            nn = nb.size
+        err = C.create_string_buffer(512)
+        h = lib().orc_graph_from_csr(n, top, ul_max, bl_max, entry, _ptr(vi), ptrs, _ptr(nb), nn, err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        g = cls(h)
+        g._keep = (vi, gos, nb)
+        return g
+
+    @classmethod
+    def restore(cls, path: str):
+        err = C.create_string_buffer(512)
+        h = lib().orc_graph_restore(path.encode(), err, 512)
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    def snapshot(self, path: str):
+        err = C.create_string_buffer(512)
+        if lib().orc_graph_snapshot(self.h, path.encode(), err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
+    @property
+    def num_vertices(self):
+        return lib().orc_graph_num_vertices(self.h)
+
+    @property
+    def top(self):
+        return lib().orc_graph_top(self.h)
+
+    @property
+    def entry(self):
+        return lib().orc_graph_entry(self.h)
+
+    @property
+    def ul_max(self):
+        return lib().orc_graph_ul_max(self.h)
+
+    @property
+    def bl_max(self):
+        return lib().orc_graph_bl_max(self.h)
+
+    def highest(self, v):
+        return lib().orc_graph_highest(self.h, v)
+
+    def local(self, v):
+        return lib().orc_graph_local(self.h, v)
+
+    def level_nbrs(self, v, level) -> np.ndarray:
+        cnt = u32()
+        p = lib().orc_graph_level_nbrs(self.h, v, level, C.byref(cnt))
+        return np.ctypeslib.as_array(p, shape=(cnt.value,)).copy() if cnt.value else np.zeros(0, np.uint32)
+
+    def csr(self):
+        """(highest[N], local[N], [group offsets arrays], nbrs) copies."""
+        n = self.num_vertices
+        hl = np.array([self.highest(v) for v in range(n)], np.uint32)
+        lc = np.array([self.local(v) for v in range(n)], np.uint32)
+        top = self.top if n else 0
+        groups = []
+        for h in range(top + 1):
+            cnt = lib().orc_graph_group_count(self.h, h)
+            p = lib().orc_graph_group_offsets(self.h, h)
+            groups.append(np.ctypeslib.as_array(p, shape=(cnt * (h + 1) + 1,)).copy())
+        nn = u64()
+        p = lib().orc_graph_nbrs(self.h, C.byref(nn))
+        nbrs = np.ctypeslib.as_array(p, shape=(nn.value,)).copy() if nn.value else np.zeros(0, np.uint32)
+        return hl, lc, groups, nbrs
+
+
+def batch_query(g: Graph, base, metric, queries, topk, queue, flags=0, thresholds=None, nthreads=0):
+    base, queries = _f32(base), _f32(queries)
+    nq = queries.shape[0]
+    ov = np.empty((nq, topk), np.uint32)
+    od = np.empty((nq, topk), np.float32)
+    ndc = np.empty(nq, np.uint64)
+    thr = _f32(thresholds if thresholds is not None else np.zeros(1, np.float32))
+    err = C.create_string_buffer(512)
+    rc = lib().orc_batch_query(g.h, _ptr(base), base.shape[1], metric, _ptr(queries), nq, topk, queue, flags,
+                               _ptr(thr), _ptr(ov), _ptr(od), _ptr(ndc), nthreads, err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return ov, od, ndc
+
+
+def query_trace(g: Graph, base, metric, query, topk, queue, flags=0, thresholds=None):
+    base, query = _f32(base), _f32(query)
+    ov = np.empty(topk, np.uint32)
+    od = np.empty(topk, np.float32)
+    cnt = u32()
+    cur = np.full(max(g.top + 1, 1) if g.num_vertices else 1, 0xFFFFFFFF, np.uint32)
+    ndc = u64()
+    thr = _f32(thresholds if thresholds is not None else np.zeros(1, np.float32))
+    rc = lib().orc_query_trace(g.h, _ptr(base), base.shape[1], metric, _ptr(query), topk, queue, flags, _ptr(thr),
+                               _ptr(ov), _ptr(od), C.byref(cnt), _ptr(cur), C.byref(ndc))
+    if rc != 0:
+        raise RuntimeError("query_trace failed")
+    return ov[:cnt.value], od[:cnt.value], cur, ndc.value
+
+
+# --------------------------------------------------------------------- queue
+class Queue:
+    def __init__(self, capacity):
+        self.h = lib().orc_queue_create(capacity)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_queue_free(self.h)
+            self.h = None
+
+    def try_push(self, vid, dist):
+        return bool(lib().orc_queue_try_push(self.h, vid, dist))
+
+    def pop_best(self):
+        v, d = u32(), f32()
+        if not lib().orc_queue_pop_best(self.h, C.byref(v), C.byref(d)):
+            return None
+        return v.value, d.value
+
+    def should_terminate(self):
+        return bool(lib().orc_queue_should_terminate(self.h))
+
+    @property
+    def lower_bound(self):
+        return lib().orc_queue_lower_bound(self.h)
+
+    @property
+    def result_size(self):
+        return lib().orc_queue_result_size(self.h)
+
+    @property
+    def unexplored_size(self):
+        return lib().orc_queue_unexplored_size(self.h)
+
+    def reset_exploration(self):
+        lib().orc_queue_reset_exploration(self.h)
+
+    def extract(self, k):
+        v = np.empty(max(k, 1), np.uint32)
+        d = np.empty(max(k, 1), np.float32)
+        m = lib().orc_queue_extract(self.h, k, _ptr(v), _ptr(d))
+        return v[:m], d[:m]
+
+
+# -------------------------------------------------------------------- refine
+def nbrs(pairs, new=False):
+    """[(vid, dist), ...] -> structured array of Neighbor records."""
+    a = np.zeros(len(pairs), NBR_DTYPE)
+    for i, (v, d) in enumerate(pairs):
+        a[i] = (v | (NBR_NEW if new else 0), d)
+    return a
+
+
+class Layer:
+    def __init__(self, vecs, metric, cap, local_to_global=None):
+        self.vecs = _f32(vecs)
+        l2g = None if local_to_global is None else np.ascontiguousarray(local_to_global, np.uint32)
+        self._l2g = l2g
+        self.h = lib().orc_layer_create(_ptr(self.vecs), self.vecs.shape[0], self.vecs.shape[1], metric, cap,
+                                        _ptr(l2g) if l2g is not None else None,
+                                        0 if l2g is None else len(l2g))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_layer_free(self.h)
+            self.h = None
+
+    @property
+    def num_rows(self):
+        return lib().orc_layer_num_rows(self.h)
+
+    def fill_row(self, local, entries):
+        e = np.ascontiguousarray(entries, NBR_DTYPE)
+        lib().orc_layer_fill_row(self.h, local, _ptr(e), len(e))
+
+    def set_row_raw(self, local, entries):
+        e = np.ascontiguousarray(entries, NBR_DTYPE)
+        lib().orc_layer_set_row_raw(self.h, local, _ptr(e), len(e))
+
+    def row(self, local):
+        n = lib().orc_layer_row_size(self.h, local)
+        out = np.zeros(max(n, 1), NBR_DTYPE)
+        lib().orc_layer_get_row(self.h, local, _ptr(out), n)
+        return out[:n]
+
+    def export(self):
+        n = self.num_rows
+        counts = np.zeros(n, np.uint32)
+        total = lib().orc_layer_export(self.h, _ptr(counts), None, 0)
+        ent = np.zeros(max(total, 1), NBR_DTYPE)
+        lib().orc_layer_export(self.h, _ptr(counts), _ptr(ent), total)
+        return counts, ent[:total]
+
+    def prefill(self, gen_size, start, end, row_begin=0, row_end=None, seed=0, stream=0, nthreads=0):
+        lib().orc_next_prefill(self.h, gen_size, start, end, row_begin,
+                               self.num_rows if row_end is None else row_end, seed, stream, nthreads)
+
+    def triangle(self, scale, shift, nthreads=0):
+        lib().orc_next_triangle(self.h, scale, shift, nthreads)
+
+    def truncate(self, nthreads=0):
+        lib().orc_next_truncate(self.h, nthreads)
+
+    def reverse(self, nthreads=0):
+        lib().orc_next_reverse(self.h, nthreads)
+
+    def schedule(self, build_loops, triu_iters, scale, shift, nthreads=0):
+        lib().orc_refine_schedule(self.h, build_loops, triu_iters, scale, shift, nthreads)
+
+    @property
+    def last_logs(self):
+        return lib().orc_layer_last_logs(self.h)
+
+    @property
+    def pair_distances(self):
+        return lib().orc_layer_pair_distances(self.h)
+
+
+def rand_draw(seed, stream, row, j, lo, hi):
+    return lib().orc_rand_draw(seed, stream, row, j, lo, hi)
+
+
+def bruteforce_knn(metric, base, queries, k, nthreads=0):
+    base, queries = _f32(base), _f32(queries)
+    nq = queries.shape[0]
+    ov = np.empty((nq, k), np.uint32)
+    od = np.empty((nq, k), np.float64)
+    lib().orc_bruteforce_knn(metric, _ptr(base), base.shape[0], _ptr(queries), nq, base.shape[1], k, _ptr(ov),
+                             _ptr(od), nthreads)
+    return ov, od
