@@ -1,0 +1,40 @@
+// Shared host/device declarations of libartea_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "host.hpp"
+
+#define ARTEA_HIP_TRY(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            ::artea::set_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));      \
+            return ARTEA_EHIP;                                                                \
+        }                                                                                     \
+    } while (0)
+
+
+// Device layout of an attached graph (the MI355X-side form of the compact
+// CSR): L0 rows as a padded ELL array indexed directly by vid, so a beam
+// expansion is one coalesced row read with no pointer chasing; upper-level
+// rows as a second ELL array addressed through vinfo[v] = {highest,
+// first upper row}.  Row order inside every row equals the CSR order.
+struct DeviceGraph {
+    uint32_t top = 0, entry = 0, ul_max = 0, bl_max = 0;
+    uint32_t stride0 = 16, strideU = 16;  // ELL row strides (multiples of 16)
+    uint32_t* ell0 = nullptr;             // n * stride0
+    uint2* vinfo = nullptr;               // n
+    uint32_t* ellU = nullptr;             // rowsU * strideU
+    uint64_t rowsU = 0;
+    bool attached = false;
+};
+
+struct artea_index {
+    uint32_t n = 0, dim_pad = 0;
+    int metric = 0;
+    float* d_vecs = nullptr;
+    DeviceGraph dg;
+    artea::HostCsr csr;  // host copy for snapshot / info
+    hipStream_t stream = nullptr;
+};

@@ -1,0 +1,252 @@
+// HGRA v1 snapshot / restore of the compact hierarchical graph
+// (persistence/hierarchical_graph_file_manager.hpp:20-40 wire format,
+// :88-145 snapshot, :152-324 restore).  Same validation order and message
+// texts as the reference's ARTEA_ERROR calls.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "host.hpp"
+
+namespace artea {
+
+namespace {
+constexpr uint32_t kMagic = 0x48475241u;  // "HGRA"
+constexpr uint32_t kVersion = 1u;
+
+struct File {
+    FILE* f = nullptr;
+    ~File() {
+        if (f) fclose(f);
+    }
+};
+
+bool rd32(FILE* f, uint32_t* x) { return fread(x, 4, 1, f) == 1; }
+}  // namespace
+
+int hgra_read(const char* path, HostCsr* out) {
+    const std::string p = path ? path : "";
+    File fh;
+    fh.f = fopen(p.c_str(), "rb");
+    if (!fh.f) {
+        set_error("HierarchicalGraphFileManager::restore: failed to open " + p);
+        return ARTEA_EIO;
+    }
+    FILE* f = fh.f;
+    uint32_t hdr[7];
+    if (fread(hdr, 4, 7, f) != 7) {
+        set_error("HierarchicalGraphFileManager::restore: header read failed: " + p);
+        return ARTEA_EIO;
+    }
+    char buf[256];
+    if (hdr[0] != kMagic) {
+        snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: bad magic %#x in ", hdr[0]);
+        set_error(buf + p);
+        return ARTEA_EIO;
+    }
+    if (hdr[1] != kVersion) {
+        snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: unsupported version %u in ", hdr[1]);
+        set_error(buf + p);
+        return ARTEA_EIO;
+    }
+    HostCsr g;
+    g.n = hdr[2];
+    g.top = hdr[3];
+    g.ul_max = hdr[4];
+    g.bl_max = hdr[5];
+    g.entry = hdr[6];
+    if (g.top == kInvalidLevel || (g.entry != kNoVertex && g.entry >= g.n)) {
+        set_error("HierarchicalGraphFileManager::restore: invalid header in " + p);
+        return ARTEA_EIO;
+    }
+    const uint32_t groups = g.top + 1;
+    g.highest.assign(g.n, kInvalidLevel);
+    g.local.assign(g.n, kNoVertex);
+    g.offsets.assign(groups, {});
+    // Per vid, levels h..0: counts then ids, kept in file order first.
+    std::vector<uint32_t> lens;
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> hist(groups, 0);
+    lens.reserve(size_t(g.n) * 2);
+    for (uint32_t v = 0; v < g.n; ++v) {
+        uint32_t h;
+        if (!rd32(f, &h)) {
+            snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u in ", v);
+            set_error(buf + p);
+            return ARTEA_EIO;
+        }
+        g.highest[v] = h;
+        if (h == kInvalidLevel) continue;
+        if (h > g.top) {
+            snprintf(buf, sizeof buf,
+                     "HierarchicalGraphFileManager::restore: vid %u has highest_level_id %u > top_level_id %u in ",
+                     v, h, g.top);
+            set_error(buf + p);
+            return ARTEA_EIO;
+        }
+        hist[h]++;
+        for (uint32_t l = h;; --l) {
+            uint32_t c;
+            if (!rd32(f, &c)) {
+                snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u level %u in ",
+                         v, l);
+                set_error(buf + p);
+                return ARTEA_EIO;
+            }
+            const uint32_t cap = l == 0 ? g.bl_max : g.ul_max;
+            if (c > cap) {
+                snprintf(buf, sizeof buf,
+                         "HierarchicalGraphFileManager::restore: vid %u level %u count %u > capacity %u in ", v, l,
+                         c, cap);
+                set_error(buf + p);
+                return ARTEA_EIO;
+            }
+            lens.push_back(c);
+            const size_t at = ids.size();
+            ids.resize(at + c);
+            if (c && fread(ids.data() + at, 4, c, f) != c) {
+                snprintf(buf, sizeof buf,
+                         "HierarchicalGraphFileManager::restore: truncated reading nbrs of vid %u level %u in ", v, l);
+                set_error(buf + p);
+                return ARTEA_EIO;
+            }
+            for (uint32_t i = 0; i < c; ++i) {
+                if (ids[at + i] >= g.n) {
+                    snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: invalid neighbor %u in ",
+                             ids[at + i]);
+                    set_error(buf + p);
+                    return ARTEA_EIO;
+                }
+            }
+            if (l == 0) break;
+        }
+    }
+    // Buckets in ascending vid order; row counts, prefix sum, then ids.
+    std::vector<uint64_t> fill(groups, 0);
+    for (uint32_t h = 0; h < groups; ++h) g.offsets[h].assign(hist[h] * (h + 1) + 1, 0);
+    size_t li = 0;
+    for (uint32_t v = 0; v < g.n; ++v) {
+        const uint32_t h = g.highest[v];
+        if (h == kInvalidLevel) continue;
+        const uint64_t loc = fill[h]++;
+        g.local[v] = uint32_t(loc);
+        for (uint32_t l = h;; --l) {
+            g.offsets[h][loc * (h + 1) + (h - l)] = lens[li++];
+            if (l == 0) break;
+        }
+    }
+    uint64_t total = 0;
+    for (uint32_t h = 0; h < groups; ++h) {
+        auto& o = g.offsets[h];
+        for (size_t r = 0; r + 1 < o.size(); ++r) {
+            const uint64_t len = o[r];
+            o[r] = total;
+            total += len;
+        }
+        o.back() = total;
+    }
+    g.nbrs.resize(total);
+    size_t pos = 0;
+    for (uint32_t v = 0; v < g.n; ++v) {
+        const uint32_t h = g.highest[v];
+        if (h == kInvalidLevel) continue;
+        for (uint32_t l = h;; --l) {
+            const uint64_t r = uint64_t(g.local[v]) * (h + 1) + (h - l);
+            const uint64_t c = g.offsets[h][r + 1] - g.offsets[h][r];
+            memcpy(g.nbrs.data() + g.offsets[h][r], ids.data() + pos, c * 4);
+            pos += c;
+            if (l == 0) break;
+        }
+    }
+    *out = std::move(g);
+    return ARTEA_OK;
+}
+
+int hgra_write(const HostCsr& g, const char* path) {
+    const std::string p = path ? path : "";
+    File fh;
+    fh.f = fopen(p.c_str(), "wb");
+    if (!fh.f) {
+        set_error("HierarchicalGraphFileManager::snapshot: failed to open " + p);
+        return ARTEA_EIO;
+    }
+    const uint32_t hdr[7] = {kMagic, kVersion, g.n, g.n == 0 ? 0u : g.top, g.ul_max, g.bl_max, g.entry};
+    bool ok = fwrite(hdr, 4, 7, fh.f) == 7;
+    for (uint32_t v = 0; ok && v < g.n; ++v) {
+        const uint32_t h = g.highest[v];
+        ok = fwrite(&h, 4, 1, fh.f) == 1;
+        if (h == kInvalidLevel) continue;
+        for (uint32_t l = h; ok; --l) {
+            uint32_t c;
+            const uint32_t* r = g.row(v, l, &c);
+            ok = fwrite(&c, 4, 1, fh.f) == 1 && (c == 0 || fwrite(r, 4, c, fh.f) == c);
+            if (l == 0) break;
+        }
+    }
+    if (!ok || ferror(fh.f)) {
+        set_error("HierarchicalGraphFileManager::snapshot: write failure on " + p);
+        return ARTEA_EIO;
+    }
+    return ARTEA_OK;
+}
+
+int csr_validate(const HostCsr& g) {
+    char buf[256];
+    if (g.n == 0) return ARTEA_OK;
+    if (g.offsets.size() != size_t(g.top) + 1) {
+        set_error("HierarchicalGraph: group count does not match top level");
+        return ARTEA_EINVAL;
+    }
+    std::vector<uint64_t> count(g.top + 1, 0);
+    for (uint32_t v = 0; v < g.n; ++v) {
+        const uint32_t h = g.highest[v];
+        if (h == kInvalidLevel) continue;
+        if (h > g.top) {
+            snprintf(buf, sizeof buf, "vid %u has highest level %u > top level %u", v, h, g.top);
+            set_error(buf);
+            return ARTEA_EINVAL;
+        }
+        count[h]++;
+    }
+    std::vector<std::vector<uint8_t>> seen(g.top + 1);
+    for (uint32_t h = 0; h <= g.top; ++h) {
+        if (g.offsets[h].size() != count[h] * (h + 1) + 1) {
+            snprintf(buf, sizeof buf, "group %u offsets size does not match its %llu vertices", h,
+                     (unsigned long long)count[h]);
+            set_error(buf);
+            return ARTEA_EINVAL;
+        }
+        for (size_t r = 0; r + 1 < g.offsets[h].size(); ++r) {
+            if (g.offsets[h][r + 1] < g.offsets[h][r] || g.offsets[h][r + 1] > g.nbrs.size()) {
+                snprintf(buf, sizeof buf, "group %u offsets are not monotone within the neighbour array", h);
+                set_error(buf);
+                return ARTEA_EINVAL;
+            }
+        }
+        seen[h].assign(count[h], 0);
+    }
+    for (uint32_t v = 0; v < g.n; ++v) {
+        const uint32_t h = g.highest[v];
+        if (h == kInvalidLevel) continue;
+        if (g.local[v] >= count[h] || seen[h][g.local[v]]) {
+            snprintf(buf, sizeof buf, "vid %u local id %u is out of range or repeated in group %u", v, g.local[v], h);
+            set_error(buf);
+            return ARTEA_EINVAL;
+        }
+        seen[h][g.local[v]] = 1;
+    }
+    for (uint32_t x : g.nbrs) {
+        if (x >= g.n) {
+            snprintf(buf, sizeof buf, "invalid neighbor %u", x);
+            set_error(buf);
+            return ARTEA_EINVAL;
+        }
+    }
+    if (g.entry != kNoVertex && g.entry >= g.n) {
+        set_error("entry point vid is out of range");
+        return ARTEA_EINVAL;
+    }
+    return ARTEA_OK;
+}
+
+}  // namespace artea

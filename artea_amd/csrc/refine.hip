@@ -1,0 +1,882 @@
+// Propagate engine on MI355X (refiner/propagate_engine.hpp:38-325 and its
+// updaters), plus the full-hierarchy build that drives it.
+//
+// State: one row of up to `cap` Neighbor records (vid:31|new:1, FP32
+// distance) per local vertex, sorted by (distance, vid).  Every `next` is a
+// BSP step exactly like PropagateEngine::next: a producer kernel updates rows
+// in place and emits log records (target row, vid, distance); the logs are
+// then bucketed per target row (count, exclusive scan, scatter) and merged
+// by NbrLogTable::apply_logs's rule (nbr_log_table.hpp:127-182): sort by
+// (distance, vid), row entries before equal logs, adjacent-unique by vid.
+// The merge keeps the first `cap` entries, which is the reference's merge
+// followed by the arc (truncate) updater that refine_layer always runs next
+// (index/artea_graph/index_factory.hpp:385-393); rows therefore never exceed
+// the layer's max_nbr_size on the device.  Log order inside a bucket does
+// not matter (the merge sorts), so the result is deterministic.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "device.hpp"
+
+using namespace artea;
+using namespace artea::dev;
+
+namespace {
+
+constexpr uint32_t kNew = 0x80000000u;
+constexpr uint32_t kMask = 0x7FFFFFFFu;
+constexpr uint32_t kMaxCap = 128;      // triangle / merge LDS sizing
+constexpr uint32_t kSortN = 512;       // merge sort width
+
+struct Ctx {
+    const float* vecs;
+    uint32_t n_global, D, C;
+    int metric;
+    uint32_t n, cap;
+    const uint32_t* l2g;  // null = identity
+    const uint32_t* g2l;  // null = identity; 0xFFFFFFFF = not in layer
+    uint2* rows;          // n * cap
+    uint32_t* cnt;        // n
+    uint32_t* e_t;        // n * cap emission targets
+    uint2* e_vd;          // n * cap emission (vid, dist bits)
+    uint32_t* e_cnt;      // n
+    uint32_t* t_cnt;      // n + 1
+    uint32_t* t_off;      // n + 1
+    uint32_t* t_fill;     // n
+    uint2* seg;           // n * cap
+    unsigned long long* stats;  // [0] logs, [1] pair distances
+};
+
+__device__ __forceinline__ uint32_t storage_of(const Ctx& c, uint32_t local) { return c.l2g ? c.l2g[local] : local; }
+__device__ __forceinline__ uint32_t local_of(const Ctx& c, uint32_t g) {
+    return c.g2l ? c.g2l[g] : (g < c.n ? g : 0xFFFFFFFFu);
+}
+
+// The same counter-based draw as the oracle: SplitMix64 finaliser chained
+// over (seed, stream, row, j), mapped to [lo, hi) by multiply-high.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint32_t rand_draw(uint64_t seed, uint64_t stream, uint32_t row, uint32_t j, uint32_t lo,
+                                              uint32_t hi) {
+    const uint64_t x = mix64(mix64(mix64(seed ^ mix64(stream)) ^ uint64_t(row)) + uint64_t(j));
+    return lo + uint32_t((((x >> 32) * uint64_t(hi - lo)) >> 32));
+}
+
+__device__ __forceinline__ void emit(const Ctx& c, uint32_t src, uint32_t slot, uint32_t target, uint32_t vid,
+                                     float d) {
+    c.e_t[size_t(src) * c.cap + slot] = target;
+    c.e_vd[size_t(src) * c.cap + slot] = make_uint2(vid, __float_as_uint(d));
+    atomicAdd(c.t_cnt + target, 1u);
+}
+
+// Single-thread dist_func_t (all 16 lanes of the AVX-512 chain in registers).
+__device__ __forceinline__ float thread_distance(const float* __restrict__ a, const float* __restrict__ b, int C,
+                                                 int metric) {
+    float acc[16], nb[16], na[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) acc[l] = nb[l] = na[l] = 0.f;
+    for (int ch = 0; ch < C; ++ch) {
+#pragma unroll
+        for (int l4 = 0; l4 < 4; ++l4) {
+            const float4 x = *reinterpret_cast<const float4*>(a + ch * 16 + l4 * 4);
+            const float4 y = *reinterpret_cast<const float4*>(b + ch * 16 + l4 * 4);
+            if (metric == kL2Sqr || metric == kL2) {
+                const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
+                acc[l4 * 4 + 0] = __builtin_fmaf(d0, d0, acc[l4 * 4 + 0]);
+                acc[l4 * 4 + 1] = __builtin_fmaf(d1, d1, acc[l4 * 4 + 1]);
+                acc[l4 * 4 + 2] = __builtin_fmaf(d2, d2, acc[l4 * 4 + 2]);
+                acc[l4 * 4 + 3] = __builtin_fmaf(d3, d3, acc[l4 * 4 + 3]);
+            } else {
+                acc[l4 * 4 + 0] = __builtin_fmaf(x.x, y.x, acc[l4 * 4 + 0]);
+                acc[l4 * 4 + 1] = __builtin_fmaf(x.y, y.y, acc[l4 * 4 + 1]);
+                acc[l4 * 4 + 2] = __builtin_fmaf(x.z, y.z, acc[l4 * 4 + 2]);
+                acc[l4 * 4 + 3] = __builtin_fmaf(x.w, y.w, acc[l4 * 4 + 3]);
+                if (metric == kCosine) {
+                    na[l4 * 4 + 0] = __builtin_fmaf(x.x, x.x, na[l4 * 4 + 0]);
+                    na[l4 * 4 + 1] = __builtin_fmaf(x.y, x.y, na[l4 * 4 + 1]);
+                    na[l4 * 4 + 2] = __builtin_fmaf(x.z, x.z, na[l4 * 4 + 2]);
+                    na[l4 * 4 + 3] = __builtin_fmaf(x.w, x.w, na[l4 * 4 + 3]);
+                    nb[l4 * 4 + 0] = __builtin_fmaf(y.x, y.x, nb[l4 * 4 + 0]);
+                    nb[l4 * 4 + 1] = __builtin_fmaf(y.y, y.y, nb[l4 * 4 + 1]);
+                    nb[l4 * 4 + 2] = __builtin_fmaf(y.z, y.z, nb[l4 * 4 + 2]);
+                    nb[l4 * 4 + 3] = __builtin_fmaf(y.w, y.w, nb[l4 * 4 + 3]);
+                }
+            }
+        }
+    }
+    auto red = [](const float* v) {
+        float t8[8], t4[4];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t8[i] = v[i + 8] + v[i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t4[i] = t8[i + 4] + t8[i];
+        const float x0 = t4[0] + t4[2];
+        const float x1 = t4[1] + t4[3];
+        return x0 + x1;
+    };
+    const float s = red(acc);
+    if (metric == kL2Sqr) return s;
+    if (metric == kL2) return __builtin_sqrtf(s);
+    if (metric == kDot) return -s;
+    const float sq1 = red(na), sq2 = red(nb);
+    const float denom = __builtin_sqrtf(sq1 * sq2);
+    return denom > 0.0f ? 1.0f - s / denom : 1.0f;
+}
+
+// ---------------------------------------------------------------- prefill
+// RandomUpdater::update_impl (refiner/updaters/random_updater.hpp:97-142),
+// one wavefront per row of [row_begin, row_end).
+__global__ __launch_bounds__(256) void prefill_kernel(Ctx c, uint32_t gen, uint32_t lo, uint32_t hi, uint32_t rb,
+                                                      uint32_t re, uint64_t seed, uint64_t stream) {
+    __shared__ uint32_t s_cand[4][64];
+    __shared__ float s_d[4][64];
+    const uint32_t wid = threadIdx.x / 64, lane = lane_id();
+    for (uint32_t row = rb + blockIdx.x * 4 + wid; row < re; row += gridDim.x * 4) {
+        const uint32_t n0 = c.cnt[row];
+        if (lane == 0) c.e_cnt[row] = 0;
+        if (n0 >= gen) continue;
+        const uint32_t need = gen - n0;
+        const uint32_t sv = storage_of(c, row);
+        const float* pv = c.vecs + size_t(sv) * c.D;
+        const float qn2 = c.metric == kCosine ? quad_norm2(pv, int(c.C), lane & 3u) : 0.f;
+        uint32_t emitted = 0;
+        for (uint32_t j0 = 0; j0 < need; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            uint32_t rg = 0xFFFFFFFFu;
+            if (j < need) rg = storage_of(c, rand_draw(seed, stream, row, j, lo, hi));
+            const bool ok = j < need && rg != sv;
+            const uint64_t m = ballot(ok);
+            if (ok) s_cand[wid][lanes_below(m)] = rg;
+            wave_sync();
+            const uint32_t k = uint32_t(__popcll(m));
+            for (uint32_t g = 0; g < k; g += 16) {
+                const uint32_t r = g + (lane >> 2);
+                float d = 0.f;
+                if (r < k) d = quad_distance<0>(pv, c.vecs + size_t(s_cand[wid][r]) * c.D, int(c.C), c.metric,
+                                                lane & 3u, qn2);
+                if (r < k && (lane & 3u) == 0) s_d[wid][r] = d;
+            }
+            wave_sync();
+            // Row-full skip of the reference (never true below the gate, kept for fidelity).
+            bool keep = false;
+            float d = 0.f;
+            if (lane < k) {
+                d = s_d[wid][lane];
+                keep = !(n0 >= c.cap && __uint_as_float(c.rows[size_t(row) * c.cap + c.cap - 1].y) <= d);
+            }
+            const uint64_t km = ballot(keep);
+            if (keep) emit(c, row, emitted + lanes_below(km), row, s_cand[wid][lane], d);
+            emitted += uint32_t(__popcll(km));
+            wave_sync();
+        }
+        if (lane == 0) c.e_cnt[row] = emitted;
+    }
+}
+
+// ---------------------------------------------------------------- reverse
+// ReverseUpdater::update_impl (refiner/updaters/reverse_updater.hpp:72-100).
+__global__ void reverse_kernel(Ctx c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(c.n) * c.cap) return;
+    const uint32_t row = uint32_t(i / c.cap), k = uint32_t(i % c.cap);
+    const uint32_t nr = c.cnt[row];
+    if (k == 0) c.e_cnt[row] = nr;
+    if (k >= nr) return;
+    const uint2 e = c.rows[i];
+    uint32_t t = local_of(c, e.x & kMask);
+    if (t == 0xFFFFFFFFu) t = c.n;  // outside the layer: parked in a dead bucket, never merged
+    c.e_t[i] = t;
+    c.e_vd[i] = make_uint2(storage_of(c, row), e.y);
+    atomicAdd(c.t_cnt + t, 1u);
+}
+
+// --------------------------------------------------------------- triangle
+// TriangleUpdater::update_impl / _internal_check
+// (refiner/updaters/triangle_updater.hpp:114-157, 200-237), one 256-thread
+// block per row.  The row's vectors are staged in LDS (padded stride), all
+// needed pair distances (not old-old) are computed in parallel, then wave 0
+// replays the sequential retain/recommend decisions.
+struct TriLds {
+    uint32_t* vid;   // kMaxCap
+    float* dist;     // kMaxCap
+    uint32_t* isnew; // kMaxCap
+    uint32_t* R;     // kMaxCap retained positions
+    float* P;        // cap*(cap-1)/2
+    float* vec;      // cap * vstride (staged mode)
+};
+
+__device__ __forceinline__ uint32_t tri_index(uint32_t i, uint32_t j) { return i * (i - 1) / 2 + j; }  // i > j
+
+__global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, float shift_term, int staged,
+                                                       uint32_t vstride) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    TriLds s;
+    s.vid = reinterpret_cast<uint32_t*>(smem);
+    s.dist = reinterpret_cast<float*>(s.vid + kMaxCap);
+    s.isnew = reinterpret_cast<uint32_t*>(s.dist + kMaxCap);
+    s.R = s.isnew + kMaxCap;
+    s.P = reinterpret_cast<float*>(s.R + kMaxCap);
+    const uint32_t npairs_max = c.cap * (c.cap - 1) / 2;
+    s.vec = s.P + ((npairs_max + 3) / 4 * 4);
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
+    unsigned long long pairs_done = 0;
+    for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
+        const uint32_t n = c.cnt[row];
+        if (tid == 0) c.e_cnt[row] = 0;
+        if (n == 0) {
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t k = tid; k < n; k += 256) {
+            const uint2 e = c.rows[size_t(row) * c.cap + k];
+            s.vid[k] = e.x & kMask;
+            s.dist[k] = __uint_as_float(e.y);
+            s.isnew[k] = (e.x & kNew) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (staged) {
+            const uint32_t per = c.D / 4;  // float4 per vector
+            for (uint32_t t = tid; t < n * per; t += 256) {
+                const uint32_t k = t / per, q = t % per;
+                *reinterpret_cast<float4*>(s.vec + k * vstride + q * 4) =
+                    *reinterpret_cast<const float4*>(c.vecs + size_t(s.vid[k]) * c.D + q * 4);
+            }
+            __syncthreads();
+        }
+        const uint32_t np = n * (n - 1) / 2;
+        for (uint32_t p = tid; p < np; p += 256) {
+            uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
+            while (i * (i - 1) / 2 > p) --i;
+            while ((i + 1) * i / 2 <= p) ++i;
+            const uint32_t j = p - i * (i - 1) / 2;
+            if (!s.isnew[i] && !s.isnew[j]) continue;
+            const float* a = staged ? s.vec + i * vstride : c.vecs + size_t(s.vid[i]) * c.D;
+            const float* b = staged ? s.vec + j * vstride : c.vecs + size_t(s.vid[j]) * c.D;
+            s.P[p] = thread_distance(a, b, int(c.C), c.metric);
+            pairs_done++;
+        }
+        __syncthreads();
+        if (wid == 0) {
+            uint32_t nR = 1, ne = 0;
+            if (lane == 0) s.R[0] = 0;
+            wave_sync();
+            for (uint32_t i = 1; i < n; ++i) {
+                const bool cnew = s.isnew[i] != 0;
+                const float di = s.dist[i];
+                const float prune = di * inv_scale - shift_term;
+                float t0 = 0.f, t1 = 0.f;
+                bool v0 = false, v1 = false;
+                if (lane < nR) {
+                    const uint32_t j = s.R[lane];
+                    v0 = cnew || s.isnew[j];
+                    if (v0) t0 = s.P[tri_index(i, j)];
+                }
+                if (lane + 64 < nR) {
+                    const uint32_t j = s.R[lane + 64];
+                    v1 = cnew || s.isnew[j];
+                    if (v1) t1 = s.P[tri_index(i, j)];
+                }
+                const uint64_t b0 = ballot(v0 && t0 < prune), b1 = ballot(v1 && t1 < prune);
+                // Scan runs from the farthest retained entry; the first prune hit stops it.
+                const int jstar = b1 ? 64 + 63 - __clzll(b1) : (b0 ? 63 - __clzll(b0) : -1);
+                const bool r0 = v0 && t0 < di && int(lane) >= jstar;
+                const bool r1 = v1 && t1 < di && int(lane) + 64 >= jstar;
+                float bt = FLT_MAX;
+                int bk = -1;
+                if (r0 && t0 < bt) {
+                    bt = t0;
+                    bk = int(lane);
+                }
+                if (r1 && t1 <= bt) {  // later position wins equal distances
+                    bt = t1;
+                    bk = int(lane) + 64;
+                }
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const float ot = __shfl_xor(bt, off);
+                    const int ok = __shfl_xor(bk, off);
+                    if (ok >= 0 && (bk < 0 || ot < bt || (ot == bt && ok > bk))) {
+                        bt = ot;
+                        bk = ok;
+                    }
+                }
+                bk = __builtin_amdgcn_readfirstlane(bk);
+                bt = readlane_f(bt, 0);
+                if (bk >= 0) {
+                    const uint32_t tv = s.R[bk];
+                    uint32_t target = local_of(c, s.vid[tv]);
+                    if (lane == 0 && target != 0xFFFFFFFFu) emit(c, row, ne, target, s.vid[i], bt);
+                    if (target != 0xFFFFFFFFu) ne++;
+                }
+                if (jstar < 0) {
+                    if (lane == 0) s.R[nR] = i;
+                    nR++;
+                }
+                wave_sync();
+            }
+            // Retained entries, all marked old (mark_as_old before the swap).
+            for (uint32_t k = lane; k < nR; k += 64) {
+                const uint32_t pos = s.R[k];
+                c.rows[size_t(row) * c.cap + k] = make_uint2(s.vid[pos], __float_as_uint(s.dist[pos]));
+            }
+            if (lane == 0) {
+                c.cnt[row] = nR;
+                c.e_cnt[row] = ne;
+            }
+        }
+        __syncthreads();
+    }
+    // Per-block pair count.
+    __shared__ unsigned long long s_pairs;
+    if (tid == 0) s_pairs = 0;
+    __syncthreads();
+    atomicAdd(&s_pairs, pairs_done);
+    __syncthreads();
+    if (tid == 0) atomicAdd(c.stats + 1, s_pairs);
+}
+
+// ------------------------------------------------------------ log bucketing
+__global__ void scatter_kernel(Ctx c) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(c.n) * c.cap) return;
+    const uint32_t row = uint32_t(i / c.cap), k = uint32_t(i % c.cap);
+    if (k >= c.e_cnt[row]) return;
+    const uint32_t t = c.e_t[i];
+    if (t >= c.n) return;
+    const uint32_t pos = c.t_off[t] + atomicAdd(c.t_fill + t, 1u);
+    c.seg[pos] = c.e_vd[i];
+}
+
+// Orderable encoding of (distance, vid, origin): origin 0 = row entry.
+__device__ __forceinline__ unsigned long long sort_key(float d, uint32_t vid, uint32_t origin) {
+    uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (static_cast<unsigned long long>(u) << 32) | (static_cast<unsigned long long>(vid & kMask) << 1) | origin;
+}
+
+// apply_logs merge for rows that received logs; one 256-thread block per row.
+__global__ __launch_bounds__(256) void merge_kernel(Ctx c) {
+    __shared__ unsigned long long key[kSortN];
+    __shared__ uint32_t pay[kSortN];
+    __shared__ uint32_t s_n;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
+    for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
+        const uint32_t m = c.t_cnt[row];
+        if (m == 0) continue;  // uniform across the block
+        const uint32_t base = c.t_off[row];
+        const uint32_t nrow = c.cnt[row];
+        for (uint32_t k = tid; k < nrow; k += 256) {
+            const uint2 e = c.rows[size_t(row) * c.cap + k];
+            key[k] = sort_key(__uint_as_float(e.y), e.x, 0);
+            pay[k] = e.x;
+        }
+        if (tid == 0) s_n = nrow;
+        __syncthreads();
+        const uint32_t chunk = kSortN - c.cap;
+        for (uint32_t off = 0; off < m; off += chunk) {
+            const uint32_t cur = s_n;
+            const uint32_t nc = min(chunk, m - off);
+            for (uint32_t k = tid; k < kSortN - cur; k += 256) {
+                if (k < nc) {
+                    const uint2 lg = c.seg[base + off + k];
+                    key[cur + k] = sort_key(__uint_as_float(lg.y), lg.x, 1);
+                    pay[cur + k] = (lg.x & kMask) | kNew;
+                } else {
+                    key[cur + k] = ~0ull;
+                    pay[cur + k] = 0xFFFFFFFFu;
+                }
+            }
+            __syncthreads();
+            const uint32_t total = cur + nc;
+            for (uint32_t k2 = 2; k2 <= kSortN; k2 <<= 1) {
+                for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+                    const uint32_t t = tid;  // 256 threads = kSortN / 2 pairs
+                    const uint32_t i = 2 * j * (t / j) + (t % j);
+                    const uint32_t l = i + j;
+                    const bool up = (i & k2) == 0;
+                    const unsigned long long a = key[i], b = key[l];
+                    if ((a > b) == up) {
+                        key[i] = b;
+                        key[l] = a;
+                        const uint32_t pa = pay[i];
+                        pay[i] = pay[l];
+                        pay[l] = pa;
+                    }
+                    __syncthreads();
+                }
+            }
+            // Adjacent unique by vid, first `cap` survivors, compacted by wave 0.
+            if (wid == 0) {
+                uint32_t outn = 0;
+                uint32_t keep_v[kSortN / 64];
+                unsigned long long keep_k[kSortN / 64];
+                uint32_t dst[kSortN / 64];
+#pragma unroll
+                for (uint32_t b = 0; b < kSortN / 64; ++b) {
+                    const uint32_t i = b * 64 + lane;
+                    bool keep = false;
+                    if (i < total) {
+                        const uint32_t v = pay[i] & kMask;
+                        keep = (i == 0) || ((pay[i - 1] & kMask) != v);
+                    }
+                    const uint64_t km = ballot(keep);
+                    keep_v[b] = keep ? pay[i] : 0xFFFFFFFFu;
+                    keep_k[b] = keep ? key[i] : ~0ull;
+                    dst[b] = keep ? outn + lanes_below(km) : 0xFFFFFFFFu;
+                    outn += uint32_t(__popcll(km));
+                }
+                wave_sync();
+#pragma unroll
+                for (uint32_t b = 0; b < kSortN / 64; ++b) {
+                    if (dst[b] < c.cap) {
+                        key[dst[b]] = keep_k[b];
+                        pay[dst[b]] = keep_v[b];
+                    }
+                }
+                if (lane == 0) s_n = min(outn, c.cap);
+            }
+            __syncthreads();
+        }
+        const uint32_t fin = s_n;
+        for (uint32_t k = tid; k < fin; k += 256) {
+            const unsigned long long kk = key[k];
+            uint32_t u = uint32_t(kk >> 32);
+            u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+            c.rows[size_t(row) * c.cap + k] = make_uint2(pay[k], u);
+        }
+        if (tid == 0) c.cnt[row] = fin;
+        __syncthreads();
+    }
+}
+
+__global__ void count_logs_kernel(const uint32_t* t_off, uint32_t n, unsigned long long* stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) stats[0] += t_off[n];
+}
+
+}  // namespace
+
+struct artea_refine {
+    Ctx c{};
+    hipStream_t s = nullptr;
+    uint64_t seed = 0;
+    void* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+    uint32_t* d_l2g = nullptr;
+    uint32_t* d_g2l = nullptr;
+    int dev_cus = 0;
+};
+
+namespace {
+
+void refine_free(artea_refine* r) {
+    Ctx& c = r->c;
+    for (void* p : {(void*)c.rows, (void*)c.cnt, (void*)c.e_t, (void*)c.e_vd, (void*)c.e_cnt, (void*)c.t_cnt,
+                    (void*)c.t_off, (void*)c.t_fill, (void*)c.seg, (void*)c.stats, r->scan_tmp, (void*)r->d_l2g,
+                    (void*)r->d_g2l})
+        if (p) (void)hipFree(p);
+    if (r->s) (void)hipStreamDestroy(r->s);
+}
+
+uint32_t grid_for(const artea_refine* r, uint32_t rows_per_block, uint32_t blocks_per_cu) {
+    const uint32_t want = (r->c.n + rows_per_block - 1) / rows_per_block;
+    return std::max(1u, std::min(want, uint32_t(r->dev_cus) * blocks_per_cu));
+}
+
+// Bucket, merge and account the logs emitted by the last producer kernel.
+int merge_logs(artea_refine* r) {
+    Ctx& c = r->c;
+    hipStream_t s = r->s;
+    ARTEA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(r->scan_tmp, r->scan_bytes, c.t_cnt, c.t_off, c.n + 1, s));
+    const size_t slots = size_t(c.n) * c.cap;
+    const uint32_t blocks = uint32_t((slots + 255) / 256);
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 1, 8)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(count_logs_kernel, dim3(1), dim3(64), 0, s, c.t_off, c.n, c.stats);
+    ARTEA_HIP_TRY(hipGetLastError());
+    ARTEA_HIP_TRY(hipMemsetAsync(c.t_cnt, 0, sizeof(uint32_t) * (c.n + 1), s));
+    ARTEA_HIP_TRY(hipMemsetAsync(c.t_fill, 0, sizeof(uint32_t) * c.n, s));
+    return ARTEA_OK;
+}
+
+int do_prefill(artea_refine* r, uint32_t gen, uint32_t lo, uint32_t hi, uint32_t rb, uint32_t re, uint64_t stream) {
+    Ctx& c = r->c;
+    ARTEA_CHECK_ARG(lo <= hi && hi <= c.n, "prefill: id window must lie in [0, num_rows]");
+    ARTEA_CHECK_ARG(gen <= c.cap, "prefill: gen_size exceeds the row capacity");
+    re = std::min(re, c.n);
+    if (rb >= re || gen == 0 || lo == hi) return ARTEA_OK;
+    ARTEA_HIP_TRY(hipMemsetAsync(c.e_cnt, 0, sizeof(uint32_t) * c.n, r->s));
+    hipLaunchKernelGGL(prefill_kernel, dim3(grid_for(r, 4, 8)), dim3(256), 0, r->s, c, gen, lo, hi, rb, re, r->seed,
+                       stream);
+    ARTEA_HIP_TRY(hipGetLastError());
+    return merge_logs(r);
+}
+
+size_t tri_lds(const Ctx& c, bool staged, uint32_t vstride) {
+    const size_t np = size_t(c.cap) * (c.cap - 1) / 2;
+    size_t b = kMaxCap * 16 + (np + 3) / 4 * 4 * 4;
+    if (staged) b += size_t(c.cap) * vstride * 4;
+    return b;
+}
+
+int do_triangle(artea_refine* r, float scale, float shift) {
+    Ctx& c = r->c;
+    const float inv = 1.0f / scale;
+    const float shift_term = (1.0f + inv) * shift;
+    const uint32_t vstride = c.D + 4;
+    int dev = 0, max_lds = 65536;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    bool staged = tri_lds(c, true, vstride) <= size_t(max_lds);
+    const size_t lds = tri_lds(c, staged, vstride);
+    ARTEA_CHECK_ARG(lds <= size_t(max_lds), "triangle: row capacity too large for LDS");
+    if (lds > 65536)
+        ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(triangle_kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(triangle_kernel, dim3(grid_for(r, 1, 2)), dim3(256), lds, r->s, c, inv, shift_term,
+                       staged ? 1 : 0, vstride);
+    ARTEA_HIP_TRY(hipGetLastError());
+    return merge_logs(r);
+}
+
+int do_reverse(artea_refine* r) {
+    Ctx& c = r->c;
+    const size_t slots = size_t(c.n) * c.cap;
+    hipLaunchKernelGGL(reverse_kernel, dim3(uint32_t((slots + 255) / 256)), dim3(256), 0, r->s, c);
+    ARTEA_HIP_TRY(hipGetLastError());
+    return merge_logs(r);
+}
+
+int do_schedule(artea_refine* r, uint32_t loops, uint32_t triu, float scale, float shift) {
+    for (uint32_t b = 0; b < loops; ++b) {
+        for (uint32_t t = 0; t < triu; ++t) {
+            int rc = do_triangle(r, scale, shift);
+            if (rc) return rc;
+            // arc (truncate): rows are already capped at `cap` by the merge.
+        }
+        if (b + 1 < loops) {
+            int rc = do_reverse(r);
+            if (rc) return rc;
+        }
+    }
+    return ARTEA_OK;
+}
+
+int refine_make(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int metric, uint32_t cap,
+                const uint32_t* l2g, uint32_t n_local, uint64_t seed, artea_refine** out) {
+    ARTEA_CHECK_ARG(out, "artea_refine_create: null output");
+    ARTEA_CHECK_ARG(d_vecs || n_global == 0, "artea_refine_create: null vectors");
+    ARTEA_CHECK_ARG(dim_pad > 0 && dim_pad % 16 == 0, "artea_refine_create: dim_pad must be a multiple of 16");
+    ARTEA_CHECK_ARG(metric >= 0 && metric <= 3, "artea_refine_create: metric must be 0..3");
+    ARTEA_CHECK_ARG(cap >= 1 && cap <= kMaxCap, "artea_refine_create: cap must be in [1, 128]");
+    auto r = std::make_unique<artea_refine>();
+    Ctx& c = r->c;
+    c.vecs = d_vecs;
+    c.n_global = n_global;
+    c.D = dim_pad;
+    c.C = dim_pad / 16;
+    c.metric = metric;
+    c.cap = cap;
+    c.n = l2g ? n_local : n_global;
+    r->seed = seed;
+    const size_t slots = std::max<size_t>(size_t(c.n) * cap, 1);
+    auto fail = [&](hipError_t e) {
+        refine_free(r.get());
+        r->c = Ctx{};
+        set_error(std::string("artea_refine_create: allocation failed: ") + hipGetErrorString(e));
+        return ARTEA_ENOMEM;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&r->s, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.rows, slots * sizeof(uint2))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.cnt, (c.n + 1) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.e_t, slots * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.e_vd, slots * sizeof(uint2))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.e_cnt, (c.n + 1) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.t_cnt, (c.n + 2) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.t_off, (c.n + 2) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.t_fill, (c.n + 1) * 4)) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.seg, slots * sizeof(uint2))) != hipSuccess) return fail(e);
+    if ((e = hipMalloc(&c.stats, 4 * sizeof(unsigned long long))) != hipSuccess) return fail(e);
+    if (l2g) {
+        std::vector<uint32_t> g2l(std::max<uint32_t>(n_global, 1), 0xFFFFFFFFu);
+        for (uint32_t i = 0; i < n_local; ++i) {
+            if (l2g[i] >= n_global) {
+                refine_free(r.get());
+                set_error("artea_refine_create: local_to_global id out of range");
+                return ARTEA_EINVAL;
+            }
+            g2l[l2g[i]] = i;
+        }
+        if ((e = hipMalloc(&r->d_l2g, std::max<size_t>(n_local, 1) * 4)) != hipSuccess) return fail(e);
+        if ((e = hipMalloc(&r->d_g2l, g2l.size() * 4)) != hipSuccess) return fail(e);
+        if ((e = hipMemcpy(r->d_l2g, l2g, size_t(n_local) * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+        if ((e = hipMemcpy(r->d_g2l, g2l.data(), g2l.size() * 4, hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+        c.l2g = r->d_l2g;
+        c.g2l = r->d_g2l;
+    }
+    (void)hipMemset(c.cnt, 0, (c.n + 1) * 4);
+    (void)hipMemset(c.e_cnt, 0, (c.n + 1) * 4);
+    (void)hipMemset(c.t_cnt, 0, (c.n + 2) * 4);
+    (void)hipMemset(c.t_fill, 0, (c.n + 1) * 4);
+    (void)hipMemset(c.stats, 0, 4 * sizeof(unsigned long long));
+    if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, r->scan_bytes, c.t_cnt, c.t_off, c.n + 1)) != hipSuccess)
+        return fail(e);
+    if ((e = hipMalloc(&r->scan_tmp, std::max<size_t>(r->scan_bytes, 16))) != hipSuccess) return fail(e);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&r->dev_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (r->dev_cus <= 0) r->dev_cus = 256;
+    (void)hipDeviceSynchronize();
+    *out = r.release();
+    return ARTEA_OK;
+}
+
+// FP64 centroid partial sums in a fixed order (deterministic), then argmin
+// of dist(centroid, v) with ties to the smaller vid.
+__global__ void centroid_partial_kernel(const float* vecs, uint32_t n, uint32_t D, uint32_t per_block, double* part) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t lo = b * per_block, hi = min(n, lo + per_block);
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
+        double s = 0;
+        for (uint32_t v = lo; v < hi; ++v) s += double(vecs[size_t(v) * D + d]);
+        part[size_t(b) * D + d] = s;
+    }
+}
+__global__ void centroid_final_kernel(const double* part, uint32_t blocks, uint32_t n, uint32_t D, float* out) {
+    for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
+        double s = 0;
+        for (uint32_t b = 0; b < blocks; ++b) s += part[size_t(b) * D + d];
+        out[d] = float(s * (1.0 / double(n)));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int artea_refine_create(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int metric, uint32_t cap,
+                        const uint32_t* local_to_global, uint32_t n_local, uint64_t seed, artea_refine** out) {
+    clear_error();
+    return refine_make(d_vecs, n_global, dim_pad, metric, cap, local_to_global, n_local, seed, out);
+}
+
+int artea_refine_destroy(artea_refine* r) {
+    if (!r) return ARTEA_OK;
+    refine_free(r);
+    delete r;
+    return ARTEA_OK;
+}
+
+int artea_refine_upload_rows(artea_refine* r, const uint32_t* counts, const uint32_t* entries) {
+    clear_error();
+    ARTEA_CHECK_ARG(r && counts, "artea_refine_upload_rows: null argument");
+    Ctx& c = r->c;
+    std::vector<uint2> dense(std::max<size_t>(size_t(c.n) * c.cap, 1), make_uint2(0xFFFFFFFFu, 0));
+    std::vector<uint32_t> cnt(c.n);
+    size_t pos = 0;
+    for (uint32_t i = 0; i < c.n; ++i) {
+        ARTEA_CHECK_ARG(counts[i] <= c.cap, "artea_refine_upload_rows: row longer than cap");
+        cnt[i] = counts[i];
+        for (uint32_t k = 0; k < counts[i]; ++k, ++pos)
+            dense[size_t(i) * c.cap + k] = make_uint2(entries[2 * pos], entries[2 * pos + 1]);
+    }
+    ARTEA_HIP_TRY(hipMemcpyAsync(c.rows, dense.data(), dense.size() * sizeof(uint2), hipMemcpyHostToDevice, r->s));
+    ARTEA_HIP_TRY(hipMemcpyAsync(c.cnt, cnt.data(), size_t(c.n) * 4, hipMemcpyHostToDevice, r->s));
+    ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return ARTEA_OK;
+}
+
+int artea_refine_download_rows(artea_refine* r, uint32_t* counts, uint32_t* entries) {
+    clear_error();
+    ARTEA_CHECK_ARG(r && counts, "artea_refine_download_rows: null argument");
+    Ctx& c = r->c;
+    std::vector<uint2> dense(std::max<size_t>(size_t(c.n) * c.cap, 1));
+    ARTEA_HIP_TRY(hipMemcpyAsync(dense.data(), c.rows, size_t(c.n) * c.cap * sizeof(uint2), hipMemcpyDeviceToHost, r->s));
+    ARTEA_HIP_TRY(hipMemcpyAsync(counts, c.cnt, size_t(c.n) * 4, hipMemcpyDeviceToHost, r->s));
+    ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    if (entries) {
+        size_t pos = 0;
+        for (uint32_t i = 0; i < c.n; ++i)
+            for (uint32_t k = 0; k < counts[i]; ++k, ++pos) {
+                entries[2 * pos] = dense[size_t(i) * c.cap + k].x;
+                entries[2 * pos + 1] = dense[size_t(i) * c.cap + k].y;
+            }
+    }
+    return ARTEA_OK;
+}
+
+int artea_refine_prefill(artea_refine* r, uint32_t gen_size, uint32_t start, uint32_t end, uint32_t row_begin,
+                         uint32_t row_end, uint64_t stream) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_prefill: null handle");
+    int rc = do_prefill(r, gen_size, start, end, row_begin, row_end, stream);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_triangle(artea_refine* r, float scale, float shift) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_triangle: null handle");
+    int rc = do_triangle(r, scale, shift);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_truncate(artea_refine* r) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_truncate: null handle");
+    return ARTEA_OK;  // rows never exceed cap on the device
+}
+
+int artea_refine_reverse(artea_refine* r) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_reverse: null handle");
+    int rc = do_reverse(r);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_iters, float scale, float shift) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_schedule: null handle");
+    int rc = do_schedule(r, build_loops, triu_iters, scale, shift);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* pairs) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_last_stats: null handle");
+    unsigned long long h[4];
+    ARTEA_HIP_TRY(hipMemcpy(h, r->c.stats, sizeof h, hipMemcpyDeviceToHost));
+    if (logs) *logs = h[0];
+    if (pairs) *pairs = h[1];
+    ARTEA_HIP_TRY(hipMemset(r->c.stats, 0, sizeof h));
+    return ARTEA_OK;
+}
+
+int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
+                     uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
+                     float l0_min_distance, uint64_t seed, double* seconds_per_level) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx, "artea_build_full: null index");
+    ARTEA_CHECK_ARG(max_level < 64, "artea_build_full: max_level must be < 64");
+    ARTEA_CHECK_ARG(uint32_t(float(bl_max) * 1.5f) <= kMaxCap && uint32_t(float(ul_max) * 1.5f) <= kMaxCap,
+                    "artea_build_full: refine caps (1.5 x max) must be <= 128");
+    const uint32_t n = idx->n;
+    // Build metric: L2 for L2SQR search (build_infra_dispatch,
+    // framework/type_context/infra_dispatcher.hpp:224-230).
+    const int bmetric = idx->metric == ARTEA_L2SQR ? ARTEA_L2 : idx->metric;
+    HostCsr g;
+    g.n = n;
+    g.top = max_level;
+    g.ul_max = ul_max;
+    g.bl_max = bl_max;
+    g.highest.assign(n, max_level);
+    g.local.resize(n);
+    for (uint32_t v = 0; v < n; ++v) g.local[v] = v;
+    std::vector<std::vector<uint32_t>> cnts(max_level + 1), ids(max_level + 1);
+    for (uint32_t l = 0; l <= max_level; ++l) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const uint32_t slot_cap = l == 0 ? bl_max : ul_max;
+        const uint32_t cap = uint32_t(float(slot_cap) * 1.5f);
+        artea_refine* r = nullptr;
+        int rc = refine_make(idx->d_vecs, n, idx->dim_pad, bmetric, cap, nullptr, n, seed, &r);
+        if (rc) return rc;
+        std::unique_ptr<artea_refine, int (*)(artea_refine*)> guard(r, artea_refine_destroy);
+        const uint32_t gen = uint32_t(float(cap) * prefill_ratio);
+        if (gen > 0) {
+            rc = do_prefill(r, gen, 0, n, 0, n, l);
+            if (rc) return rc;
+        }
+        const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
+        rc = do_schedule(r, build_loops, triu_iters, scale_coeffs, shift);
+        if (rc) return rc;
+        // writeback_layer_from_single_layer_graph: first slot_cap entries.
+        std::vector<uint2> dense(std::max<size_t>(size_t(n) * cap, 1));
+        cnts[l].resize(n);
+        ARTEA_HIP_TRY(hipMemcpyAsync(dense.data(), r->c.rows, size_t(n) * cap * sizeof(uint2), hipMemcpyDeviceToHost,
+                                     r->s));
+        ARTEA_HIP_TRY(hipMemcpyAsync(cnts[l].data(), r->c.cnt, size_t(n) * 4, hipMemcpyDeviceToHost, r->s));
+        ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+        ids[l].clear();
+        ids[l].reserve(size_t(n) * slot_cap);
+        for (uint32_t v = 0; v < n; ++v) {
+            cnts[l][v] = std::min(cnts[l][v], slot_cap);
+            for (uint32_t k = 0; k < cnts[l][v]; ++k) ids[l].push_back(dense[size_t(v) * cap + k].x & kMask);
+        }
+        if (seconds_per_level)
+            seconds_per_level[l] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    // Grouped CSR: one group (h = max_level), rows ordered max_level..0.
+    g.offsets.assign(max_level + 1, std::vector<uint64_t>(1, 0));
+    auto& o = g.offsets[max_level];
+    o.assign(size_t(n) * (max_level + 1) + 1, 0);
+    std::vector<size_t> pos(max_level + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t l = max_level;; --l) {
+            o[size_t(v) * (max_level + 1) + (max_level - l)] = total;
+            const uint32_t c = cnts[l][v];
+            g.nbrs.insert(g.nbrs.end(), ids[l].begin() + pos[l], ids[l].begin() + pos[l] + c);
+            pos[l] += c;
+            total += c;
+            if (l == 0) break;
+        }
+    }
+    o.back() = total;
+    // Entry point: vertex closest to the FP64 centroid (build metric).
+    {
+        const uint32_t per = 4096, blocks = std::max(1u, (n + per - 1) / per);
+        double* part = nullptr;
+        float* cen = nullptr;
+        float* dists = nullptr;
+        ARTEA_HIP_TRY(hipMalloc(&part, size_t(blocks) * idx->dim_pad * 8));
+        ARTEA_HIP_TRY(hipMalloc(&cen, size_t(idx->dim_pad) * 4));
+        ARTEA_HIP_TRY(hipMalloc(&dists, std::max<size_t>(n, 1) * 4));
+        hipLaunchKernelGGL(centroid_partial_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->d_vecs, n,
+                           idx->dim_pad, per, part);
+        hipLaunchKernelGGL(centroid_final_kernel, dim3(1), dim3(256), 0, idx->stream, part, blocks, n, idx->dim_pad,
+                           cen);
+        ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
+        int rc = artea_distance_matrix_device(bmetric, cen, 1, idx->d_vecs, n, idx->dim_pad, dists, idx->stream);
+        if (rc) return rc;
+        std::vector<float> h(n);
+        ARTEA_HIP_TRY(hipMemcpyAsync(h.data(), dists, size_t(n) * 4, hipMemcpyDeviceToHost, idx->stream));
+        ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
+        float best = FLT_MAX;
+        uint32_t bv = kNoVertex;
+        for (uint32_t v = 0; v < n; ++v)
+            if (h[v] < best) {
+                best = h[v];
+                bv = v;
+            }
+        g.entry = bv;
+        (void)hipFree(part);
+        (void)hipFree(cen);
+        (void)hipFree(dists);
+    }
+    std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2));
+    for (uint32_t v = 0; v < n; ++v) {
+        vi[2 * size_t(v)] = g.highest[v];
+        vi[2 * size_t(v) + 1] = g.local[v];
+    }
+    std::vector<const uint64_t*> ptrs;
+    for (auto& x : g.offsets) ptrs.push_back(x.data());
+    return artea_index_set_graph_csr(idx, g.top, g.ul_max, g.bl_max, g.entry, vi.data(), ptrs.data(), g.nbrs.data(),
+                                     g.nbrs.size());
+}
+
+}  // extern "C"

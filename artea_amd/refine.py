@@ -1,0 +1,99 @@
+"""PropagateEngine on the GPU (refiner/propagate_engine.hpp) and the
+full-hierarchy build, through libartea_hip.so."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+
+from ._lib import check, lib
+
+NBR_NEW = 0x80000000
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class PropagateEngine:
+    """One single-layer graph plus the propagate engine over it.
+
+    ``rows`` are Neighbor records as a (count, 2) uint32 view: vid:31|new:1
+    and the FP32 distance bits.  Every ``next_*`` call is propagate followed
+    by merge_logs (propagate_engine.hpp:200-225)."""
+
+    def __init__(self, index, cap: int, metric: Optional[int] = None, local_to_global: Optional[Sequence[int]] = None,
+                 seed: int = 0):
+        m = index.metric if metric is None else metric
+        h = C.c_void_p()
+        l2g = None if local_to_global is None else np.ascontiguousarray(local_to_global, np.uint32)
+        self._l2g = l2g
+        self.n_local = index.n if l2g is None else len(l2g)
+        self.cap = cap
+        check(lib().artea_refine_create(C.c_void_p(index.device_vectors()), index.n, index.dim_pad, m, cap,
+                                        _ptr(l2g) if l2g is not None else None, self.n_local, seed, C.byref(h)))
+        self.h = h
+        self.index = index
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().artea_refine_destroy(self.h)
+            self.h = None
+
+    def upload(self, counts, entries):
+        c = np.ascontiguousarray(counts, np.uint32)
+        e = np.ascontiguousarray(entries).view(np.uint32).reshape(-1)
+        if e.size == 0:
+            e = np.zeros(2, np.uint32)
+        check(lib().artea_refine_upload_rows(self.h, _ptr(c), _ptr(e)))
+
+    def download(self):
+        c = np.zeros(self.n_local, np.uint32)
+        e = np.zeros(max(self.n_local * self.cap, 1) * 2, np.uint32)
+        check(lib().artea_refine_download_rows(self.h, _ptr(c), _ptr(e)))
+        total = int(c.sum())
+        return c, e[: 2 * total].reshape(-1, 2)
+
+    def next_prefill(self, gen_size, start, end, row_begin=0, row_end=None, stream=0):
+        check(lib().artea_refine_prefill(self.h, gen_size, start, end, row_begin,
+                                         self.n_local if row_end is None else row_end, stream))
+
+    def next_triangle(self, scale_coeffs, shifted_coeffs):
+        check(lib().artea_refine_triangle(self.h, scale_coeffs, shifted_coeffs))
+
+    def next_truncate(self):
+        check(lib().artea_refine_truncate(self.h))
+
+    def next_reverse(self):
+        check(lib().artea_refine_reverse(self.h))
+
+    def schedule(self, build_loops, triu_iters, scale_coeffs, effective_shift):
+        check(lib().artea_refine_schedule(self.h, build_loops, triu_iters, scale_coeffs, effective_shift))
+
+    def last_stats(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().artea_refine_last_stats(self.h, C.byref(a), C.byref(b)))
+        return {"logs": a.value, "pair_distances": b.value}
+
+
+def max_level_for(n: int) -> int:
+    """compute_max_allowed_level_id (index/stacked_rgraph/configs.hpp:241-249)."""
+    if n == 0:
+        return 1
+    ratio = n / 1000.0
+    if ratio <= 1.0:
+        return 1
+    return max(int(math.ceil(math.log(ratio))), 1)
+
+
+def build_full(index, ul_max=32, bl_max=64, max_level=None, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
+               shift=1.5, l0_min_distance=1.0, seed=0):
+    """Full-hierarchy build on the GPU; attaches the graph to ``index``.
+    Returns seconds per level."""
+    ml = max_level_for(index.n) if max_level is None else max_level
+    secs = np.zeros(ml + 1, np.float64)
+    check(lib().artea_build_full(index.h, ul_max, bl_max, ml, build_loops, triu_iters, prefill, scale, shift,
+                                 l0_min_distance, seed, _ptr(secs)))
+    return secs

@@ -1,0 +1,265 @@
+"""Benchmark: QPS at recall@100 >= 0.95 on 1,000,000 x 128 L2 (10,000 queries).
+
+Workload (one "step" = one pass of the router over the whole 10,000-query
+batch, inputs and index resident in HBM): the integer-rounded 256-component
+Gaussian mixture (seed 20261204), full-hierarchy index built on this GPU
+(build_loops 15, triu_iters 4, prefill 0.4, scale 1.10, shift 1.5,
+l0_min_distance 1), queries of mixed topk {10, 100} (seeded 50/50 draw).  The
+queue size sweep {100, 200, 400, 800} picks the smallest queue with
+recall@100 >= 0.95 against an FP64 brute-force ground truth; the timed steps
+run the batched persistent router at that queue.  The plain router (one
+launch per topk) is timed beside it, and the oracle (C/OpenMP restatement of
+the reference router) is timed on the host cores for the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def ground_truth(base_t, q_t, k, extra=32):
+    """FP64 brute-force top-k by (distance, vid) on the GPU (torch DGEMM)."""
+    import torch
+
+    n = base_t.shape[0]
+    nq = q_t.shape[0]
+    kk = k + extra
+    bn = (base_t.double() ** 2).sum(1)
+    best_d = torch.full((nq, kk), float("inf"), dtype=torch.float64, device="cuda")
+    best_i = torch.zeros((nq, kk), dtype=torch.int64, device="cuda")
+    qchunk, bchunk = 2000, 200000
+    for qs in range(0, nq, qchunk):
+        q = q_t[qs:qs + qchunk].double()
+        qn = (q ** 2).sum(1, keepdim=True)
+        cd, ci = best_d[qs:qs + qchunk], best_i[qs:qs + qchunk]
+        for bs in range(0, n, bchunk):
+            x = base_t[bs:bs + bchunk].double()
+            d = qn + bn[bs:bs + bchunk][None, :] - 2.0 * (q @ x.T)
+            td, ti = torch.topk(d, min(kk, d.shape[1]), dim=1, largest=False)
+            md = torch.cat([cd, td], 1)
+            mi = torch.cat([ci, ti + bs], 1)
+            sd, si = torch.topk(md, kk, dim=1, largest=False)
+            cd = sd
+            ci = torch.gather(mi, 1, si)
+        best_d[qs:qs + qchunk], best_i[qs:qs + qchunk] = cd, ci
+    d = best_d.cpu().numpy()
+    i = best_i.cpu().numpy()
+    out_i = np.empty((nq, k), np.int64)
+    out_d = np.empty((nq, k), np.float64)
+    for r in range(nq):
+        o = np.lexsort((i[r], d[r]))[:k]
+        out_i[r], out_d[r] = i[r, o], d[r, o]
+    return out_i, out_d
+
+
+def recall(pred, gt, k):
+    """RecallEstimator (utils/recall_estimator.hpp:53-150): |pred ∩ GT| / k."""
+    hits = 0
+    for p, g in zip(pred, gt):
+        hits += len(set(p[:k].tolist()) & set(g[:k].tolist()))
+    return hits / (len(pred) * k)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--queues", type=str, default="100,200,400,800")
+    ap.add_argument("--target-recall", type=float, default=0.95)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this path uses no collectives; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import artea_amd as aa
+    from artea_amd.refine import build_full
+
+    D, DIM = 128, 128
+    seed = 20261203 + 1
+    t0 = time.time()
+    base = aa.generate(0, seed, 0, args.n, DIM)
+    queries = aa.generate(0, seed, 1, args.nq, DIM)
+    log("generated %d x %d base, %d queries in %.1fs" % (args.n, D, args.nq, time.time() - t0))
+    rng = np.random.default_rng(seed)
+    topk_q = rng.choice(np.array([10, 100], np.uint32), args.nq)
+
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
+                      l0_min_distance=1.0, seed=seed)
+    build_s = time.time() - t0
+    info = idx.info()
+    log("GPU build %.1fs (levels %s), top %d entry %d" % (build_s, np.round(secs, 2).tolist(), info["top"],
+                                                          info["entry"]))
+
+    base_t = torch.from_numpy(base).cuda()
+    q_t = torch.from_numpy(queries).cuda()
+    t0 = time.time()
+    gt_i, gt_d = ground_truth(base_t, q_t, 100)
+    log("FP64 ground truth in %.1fs" % (time.time() - t0))
+    del base_t
+
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    nq = args.nq
+    tk_t = torch.from_numpy(topk_q.view(np.int32)).cuda()
+    ov = torch.empty((nq, 100), dtype=torch.int32, device="cuda")
+    od = torch.empty((nq, 100), dtype=torch.float32, device="cuda")
+    sel10 = np.where(topk_q == 10)[0]
+    sel100 = np.where(topk_q == 100)[0]
+    q10 = q_t[torch.from_numpy(sel10).cuda()].contiguous()
+    q100 = q_t[torch.from_numpy(sel100).cuda()].contiguous()
+    o10v = torch.empty((len(sel10), 10), dtype=torch.int32, device="cuda")
+    o10d = torch.empty((len(sel10), 10), dtype=torch.float32, device="cuda")
+    o100v = torch.empty((len(sel100), 100), dtype=torch.int32, device="cuda")
+    o100d = torch.empty((len(sel100), 100), dtype=torch.float32, device="cuda")
+
+    def time_it(fn, reps):
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps / 1e3
+
+    sweep = []
+    routers = {}
+    for L in [int(x) for x in args.queues.split(",")]:
+        queue_q = np.full(nq, L, np.uint32)
+        ql_t = torch.from_numpy(queue_q.view(np.int32)).cuda()
+        br = aa.BatchedRouter(idx, 100, L)
+        p10 = aa.HierarchicalGraphRouter(idx, None, 10, L)
+        p100 = aa.HierarchicalGraphRouter(idx, None, 100, L)
+
+        def run_batched(br=br, ql_t=ql_t):
+            br.query_device(q_t.data_ptr(), nq, tk_t.data_ptr(), ql_t.data_ptr(), ov.data_ptr(), od.data_ptr(),
+                            stream=sh)
+
+        def run_plain(p10=p10, p100=p100):
+            p10.batch_query_device(q10.data_ptr(), len(sel10), o10v.data_ptr(), o10d.data_ptr(), stream=sh)
+            p100.batch_query_device(q100.data_ptr(), len(sel100), o100v.data_ptr(), o100d.data_ptr(), stream=sh)
+
+        run_batched()
+        torch.cuda.synchronize()
+        tb = time_it(run_batched, 1)
+        st = br.last_stats()
+        pred = ov.cpu().numpy().view(np.uint32)
+        r100 = recall(pred[sel100], gt_i[sel100], 100)
+        r10 = recall(pred[sel10], gt_i[sel10], 10)
+        run_plain()
+        tp = time_it(run_plain, 1)
+        pv = o100v.cpu().numpy().view(np.uint32)
+        same = bool(np.array_equal(pv, pred[sel100]))
+        sweep.append({"queue": L, "recall@100": round(r100, 4), "recall@10": round(r10, 4),
+                      "batched_qps": round(nq / tb), "plain_qps": round(nq / tp), "ndc_per_query":
+                      round(st["ndc"] / nq, 1), "batched_equals_plain": same, "tail_overflows": st["tail_overflows"]})
+        log("queue %d: recall@100 %.4f recall@10 %.4f batched %.0f q/s plain %.0f q/s ndc %.0f same %s" % (
+            L, r100, r10, nq / tb, nq / tp, st["ndc"] / nq, same))
+        routers[L] = (br, p10, p100, run_batched, run_plain, st)
+    ok = [s for s in sweep if s["recall@100"] >= args.target_recall]
+    chosen = ok[0]["queue"] if ok else sweep[-1]["queue"]
+    br, p10, p100, run_batched, run_plain, st = routers[chosen]
+
+    for _ in range(args.warmup):
+        run_batched()
+    torch.cuda.synchronize()
+    t_step = time_it(run_batched, args.steps)
+    st = br.last_stats()
+    for _ in range(args.warmup):
+        run_plain()
+    t_plain = time_it(run_plain, args.steps)
+    qps = nq / t_step
+    qps_plain = nq / t_plain
+    log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
+
+    # Algorithmic bytes of one launch: per query the query row and the result
+    # rows, per distance evaluation one base row (4*D bytes), per beam
+    # expansion one neighbour row (4 bytes per stored neighbour id).
+    row_bytes = 4 * 64
+    alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
+    achieved = alg_bytes / t_step / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "c1.graph")
+            idx.snapshot_hgra(p)
+            g = orc.Graph.restore(p)
+        cores = len(os.sched_getaffinity(0))
+        sample = queries[sel100[:1000]]
+        t0 = time.time()
+        reps = 0
+        cv = None
+        while time.time() - t0 < args.cpu_seconds or reps == 0:
+            cv, cd, cndc = orc.batch_query(g, base, orc.L2SQR, sample, 100, chosen, nthreads=cores)
+            reps += 1
+        cpu_t = (time.time() - t0) / reps
+        # parity of the sample against the batched GPU router (top-100 queries)
+        gpu100 = ov.cpu().numpy().view(np.uint32)
+        match = float(np.mean(cv == gpu100[sel100[:1000], :100]))
+        cpu = {"value": round(sample.shape[0] / cpu_t, 1), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": "%d top-100 queries of the same batch at queue %d, %d repetitions (%.1fs), oracle "
+                         "restatement of HierarchicalGraphRouter::batch_query with OpenMP over queries" % (
+                             sample.shape[0], chosen, reps, time.time() - t0),
+               "slot_match_vs_gpu": round(match, 6), "ndc_per_query": round(float(cndc.mean()), 1)}
+        log("cpu baseline %.0f q/s on %d cores, GPU slot match %.6f" % (cpu["value"], cores, match))
+
+    out = {
+        "metric": "QPS @ recall@100>=0.95 (1M x 128 L2), batched vs plain GPU; p99 ms",
+        "value": round(qps, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 256-component integer Gaussian mixture; index built on the GPU)",
+        "config": {"workload": "batched search, 1,000,000 x 128 base, 10,000 queries, L2, topk mixed {10,100}",
+                   "n": args.n, "dim": D, "nq": nq, "queue": chosen, "parallelism": "single GPU"},
+        "plain_qps": round(qps_plain, 1),
+        "batched_over_plain": round(qps / qps_plain, 3),
+        "recall@100": next(s["recall@100"] for s in sweep if s["queue"] == chosen),
+        "build_seconds": round(build_s, 2),
+        "sweep": sweep,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+/*
+ * artea_hip.h -- C-ABI of the MI355X (gfx950) ARTEA search and refine path.
+ *
+ * One shared library, artea_amd/libartea_hip.so.  Every entry point is
+ * extern "C", takes plain pointers and sizes, returns an int status
+ * (ARTEA_OK = 0) and leaves a thread-local message in artea_last_error() on
+ * failure.  Handles are not thread-safe; each handle owns one HIP stream
+ * created with the default priority.  Reference citations are relative to
+ * include/artea/cpu/ of the reference checkout.
+ */
+#ifndef ARTEA_HIP_H
+#define ARTEA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum artea_status {
+    ARTEA_OK = 0,
+    ARTEA_EINVAL = 1,   /* std::runtime_error / std::invalid_argument in the reference */
+    ARTEA_EIO = 2,      /* HGRA snapshot/restore failures (ARTEA_ERROR -> runtime_error) */
+    ARTEA_EHIP = 3,     /* a HIP runtime call failed */
+    ARTEA_ENOMEM = 4,   /* device or host allocation failed */
+    ARTEA_EOVERFLOW = 5 /* std::overflow_error in the reference */
+};
+
+/* DistanceMetricsT (framework/type_traits/computer_traits.hpp:38-43). */
+enum artea_metric { ARTEA_L2SQR = 0, ARTEA_DOT = 1, ARTEA_COSINE = 2, ARTEA_L2 = 3 };
+
+/* Query flags: template switches of HierarchicalGraphRouter::batch_query
+ * (router/hierarchical_graph_router.hpp:293-319) and batch_query_l0_only
+ * (:353-385). */
+enum artea_query_flags { ARTEA_EARLY_STOP = 1u, ARTEA_L0_ONLY = 2u };
+
+#define ARTEA_INVALID_VID 0x7FFFFFFFu   /* CandidateEntry::make_invalid_entry vid */
+#define ARTEA_INVALID_LEVEL 0xFFFFFFFFu /* compact::HierarchicalGraph::invalid_level_id */
+
+/* Thread-local message of the last failing call (ARTEA_ERROR's text). */
+const char* artea_last_error(void);
+/* Library build id string (compiler, offload arch). */
+const char* artea_version(void);
+
+/* ------------------------------------------------------------------ index */
+typedef struct artea_index artea_index;
+
+/* Device copy of the base vectors: vector_array_t rows of dim_pad floats
+ * (containers/vector_array.hpp:26-506; dim_pad = the SIMD-padded dimension,
+ * a multiple of 16, containers/vector_dataset.hpp:242-269).  `vecs` is a
+ * host pointer; artea_index_create_device takes a device pointer instead
+ * (copied device-to-device).  metric is the search metric of dist_func_t
+ * (framework/type_context/default_context.hpp:147). */
+int artea_index_create(const float* vecs, uint32_t n, uint32_t dim_pad, int metric, artea_index** out);
+int artea_index_create_device(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric,
+                              artea_index** out);
+
+/* Attach a compact::hierarchical_graph_t given as its grouped CSR
+ * (index/compact_structure/hierarchical_graph.hpp:33-201): vertex_info[2v] =
+ * highest level (ARTEA_INVALID_LEVEL = unassigned), vertex_info[2v+1] =
+ * local id in its group; group_offsets[h] (h = 0..top) holds
+ * count_h*(h+1)+1 uint64 offsets into nbrs, row index local*(h+1)+(h-level)
+ * (:99-108).  The graph is converted once into the device layout.  n must
+ * equal the index's vector count. */
+int artea_index_set_graph_csr(artea_index* idx, uint32_t top, uint32_t ul_max, uint32_t bl_max,
+                              uint32_t entry_vid, const uint32_t* vertex_info,
+                              const uint64_t* const* group_offsets, const uint32_t* nbrs,
+                              uint64_t n_nbrs);
+/* HierarchicalGraphFileManager::restore (persistence/
+ * hierarchical_graph_file_manager.hpp:152-324) straight into the device
+ * layout, with the same validation and error texts. */
+int artea_index_set_graph_hgra(artea_index* idx, const char* path);
+/* HierarchicalGraphFileManager::snapshot (:88-145) of the attached graph. */
+int artea_index_snapshot_hgra(const artea_index* idx, const char* path);
+/* top_occupied_level_id / entry_point_vid / get_num_vertices (:83-132). */
+int artea_index_info(const artea_index* idx, uint32_t* n, uint32_t* dim_pad, uint32_t* top,
+                     uint32_t* entry_vid, uint32_t* ul_max, uint32_t* bl_max);
+/* Device pointer of the base vectors (n x dim_pad floats). */
+const float* artea_index_device_vectors(const artea_index* idx);
+int artea_index_destroy(artea_index* idx);
+
+/* ----------------------------------------------------------------- router */
+typedef struct artea_router artea_router;
+
+/* HierarchicalGraphRouter ctor (router/hierarchical_graph_router.hpp:93-109):
+ * fails with ARTEA_EINVAL "candidate_queue_size (Q) must be >= topk (K)". */
+int artea_router_create(artea_index* idx, uint32_t topk, uint32_t queue, artea_router** out);
+int artea_router_destroy(artea_router* r);
+
+/* batch_query / batch_query_l0_only (:293-385), synchronous, host buffers:
+ * queries nq x dim_pad floats; results row-major nq x topk, ascending by
+ * (distance, vid), padded with (ARTEA_INVALID_VID, FLT_MAX).  thresholds
+ * (read only with ARTEA_EARLY_STOP) holds one value per level, index =
+ * level, already in query distance units (squared for L2SQR), as
+ * _early_stop_thresholds produces them (:388-408). */
+int artea_router_batch_query(artea_router* r, const float* queries, uint32_t nq, uint32_t flags,
+                             const float* thresholds, uint32_t* out_vids, float* out_dists);
+
+/* Same, with device buffers, enqueued on `stream` (a hipStream_t; NULL =
+ * the router's own stream) and returning without a synchronisation. */
+int artea_router_batch_query_device(artea_router* r, const float* d_queries, uint32_t nq, uint32_t flags,
+                                    const float* thresholds, uint32_t* d_out_vids, float* d_out_dists,
+                                    void* stream);
+
+/* Counters of the last launch on this router (device-counted with the
+ * reference profiler's definitions, router/profiler/hg_router_profiler.hpp
+ * :153-197): distance evaluations (NDC, entry included), beam expansions,
+ * and tie-tail overflows (0 = the candidate list held every tied evictee). */
+int artea_router_last_stats(artea_router* r, uint64_t* ndc, uint64_t* expansions, uint64_t* tail_overflows);
+
+/* --------------------------------------------------------- batched router */
+/* Persistent fused router: queries of mixed topk / queue size are pulled
+ * from a device work queue.  topk and queue are per query (device arrays of
+ * nq uint32); results use row stride max_topk.  No reference counterpart:
+ * it is the batched form of batch_query for the same graph and dist_func_t,
+ * and returns for each query exactly what batch_query returns for it. */
+typedef struct artea_batched_router artea_batched_router;
+int artea_batched_router_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue,
+                                artea_batched_router** out);
+int artea_batched_router_destroy(artea_batched_router* r);
+int artea_batched_router_query_device(artea_batched_router* r, const float* d_queries, uint32_t nq,
+                                      const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
+                                      const float* thresholds, uint32_t* d_out_vids, float* d_out_dists,
+                                      void* stream);
+int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint64_t* expansions,
+                                    uint64_t* tail_overflows);
+
+/* ----------------------------------------------------------------- refine */
+/* PropagateEngine over one dynamic::SingleLayerGraph
+ * (refiner/propagate_engine.hpp:38-325, index/dynamic_structure/
+ * single_layer_graph.hpp:29-227).  Rows are indexed by local id; entries
+ * are Neighbor records (index/neighbor.hpp:43-67) packed as two uint32:
+ * vid:31 | new:1, then the FP32 distance bits.  cap = the layer's
+ * max_nbr_size (at most 128).  d_vecs is a device pointer (e.g.
+ * artea_index_device_vectors); local_to_global is a host array of n_local
+ * global ids, or NULL for the dense identity layer (n_local = n_global).
+ * seed keys the prefill's counter-based random stream. */
+typedef struct artea_refine artea_refine;
+int artea_refine_create(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int metric, uint32_t cap,
+                        const uint32_t* local_to_global, uint32_t n_local, uint64_t seed, artea_refine** out);
+int artea_refine_destroy(artea_refine* r);
+/* fill / export of rows: counts[n_local] and the rows' entries back to back
+ * (2 uint32 per entry).  Upload expects rows sorted by (distance, vid) and
+ * at most cap long, as fill_single_layer_graph_from_layer leaves them
+ * (refiner/refiner_utils.hpp:73-110). */
+int artea_refine_upload_rows(artea_refine* r, const uint32_t* counts, const uint32_t* entries);
+int artea_refine_download_rows(artea_refine* r, uint32_t* counts, uint32_t* entries);
+/* next_range(RandomUpdater(gen_size, start, end), row_begin, row_end)
+ * (refiner/updaters/random_updater.hpp:97-142); `stream` selects the
+ * random stream (the full build uses the level id). */
+int artea_refine_prefill(artea_refine* r, uint32_t gen_size, uint32_t start, uint32_t end, uint32_t row_begin,
+                         uint32_t row_end, uint64_t stream);
+/* next(TriangleUpdater(scale_coeffs, shifted_coeffs))
+ * (refiner/updaters/triangle_updater.hpp:114-237).  The merge of every
+ * next() keeps at most cap entries per row, which equals the reference's
+ * merge followed by the arc updater that refine_layer always runs next. */
+int artea_refine_triangle(artea_refine* r, float scale_coeffs, float shifted_coeffs);
+/* next(arc_updater_t) (refiner/updaters/truncate_updater.hpp:58-76). */
+int artea_refine_truncate(artea_refine* r);
+/* next(ReverseUpdater) (refiner/updaters/reverse_updater.hpp:72-100). */
+int artea_refine_reverse(artea_refine* r);
+/* refine_layer's loop (index/artea_graph/index_factory.hpp:385-393):
+ * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */
+int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_iters, float scale_coeffs,
+                          float effective_shift);
+/* Log records merged and pair distances evaluated since the last call. */
+int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* pair_distances);
+
+/* Full-hierarchy build on the GPU, attached to idx: every vertex on levels
+ * 0..max_level (the r-net hierarchy when no vertex is absorbed,
+ * index/stacked_rgraph/index_factory.hpp:364-381); each level refined by
+ * refine_layer with caps floor(1.5 x max), prefill floor(cap x prefill_ratio)
+ * from the level's random stream, shift = shifted_coeffs x l0_min_distance on
+ * L0 and 0 above; rows written back truncated to bl_max / ul_max and
+ * compacted with the FP64-centroid entry point
+ * (compactor/hierarchical_graph_compactor.hpp:118-327).  Build metric is L2
+ * for an L2SQR index.  seconds_per_level (optional) gets max_level+1 values. */
+int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
+                     uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
+                     float l0_min_distance, uint64_t seed, double* seconds_per_level);
+
+/* -------------------------------------------------------------- utilities */
+/* dist_func_t on the device (utils/simd_distance.hpp:64-79): dists[i*nb+j]
+ * = dist(a_i, b_j) for device arrays, bit-identical to the AVX-512 kernel's
+ * 16-lane FMA chain and reduce tree. */
+int artea_distance_matrix_device(int metric, const float* d_a, uint32_t na, const float* d_b, uint32_t nb,
+                                 uint32_t dim_pad, float* d_out, void* stream);
+
+/* Seeded synthetic workloads, host side, identical on every machine:
+ * kind 0 = integer-rounded 256-component Gaussian mixture (centres U[0,128],
+ * sigma 12, clamped to [0,255]); 1 = power-law-spectrum Gaussian on a random
+ * orthogonal basis, unit-normalised; 2 = 1024-component mixture (centres
+ * N(0,1), sigma 0.3), unit-normalised.  which = 0 base, 1 queries.  Writes
+ * n x dim_pad floats (zero tail beyond dim). */
+int artea_generate(int kind, uint64_t seed, uint32_t which, uint64_t first, uint64_t n, uint32_t dim,
+                   uint32_t dim_pad, float* out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

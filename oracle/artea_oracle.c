@@ -1234,6 +1234,58 @@ void orc_refine_schedule(orc_layer* L, uint32_t build_loops, uint32_t triu_iters
     }
 }
 
+/* compute_max_allowed_level_id (index/stacked_rgraph/configs.hpp:241-249) */
+uint32_t orc_max_level(uint32_t n) {
+    if (n == 0) return 1;
+    const double ratio = (double)n / 1000.0;
+    if (ratio <= 1.0) return 1;
+    const uint32_t c = (uint32_t)ceil(log(ratio));
+    return c > 1 ? c : 1;
+}
+
+orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t D, int build_metric, uint32_t ul_max,
+                          uint32_t bl_max, uint32_t max_level, uint32_t build_loops, uint32_t triu_iters,
+                          float prefill_ratio, float scale_coeffs, float shifted_coeffs, float l0_min_distance,
+                          uint64_t seed, int nthreads, double* secs) {
+    uint64_t** offs = (uint64_t**)calloc(max_level + 1, sizeof(uint64_t*));
+    uint32_t** ids = (uint32_t**)calloc(max_level + 1, sizeof(uint32_t*));
+    for (uint32_t l = 0; l <= max_level; ++l) {
+        const double t0 = omp_get_wtime();
+        const uint32_t slot_cap = l == 0 ? bl_max : ul_max;
+        /* IndexStructure refine caps: floor(1.5 * max) (artea_graph/index_structure.hpp:52-57) */
+        const uint32_t cap = (uint32_t)((float)slot_cap * 1.5f);
+        orc_layer* L = orc_layer_create(vecs, n, D, build_metric, cap, NULL, n);
+        const uint32_t g = (uint32_t)((float)cap * prefill_ratio);
+        if (g > 0) orc_next_prefill(L, g, 0, n, 0, n, seed, l, nthreads);
+        const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
+        orc_refine_schedule(L, build_loops, triu_iters, scale_coeffs, shift, nthreads);
+        offs[l] = (uint64_t*)malloc(sizeof(uint64_t) * ((uint64_t)n + 1));
+        offs[l][0] = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            const uint32_t c = L->rows[v].n < slot_cap ? L->rows[v].n : slot_cap;
+            offs[l][v + 1] = offs[l][v] + c;
+        }
+        ids[l] = (uint32_t*)malloc(sizeof(uint32_t) * (offs[l][n] ? offs[l][n] : 1));
+        for (uint32_t v = 0; v < n; ++v)
+            for (uint64_t i = 0; i < offs[l][v + 1] - offs[l][v]; ++i)
+                ids[l][offs[l][v] + i] = L->rows[v].a[i].vid_status & ORC_INVALID_VID;
+        orc_layer_free(L);
+        if (secs) secs[l] = omp_get_wtime() - t0;
+    }
+    uint32_t* highest = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t v = 0; v < n; ++v) highest[v] = max_level;
+    orc_graph* gr = orc_compact_graph(n, max_level, ul_max, bl_max, highest, (const uint64_t* const*)offs,
+                                      (const uint32_t* const*)ids, vecs, D, build_metric, NULL, 0);
+    for (uint32_t l = 0; l <= max_level; ++l) {
+        free(offs[l]);
+        free(ids[l]);
+    }
+    free(offs);
+    free(ids);
+    free(highest);
+    return gr;
+}
+
 /* ======================================================================
  * FP64 brute force (the recall oracle's ground truth).
  * ====================================================================== */

@@ -172,6 +172,26 @@ void orc_next_reverse(orc_layer* L, int nthreads);
  * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */
 void orc_refine_schedule(orc_layer* L, uint32_t build_loops, uint32_t triu_iters, float scale_coeffs,
                          float effective_shift, int nthreads);
+/* Full-hierarchy build, the degenerate r-net case of configs whose covering
+ * radii R_h = l0 * (1+tau_k) * beta^(h-1) stay below every nearest-neighbour
+ * distance (index/stacked_rgraph/index_factory.hpp:364-381 never absorbs a
+ * vertex, so every vertex reaches max_level = ceil(ln(N/1000)),
+ * index/stacked_rgraph/configs.hpp:241-249).  Each level l = 0..max_level is
+ * refined by refine_layer (index/artea_graph/index_factory.hpp:261-408) with
+ * refine caps floor(1.5*bl_max) / floor(1.5*ul_max), written back truncated
+ * to bl_max / ul_max (refiner/refiner_utils.hpp:122-151) and compacted.
+ * Every level starts from the random prefill of floor(cap * prefill) ids
+ * (the reference fills L0 this way; its upper levels start from r-net
+ * insertion edges, which this builder replaces by the same prefill).
+ * Level l draws stream l.  Returns the compact graph; build metric is L2 or
+ * COSINE/DOT as given (the search metric for L2 builds is L2SQR). */
+orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t dim_pad, int build_metric,
+                          uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
+                          uint32_t triu_iters, float prefill_ratio, float scale_coeffs,
+                          float shifted_coeffs, float l0_min_distance, uint64_t seed, int nthreads,
+                          double* seconds_per_level);
+uint32_t orc_max_level(uint32_t n);
+
 /* Counters of the last call (logs merged, pair distances evaluated). */
 uint64_t orc_layer_last_logs(const orc_layer* L);
 uint64_t orc_layer_pair_distances(const orc_layer* L);

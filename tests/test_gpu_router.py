@@ -1,0 +1,214 @@
+"""GPU router parity: libartea_hip.so's HIP routers against the CPU oracle.
+
+Bar: bit-exact ids and FP32 distances (the device distance kernel follows
+the reference's lane order and the queue emulation its try_push rule), on
+the reference's KATs, on oracle-built random indexes for every metric, and
+for the batched persistent router against the plain one.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import artea_amd as aa
+
+pytestmark = pytest.mark.gpu
+
+
+def gmm(n, dim=128, seed=1, which=0, kind=0):
+    return aa.generate(kind, seed, which, n, dim)
+
+
+def oracle_graph_to_csr(orc, g):
+    hl, lc, groups, nbrs = g.csr()
+    vi = np.stack([hl, lc], 1)
+    top = g.top if g.num_vertices else 0
+    return aa.CompactHierarchicalGraph(g.num_vertices, top, g.ul_max, g.bl_max, g.entry, vi, groups, nbrs)
+
+
+def device_index(orc, g, vecs, metric):
+    idx = aa.DeviceIndex(vecs, metric)
+    idx.set_graph(oracle_graph_to_csr(orc, g))
+    return idx
+
+
+def assert_same(gv, gd, ov, od, what=""):
+    assert gv.shape == ov.shape
+    bad = np.argwhere((gv != ov) | (gd != od))
+    assert bad.size == 0, "%s: %d slot mismatches, first %s: gpu (%s, %s) oracle (%s, %s)" % (
+        what, len(bad), bad[0], gv[tuple(bad[0])], gd[tuple(bad[0])], ov[tuple(bad[0])], od[tuple(bad[0])])
+
+
+def test_line_graph_matches_exact_sort_and_restore(orc, tmp_path):
+    # test_compact_csr_query.cpp:20-87 on the GPU.
+    n = 80
+    vecs = np.zeros((n, 128), np.float32)
+    vecs[:, 0] = np.arange(n)
+    highest = [1 if v < 40 else 0 for v in range(n)]
+    g = orc.Graph.compact(highest, [[list(range(n))] * n, [list(range(40)) if v < 40 else [] for v in range(n)]],
+                          vecs, orc.L2SQR, n, n)
+    idx = device_index(orc, g, vecs, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    router = aa.HierarchicalGraphRouter(idx, aa.DistFunc(0, 128), 10, n)
+    qs = np.zeros((5, 128), np.float32)
+    qs[:, 0] = [-2.0, 12.25, 39.5, 60.75, 90.0]
+    v, d = router.batch_query(qs)
+    for i in range(5):
+        dist = orc.distance_matrix(orc.L2SQR, qs[i:i + 1], vecs)[0]
+        order = sorted(range(n), key=lambda u: (dist[u], u))[:10]
+        assert list(v[i]) == order and np.array_equal(d[i], dist[order])
+    # restored from the oracle's HGRA file into the device index
+    path = str(tmp_path / "line.graph")
+    g.snapshot(path)
+    idx2 = aa.DeviceIndex(vecs, 0)
+    idx2.load_hgra(path)
+    v2, d2 = aa.HierarchicalGraphRouter(idx2, None, 10, n).batch_query(qs)
+    assert_same(v2, d2, v, d, "restored")
+    # device snapshot is byte-identical to the oracle's
+    p2 = str(tmp_path / "line2.graph")
+    idx2.snapshot_hgra(p2)
+    assert open(p2, "rb").read() == open(path, "rb").read()
+
+
+def early_stop_index(orc, metric):
+    from test_oracle_kats import early_stop_graph
+
+    g, vecs = early_stop_graph(orc)
+    return g, vecs, device_index(orc, g, vecs, metric)
+
+
+@pytest.mark.parametrize("metric,thr", [(3, [0.0, 8.0, 16.0]), (0, [0.0, 64.0, 256.0])])
+def test_early_stop_distance_counts(orc, metric, thr):
+    # test_router_early_stop.cpp:122-137: 7 calls plain, 5 with early stop.
+    g, vecs, idx = early_stop_index(orc, metric)
+    r = aa.HierarchicalGraphRouter(idx, None, 1, 4)
+    v, d = r.batch_query(vecs[3:4])
+    assert v[0, 0] == 3 and r.last_stats()["ndc"] == 7
+    v, d = r.batch_query(vecs[3:4], enable_early_stop=True, thresholds=thr)
+    assert v[0, 0] == 3 and d[0, 0] == 0 and r.last_stats()["ndc"] == 5
+    v, _ = r.batch_query(vecs)
+    assert list(v[:, 0]) == [0, 1, 2, 3]
+
+
+def test_threshold_overflow_and_queue_errors(orc):
+    g, vecs, idx = early_stop_index(orc, 0)
+    r = aa.HierarchicalGraphRouter(idx, None, 1, 4)
+    with pytest.raises(OverflowError):
+        r.batch_query(vecs[3:4], enable_early_stop=True, thresholds=[0, 0.0, 16.0])
+    with pytest.raises(aa.ArteaError, match="must be >= topk"):
+        aa.HierarchicalGraphRouter(idx, None, 5, 4)
+
+
+def test_empty_graph_returns_invalid(orc):
+    idx = aa.DeviceIndex(np.zeros((0, 16), np.float32), 0)
+    idx.set_graph(aa.CompactHierarchicalGraph(0, 0, 4, 4, 0xFFFFFFFF, np.zeros((0, 2), np.uint32), [np.zeros(1, np.uint64)],
+                                              np.zeros(0, np.uint32)))
+    r = aa.HierarchicalGraphRouter(idx, None, 2, 4)
+    v, d = r.batch_query(np.zeros((3, 16), np.float32))
+    assert np.all(v == aa.INVALID_VID) and np.all(d == np.finfo(np.float32).max)
+
+
+@pytest.fixture(scope="module")
+def small_l2(orc):
+    vecs = gmm(4000, seed=3)
+    g, _ = orc.build_full(vecs, metric=orc.L2, seed=5, nthreads=8)
+    qs = gmm(300, seed=3, which=1)
+    return g, vecs, qs
+
+
+@pytest.mark.parametrize("topk,queue", [(1, 1), (10, 10), (10, 64), (100, 100), (100, 200), (50, 400), (100, 800)])
+def test_random_index_parity_l2sqr(orc, small_l2, topk, queue):
+    g, vecs, qs = small_l2
+    idx = device_index(orc, g, vecs, 0)
+    r = aa.HierarchicalGraphRouter(idx, None, topk, queue)
+    gv, gd = r.batch_query(qs)
+    ov, od, ndc = orc.batch_query(g, vecs, orc.L2SQR, qs, topk, queue)
+    assert_same(gv, gd, ov, od, "topk=%d queue=%d" % (topk, queue))
+    st = r.last_stats()
+    assert st["tail_overflows"] == 0
+    assert st["ndc"] >= ndc.sum()  # the lossy visited cache can only add work
+
+
+def test_random_index_parity_l0_only_and_early_stop(orc, small_l2):
+    g, vecs, qs = small_l2
+    idx = device_index(orc, g, vecs, 0)
+    r = aa.HierarchicalGraphRouter(idx, None, 20, 40)
+    gv, gd = r.batch_query_l0_only(qs)
+    ov, od, _ = orc.batch_query(g, vecs, orc.L2SQR, qs, 20, 40, flags=orc.L0_ONLY)
+    assert_same(gv, gd, ov, od, "l0_only")
+    thr = np.array([0] + [40000.0 * (h + 1) for h in range(g.top)], np.float32)
+    gv, gd = r.batch_query(qs, enable_early_stop=True, thresholds=thr)
+    ov, od, _ = orc.batch_query(g, vecs, orc.L2SQR, qs, 20, 40, flags=orc.EARLY_STOP, thresholds=thr)
+    assert_same(gv, gd, ov, od, "early_stop")
+
+
+@pytest.mark.parametrize("metric,kind,dim", [(2, 1, 100), (1, 1, 100), (2, 2, 96)])
+def test_random_index_parity_cosine_dot(orc, metric, kind, dim):
+    vecs = gmm(3000, dim=dim, seed=11, kind=kind)
+    g, _ = orc.build_full(vecs, metric=metric, seed=2, nthreads=8, shift=0.0)
+    qs = gmm(200, dim=dim, seed=11, which=1, kind=kind)
+    idx = device_index(orc, g, vecs, metric)
+    for topk, queue in ((10, 32), (100, 160)):
+        r = aa.HierarchicalGraphRouter(idx, None, topk, queue)
+        gv, gd = r.batch_query(qs)
+        ov, od, _ = orc.batch_query(g, vecs, metric, qs, topk, queue)
+        assert_same(gv, gd, ov, od, "metric=%d topk=%d" % (metric, topk))
+
+
+def test_distance_matrix_bit_exact(orc):
+    import torch
+
+    for metric, kind, dim in ((0, 0, 128), (3, 0, 128), (1, 1, 100), (2, 1, 100), (2, 2, 96)):
+        a = gmm(37, dim=dim, seed=21, kind=kind)
+        b = gmm(53, dim=dim, seed=21, which=1, kind=kind)
+        ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+        out = torch.empty((37, 53), dtype=torch.float32, device="cuda")
+        aa._lib.check(aa._lib.lib().artea_distance_matrix_device(metric, ta.data_ptr(), 37, tb.data_ptr(), 53,
+                                                                 a.shape[1], out.data_ptr(), None))
+        torch.cuda.synchronize()
+        ref = orc.distance_matrix(metric, a, b)
+        assert np.array_equal(out.cpu().numpy(), ref), "metric %d" % metric
+
+
+def test_batched_router_mixed_work_matches_plain(orc, small_l2):
+    import torch
+
+    g, vecs, qs = small_l2
+    idx = device_index(orc, g, vecs, 0)
+    nq = qs.shape[0]
+    rng = np.random.default_rng(4)
+    topk = rng.choice([10, 100], nq).astype(np.uint32)
+    queue = np.maximum(topk, rng.choice([100, 200, 400], nq)).astype(np.uint32)
+    br = aa.BatchedRouter(idx, 100, 400)
+    tq = torch.from_numpy(qs).cuda()
+    tk = torch.from_numpy(topk.view(np.int32)).cuda()
+    tl = torch.from_numpy(queue.view(np.int32)).cuda()
+    ov = torch.empty((nq, 100), dtype=torch.int32, device="cuda")
+    od = torch.empty((nq, 100), dtype=torch.float32, device="cuda")
+    br.query_device(tq.data_ptr(), nq, tk.data_ptr(), tl.data_ptr(), ov.data_ptr(), od.data_ptr())
+    torch.cuda.synchronize()
+    bv = ov.cpu().numpy().view(np.uint32)
+    bd = od.cpu().numpy()
+    for k in (10, 100):
+        for L in (100, 200, 400):
+            sel = np.where((topk == k) & (queue == L))[0]
+            if sel.size == 0:
+                continue
+            pv, pd = aa.HierarchicalGraphRouter(idx, None, k, L).batch_query(qs[sel])
+            assert_same(bv[sel, :k], bd[sel, :k], pv, pd, "k=%d L=%d" % (k, L))
+    assert br.last_stats()["tail_overflows"] == 0
+
+
+def test_device_entry_point_matches_host_buffers(orc, small_l2):
+    import torch
+
+    g, vecs, qs = small_l2
+    idx = device_index(orc, g, vecs, 0)
+    r = aa.HierarchicalGraphRouter(idx, None, 10, 64)
+    hv, hd = r.batch_query(qs)
+    tq = torch.from_numpy(qs).cuda()
+    ov = torch.empty((qs.shape[0], 10), dtype=torch.int32, device="cuda")
+    od = torch.empty((qs.shape[0], 10), dtype=torch.float32, device="cuda")
+    r.batch_query_device(tq.data_ptr(), qs.shape[0], ov.data_ptr(), od.data_ptr(),
+                         stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert_same(ov.cpu().numpy().view(np.uint32), od.cpu().numpy(), hv, hd, "device api")
