@@ -19,6 +19,17 @@ PP = C.POINTER(C.c_void_p)
 SIGNATURES = {
     "artea_last_error": (C.c_char_p, []),
     "artea_version": (C.c_char_p, []),
+    "artea_graph_from_csr": (i32, [u32, u32, u32, u32, u32, P, P, P, u64, PP]),
+    "artea_graph_restore": (i32, [C.c_char_p, PP]),
+    "artea_graph_snapshot": (i32, [P, C.c_char_p]),
+    "artea_graph_info": (i32, [P, P, P, P, P, P, P]),
+    "artea_graph_vertex_info": (C.POINTER(u32), [P]),
+    "artea_graph_group_offsets": (C.POINTER(u64), [P, u32, P]),
+    "artea_graph_nbrs": (C.POINTER(u32), [P]),
+    "artea_graph_level_nbrs": (C.POINTER(u32), [P, u32, u32, P]),
+    "artea_graph_destroy": (i32, [P]),
+    "artea_index_set_graph": (i32, [P, P]),
+    "artea_index_get_graph": (i32, [P, PP]),
     "artea_index_create": (i32, [P, u32, u32, i32, PP]),
     "artea_index_create_device": (i32, [P, u32, u32, i32, PP]),
     "artea_index_set_graph_csr": (i32, [P, u32, u32, u32, u32, P, P, P, u64]),
@@ -38,6 +49,20 @@ SIGNATURES = {
     "artea_batched_router_last_stats": (i32, [P, P, P, P]),
     "artea_distance_matrix_device": (i32, [i32, P, u32, P, u32, u32, P, P]),
     "artea_generate": (i32, [i32, u64, u32, u64, u64, u32, u32, P, i32]),
+    "artea_bruteforce_knn_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P]),
+    "artea_device_alloc": (i32, [C.c_size_t, PP]),
+    "artea_device_free": (i32, [P]),
+    "artea_memcpy_htod": (i32, [P, P, C.c_size_t, P]),
+    "artea_memcpy_dtoh": (i32, [P, P, C.c_size_t, P]),
+    "artea_memset_device": (i32, [P, i32, C.c_size_t, P]),
+    "artea_stream_create": (i32, [PP]),
+    "artea_stream_destroy": (i32, [P]),
+    "artea_stream_synchronize": (i32, [P]),
+    "artea_event_create": (i32, [PP]),
+    "artea_event_destroy": (i32, [P]),
+    "artea_event_record": (i32, [P, P]),
+    "artea_event_elapsed_ms": (i32, [P, P, C.POINTER(f32)]),
+    "artea_device_info": (i32, [C.c_char_p, C.c_size_t, C.POINTER(i32), C.POINTER(C.c_size_t)]),
     "artea_refine_create": (i32, [P, u32, u32, i32, u32, P, u32, u64, PP]),
     "artea_refine_destroy": (i32, [P]),
     "artea_refine_upload_rows": (i32, [P, P, P]),

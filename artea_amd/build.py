@@ -22,7 +22,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ARTEA_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["index.hip", "router.hip", "refine.hip"]
+HIP_SOURCES = ["index.hip", "router.hip", "refine.hip", "util.hip"]
 CXX_SOURCES = ["hgra.cpp", "datagen.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]

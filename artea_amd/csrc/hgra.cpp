@@ -250,3 +250,115 @@ int csr_validate(const HostCsr& g) {
 }
 
 }  // namespace artea
+
+// ------------------------------------------------------------ C-ABI: graph
+using artea::HostCsr;
+
+extern "C" {
+
+int artea_graph_from_csr(uint32_t n, uint32_t top, uint32_t ul_max, uint32_t bl_max, uint32_t entry,
+                         const uint32_t* vi, const uint64_t* const* go, const uint32_t* nbrs, uint64_t n_nbrs,
+                         artea_graph** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(out, "artea_graph_from_csr: null output");
+    ARTEA_CHECK_ARG(top < 64, "artea_graph_from_csr: top level must be < 64");
+    ARTEA_CHECK_ARG(n == 0 || (vi && go), "artea_graph_from_csr: null CSR arrays");
+    ARTEA_CHECK_ARG(n_nbrs == 0 || nbrs, "artea_graph_from_csr: null neighbour array");
+    HostCsr c;
+    c.n = n;
+    c.top = top;
+    c.ul_max = ul_max;
+    c.bl_max = bl_max;
+    c.entry = entry;
+    c.highest.resize(n);
+    c.local.resize(n);
+    std::vector<uint64_t> count(top + 1, 0);
+    for (uint32_t v = 0; v < n; ++v) {
+        c.highest[v] = vi[2 * size_t(v)];
+        c.local[v] = vi[2 * size_t(v) + 1];
+        if (c.highest[v] != artea::kInvalidLevel && c.highest[v] <= top) count[c.highest[v]]++;
+    }
+    c.offsets.resize(top + 1);
+    for (uint32_t h = 0; h <= top; ++h) {
+        if (n == 0) {
+            c.offsets[h].assign(1, 0);
+            continue;
+        }
+        ARTEA_CHECK_ARG(go[h], "artea_graph_from_csr: null group offsets");
+        c.offsets[h].assign(go[h], go[h] + count[h] * (h + 1) + 1);
+    }
+    if (n_nbrs) c.nbrs.assign(nbrs, nbrs + n_nbrs);
+    int rc = artea::csr_validate(c);
+    if (rc) return rc;
+    *out = new artea_graph{std::move(c)};
+    return ARTEA_OK;
+}
+
+int artea_graph_restore(const char* path, artea_graph** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(path && out, "artea_graph_restore: null argument");
+    HostCsr c;
+    int rc = artea::hgra_read(path, &c);
+    if (rc) return rc;
+    *out = new artea_graph{std::move(c)};
+    return ARTEA_OK;
+}
+
+int artea_graph_snapshot(const artea_graph* g, const char* path) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(g && path, "artea_graph_snapshot: null argument");
+    return artea::hgra_write(g->c, path);
+}
+
+int artea_graph_info(const artea_graph* g, uint32_t* n, uint32_t* top, uint32_t* ul, uint32_t* bl, uint32_t* entry,
+                     uint64_t* n_nbrs) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(g, "artea_graph_info: null graph");
+    if (n) *n = g->c.n;
+    if (top) *top = g->c.n == 0 ? artea::kInvalidLevel : g->c.top;
+    if (ul) *ul = g->c.ul_max;
+    if (bl) *bl = g->c.bl_max;
+    if (entry) *entry = g->c.entry;
+    if (n_nbrs) *n_nbrs = g->c.nbrs.size();
+    return ARTEA_OK;
+}
+
+const uint32_t* artea_graph_vertex_info(const artea_graph* g) {
+    if (!g) return nullptr;
+    static thread_local std::vector<uint32_t> scratch;  // interleaved view built on demand
+    scratch.resize(size_t(g->c.n) * 2);
+    for (uint32_t v = 0; v < g->c.n; ++v) {
+        scratch[2 * size_t(v)] = g->c.highest[v];
+        scratch[2 * size_t(v) + 1] = g->c.local[v];
+    }
+    return scratch.data();
+}
+
+const uint64_t* artea_graph_group_offsets(const artea_graph* g, uint32_t h, uint64_t* count) {
+    if (!g || h >= g->c.offsets.size()) {
+        if (count) *count = 0;
+        return nullptr;
+    }
+    if (count) *count = g->c.count(h);
+    return g->c.offsets[h].data();
+}
+
+const uint32_t* artea_graph_nbrs(const artea_graph* g) { return g ? g->c.nbrs.data() : nullptr; }
+
+const uint32_t* artea_graph_level_nbrs(const artea_graph* g, uint32_t vid, uint32_t level, uint32_t* count) {
+    if (!g || vid >= g->c.n) {
+        if (count) *count = 0;
+        return nullptr;
+    }
+    uint32_t c;
+    const uint32_t* r = g->c.row(vid, level, &c);
+    if (count) *count = c;
+    return r;
+}
+
+int artea_graph_destroy(artea_graph* g) {
+    delete g;
+    return ARTEA_OK;
+}
+
+}  // extern "C"

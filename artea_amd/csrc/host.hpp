@@ -53,3 +53,8 @@ int csr_validate(const HostCsr& g);                 // consistency of a caller-b
 inline uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
 
 }  // namespace artea
+
+// The C-ABI's host graph handle.
+struct artea_graph {
+    artea::HostCsr c;
+};

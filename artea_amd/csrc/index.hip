@@ -223,6 +223,29 @@ int artea_index_set_graph_hgra(artea_index* idx, const char* path) {
     return ARTEA_OK;
 }
 
+int artea_index_set_graph(artea_index* idx, const artea_graph* g) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx && g, "artea_index_set_graph: null argument");
+    if (g->c.n != idx->n) {
+        set_error("artea_index_set_graph: graph has " + std::to_string(g->c.n) + " vertices, index has " +
+                  std::to_string(idx->n) + " vectors");
+        return ARTEA_EINVAL;
+    }
+    HostCsr c = g->c;
+    int rc = upload_graph(idx, c);
+    if (rc) return rc;
+    idx->csr = std::move(c);
+    return ARTEA_OK;
+}
+
+int artea_index_get_graph(const artea_index* idx, artea_graph** out) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx && out, "artea_index_get_graph: null argument");
+    ARTEA_CHECK_ARG(idx->dg.attached, "artea_index_get_graph: no graph attached");
+    *out = new artea_graph{idx->csr};
+    return ARTEA_OK;
+}
+
 int artea_index_snapshot_hgra(const artea_index* idx, const char* path) {
     clear_error();
     ARTEA_CHECK_ARG(idx && path, "artea_index_snapshot_hgra: null argument");

@@ -32,41 +32,6 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def ground_truth(base_t, q_t, k, extra=32):
-    """FP64 brute-force top-k by (distance, vid) on the GPU (torch DGEMM)."""
-    import torch
-
-    n = base_t.shape[0]
-    nq = q_t.shape[0]
-    kk = k + extra
-    bn = (base_t.double() ** 2).sum(1)
-    best_d = torch.full((nq, kk), float("inf"), dtype=torch.float64, device="cuda")
-    best_i = torch.zeros((nq, kk), dtype=torch.int64, device="cuda")
-    qchunk, bchunk = 2000, 200000
-    for qs in range(0, nq, qchunk):
-        q = q_t[qs:qs + qchunk].double()
-        qn = (q ** 2).sum(1, keepdim=True)
-        cd, ci = best_d[qs:qs + qchunk], best_i[qs:qs + qchunk]
-        for bs in range(0, n, bchunk):
-            x = base_t[bs:bs + bchunk].double()
-            d = qn + bn[bs:bs + bchunk][None, :] - 2.0 * (q @ x.T)
-            td, ti = torch.topk(d, min(kk, d.shape[1]), dim=1, largest=False)
-            md = torch.cat([cd, td], 1)
-            mi = torch.cat([ci, ti + bs], 1)
-            sd, si = torch.topk(md, kk, dim=1, largest=False)
-            cd = sd
-            ci = torch.gather(mi, 1, si)
-        best_d[qs:qs + qchunk], best_i[qs:qs + qchunk] = cd, ci
-    d = best_d.cpu().numpy()
-    i = best_i.cpu().numpy()
-    out_i = np.empty((nq, k), np.int64)
-    out_d = np.empty((nq, k), np.float64)
-    for r in range(nq):
-        o = np.lexsort((i[r], d[r]))[:k]
-        out_i[r], out_d[r] = i[r, o], d[r, o]
-    return out_i, out_d
-
-
 def recall(pred, gt, k):
     """RecallEstimator (utils/recall_estimator.hpp:53-150): |pred ∩ GT| / k."""
     hits = 0
@@ -92,9 +57,8 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import artea_amd as aa
+    from artea_amd.device import DeviceBuffer, Event, Stream, bruteforce_knn, device_info
     from artea_amd.refine import build_full
 
     D, DIM = 128, 128
@@ -105,9 +69,10 @@ def main():
     log("generated %d x %d base, %d queries in %.1fs" % (args.n, D, args.nq, time.time() - t0))
     rng = np.random.default_rng(seed)
     topk_q = rng.choice(np.array([10, 100], np.uint32), args.nq)
+    dev = device_info()
+    log("device %s, %d CUs" % (dev["arch"], dev["compute_units"]))
 
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
-    torch.cuda.synchronize()
     t0 = time.time()
     secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
                       l0_min_distance=1.0, seed=seed)
@@ -116,66 +81,63 @@ def main():
     log("GPU build %.1fs (levels %s), top %d entry %d" % (build_s, np.round(secs, 2).tolist(), info["top"],
                                                           info["entry"]))
 
-    base_t = torch.from_numpy(base).cuda()
-    q_t = torch.from_numpy(queries).cuda()
-    t0 = time.time()
-    gt_i, gt_d = ground_truth(base_t, q_t, 100)
-    log("FP64 ground truth in %.1fs" % (time.time() - t0))
-    del base_t
-
-    stream = torch.cuda.Stream()
-    sh = stream.cuda_stream
+    stream = Stream()
     nq = args.nq
-    tk_t = torch.from_numpy(topk_q.view(np.int32)).cuda()
-    ov = torch.empty((nq, 100), dtype=torch.int32, device="cuda")
-    od = torch.empty((nq, 100), dtype=torch.float32, device="cuda")
+    q_d = DeviceBuffer.from_numpy(queries)
+    base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
+    t0 = time.time()
+    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, args.n, q_d, nq, D, 100, stream)
+    log("FP64 ground truth in %.1fs" % (time.time() - t0))
+
+    tk_d = DeviceBuffer.from_numpy(topk_q)
+    ov = DeviceBuffer.empty((nq, 100), np.uint32)
+    od = DeviceBuffer.empty((nq, 100), np.float32)
     sel10 = np.where(topk_q == 10)[0]
     sel100 = np.where(topk_q == 100)[0]
-    q10 = q_t[torch.from_numpy(sel10).cuda()].contiguous()
-    q100 = q_t[torch.from_numpy(sel100).cuda()].contiguous()
-    o10v = torch.empty((len(sel10), 10), dtype=torch.int32, device="cuda")
-    o10d = torch.empty((len(sel10), 10), dtype=torch.float32, device="cuda")
-    o100v = torch.empty((len(sel100), 100), dtype=torch.int32, device="cuda")
-    o100d = torch.empty((len(sel100), 100), dtype=torch.float32, device="cuda")
+    q10 = DeviceBuffer.from_numpy(queries[sel10])
+    q100 = DeviceBuffer.from_numpy(queries[sel100])
+    o10v = DeviceBuffer.empty((len(sel10), 10), np.uint32)
+    o10d = DeviceBuffer.empty((len(sel10), 10), np.float32)
+    o100v = DeviceBuffer.empty((len(sel100), 100), np.uint32)
+    o100d = DeviceBuffer.empty((len(sel100), 100), np.float32)
+    sh = stream.h
 
     def time_it(fn, reps):
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        ev0.record(stream)
+        e0, e1 = Event(), Event()
+        stream.synchronize()
+        e0.record(stream)
         for _ in range(reps):
             fn()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / reps / 1e3
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / reps / 1e3
 
     sweep = []
     routers = {}
     for L in [int(x) for x in args.queues.split(",")]:
-        queue_q = np.full(nq, L, np.uint32)
-        ql_t = torch.from_numpy(queue_q.view(np.int32)).cuda()
+        ql_d = DeviceBuffer.from_numpy(np.full(nq, L, np.uint32))
         br = aa.BatchedRouter(idx, 100, L)
         p10 = aa.HierarchicalGraphRouter(idx, None, 10, L)
         p100 = aa.HierarchicalGraphRouter(idx, None, 100, L)
 
-        def run_batched(br=br, ql_t=ql_t):
-            br.query_device(q_t.data_ptr(), nq, tk_t.data_ptr(), ql_t.data_ptr(), ov.data_ptr(), od.data_ptr(),
-                            stream=sh)
+        def run_batched(br=br, ql_d=ql_d):
+            br.query_device(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr, stream=sh)
 
         def run_plain(p10=p10, p100=p100):
-            p10.batch_query_device(q10.data_ptr(), len(sel10), o10v.data_ptr(), o10d.data_ptr(), stream=sh)
-            p100.batch_query_device(q100.data_ptr(), len(sel100), o100v.data_ptr(), o100d.data_ptr(), stream=sh)
+            p10.batch_query_device(q10.ptr, len(sel10), o10v.ptr, o10d.ptr, stream=sh)
+            p100.batch_query_device(q100.ptr, len(sel100), o100v.ptr, o100d.ptr, stream=sh)
 
         run_batched()
-        torch.cuda.synchronize()
+        stream.synchronize()
         tb = time_it(run_batched, 1)
         st = br.last_stats()
-        pred = ov.cpu().numpy().view(np.uint32)
+        pred = ov.numpy(stream)
         r100 = recall(pred[sel100], gt_i[sel100], 100)
         r10 = recall(pred[sel10], gt_i[sel10], 10)
         run_plain()
         tp = time_it(run_plain, 1)
-        pv = o100v.cpu().numpy().view(np.uint32)
-        same = bool(np.array_equal(pv, pred[sel100]))
+        pv = o100v.numpy(stream)
+        same = bool(np.array_equal(pv, pred[sel100]) and np.array_equal(o10v.numpy(stream), pred[sel10, :10]))
         sweep.append({"queue": L, "recall@100": round(r100, 4), "recall@10": round(r10, 4),
                       "batched_qps": round(nq / tb), "plain_qps": round(nq / tp), "ndc_per_query":
                       round(st["ndc"] / nq, 1), "batched_equals_plain": same, "tail_overflows": st["tail_overflows"]})
@@ -188,7 +150,6 @@ def main():
 
     for _ in range(args.warmup):
         run_batched()
-    torch.cuda.synchronize()
     t_step = time_it(run_batched, args.steps)
     st = br.last_stats()
     for _ in range(args.warmup):
@@ -197,6 +158,7 @@ def main():
     qps = nq / t_step
     qps_plain = nq / t_plain
     log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
+    pred = ov.numpy(stream)
 
     # Algorithmic bytes of one launch: per query the query row and the result
     # rows, per distance evaluation one base row (4*D bytes), per beam
@@ -225,8 +187,7 @@ def main():
             reps += 1
         cpu_t = (time.time() - t0) / reps
         # parity of the sample against the batched GPU router (top-100 queries)
-        gpu100 = ov.cpu().numpy().view(np.uint32)
-        match = float(np.mean(cv == gpu100[sel100[:1000], :100]))
+        match = float(np.mean(cv == pred[sel100[:1000], :100]))
         cpu = {"value": round(sample.shape[0] / cpu_t, 1), "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": "%d top-100 queries of the same batch at queue %d, %d repetitions (%.1fs), oracle "
                          "restatement of HierarchicalGraphRouter::batch_query with OpenMP over queries" % (

@@ -43,6 +43,34 @@ const char* artea_last_error(void);
 /* Library build id string (compiler, offload arch). */
 const char* artea_version(void);
 
+/* ------------------------------------------------------------------ graph */
+/* Host-side compact::hierarchical_graph_t (index/compact_structure/
+ * hierarchical_graph.hpp:33-201) and its HGRA v1 persistence
+ * (persistence/hierarchical_graph_file_manager.hpp:20-40).  vertex_info[2v]
+ * = highest level (ARTEA_INVALID_LEVEL = unassigned), vertex_info[2v+1] =
+ * local id in its group; group h holds count_h*(h+1)+1 uint64 offsets into
+ * the neighbour array, row index local*(h+1)+(h-level) (:99-108). */
+typedef struct artea_graph artea_graph;
+/* Builder path (the builder-only mutators, :135-162): copies the arrays. */
+int artea_graph_from_csr(uint32_t n, uint32_t top, uint32_t ul_max, uint32_t bl_max, uint32_t entry_vid,
+                         const uint32_t* vertex_info, const uint64_t* const* group_offsets, const uint32_t* nbrs,
+                         uint64_t n_nbrs, artea_graph** out);
+/* HierarchicalGraphFileManager::restore (:152-324) / snapshot (:88-145):
+ * same validation order and messages; ARTEA_EIO on failure. */
+int artea_graph_restore(const char* path, artea_graph** out);
+int artea_graph_snapshot(const artea_graph* g, const char* path);
+/* get_num_vertices / top_occupied_level_id / ul,bl_max_nbr_size /
+ * entry_point_vid (:83-132); top is ARTEA_INVALID_LEVEL for an empty graph. */
+int artea_graph_info(const artea_graph* g, uint32_t* n, uint32_t* top, uint32_t* ul_max, uint32_t* bl_max,
+                     uint32_t* entry_vid, uint64_t* n_nbrs);
+/* Views valid until the graph is destroyed. */
+const uint32_t* artea_graph_vertex_info(const artea_graph* g);
+const uint64_t* artea_graph_group_offsets(const artea_graph* g, uint32_t h, uint64_t* count);
+const uint32_t* artea_graph_nbrs(const artea_graph* g);
+/* fetch_level_nbrs (:99-108); count 0 when level > highest. */
+const uint32_t* artea_graph_level_nbrs(const artea_graph* g, uint32_t vid, uint32_t level, uint32_t* count);
+int artea_graph_destroy(artea_graph* g);
+
 /* ------------------------------------------------------------------ index */
 typedef struct artea_index artea_index;
 
@@ -67,10 +95,14 @@ int artea_index_set_graph_csr(artea_index* idx, uint32_t top, uint32_t ul_max, u
                               uint32_t entry_vid, const uint32_t* vertex_info,
                               const uint64_t* const* group_offsets, const uint32_t* nbrs,
                               uint64_t n_nbrs);
+/* Attach a host graph (copied; the graph may be destroyed afterwards). */
+int artea_index_set_graph(artea_index* idx, const artea_graph* g);
 /* HierarchicalGraphFileManager::restore (persistence/
  * hierarchical_graph_file_manager.hpp:152-324) straight into the device
  * layout, with the same validation and error texts. */
 int artea_index_set_graph_hgra(artea_index* idx, const char* path);
+/* Copy of the attached graph as a host graph (snapshot, export). */
+int artea_index_get_graph(const artea_index* idx, artea_graph** out);
 /* HierarchicalGraphFileManager::snapshot (:88-145) of the attached graph. */
 int artea_index_snapshot_hgra(const artea_index* idx, const char* path);
 /* top_occupied_level_id / entry_point_vid / get_num_vertices (:83-132). */
@@ -186,6 +218,31 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
  * 16-lane FMA chain and reduce tree. */
 int artea_distance_matrix_device(int metric, const float* d_a, uint32_t na, const float* d_b, uint32_t nb,
                                  uint32_t dim_pad, float* d_out, void* stream);
+
+/* FP64 exact brute-force k nearest neighbours, the recall oracle of the
+ * benchmarks (utils/recall_estimator.hpp:53-150 scores against such a
+ * ground truth): device buffers, distances accumulated in FP64 from the FP32
+ * inputs, ids ascending by (distance, vid).  k <= 256. */
+int artea_bruteforce_knn_device(int metric, const float* d_base, uint32_t n, const float* d_queries, uint32_t nq,
+                                uint32_t dim_pad, uint32_t k, uint32_t* d_out_vids, double* d_out_dists,
+                                void* stream);
+
+/* Device plumbing for callers without a GPU framework of their own: plain
+ * allocations, copies, streams (default priority) and timing events of the
+ * library's HIP runtime. */
+int artea_device_alloc(size_t bytes, void** d_ptr);
+int artea_device_free(void* d_ptr);
+int artea_memcpy_htod(void* d_dst, const void* src, size_t bytes, void* stream);
+int artea_memcpy_dtoh(void* dst, const void* d_src, size_t bytes, void* stream);
+int artea_memset_device(void* d_dst, int value, size_t bytes, void* stream);
+int artea_stream_create(void** stream);
+int artea_stream_destroy(void* stream);
+int artea_stream_synchronize(void* stream);
+int artea_event_create(void** event);
+int artea_event_destroy(void* event);
+int artea_event_record(void* event, void* stream);
+int artea_event_elapsed_ms(void* start, void* end, float* ms);
+int artea_device_info(char* name, size_t name_len, int* compute_units, size_t* total_bytes);
 
 /* Seeded synthetic workloads, host side, identical on every machine:
  * kind 0 = integer-rounded 256-component Gaussian mixture (centres U[0,128],

@@ -155,22 +155,38 @@ def test_random_index_parity_cosine_dot(orc, metric, kind, dim):
 
 
 def test_distance_matrix_bit_exact(orc):
-    import torch
+    from artea_amd.device import DeviceBuffer, Stream
 
+    s = Stream()
     for metric, kind, dim in ((0, 0, 128), (3, 0, 128), (1, 1, 100), (2, 1, 100), (2, 2, 96)):
         a = gmm(37, dim=dim, seed=21, kind=kind)
         b = gmm(53, dim=dim, seed=21, which=1, kind=kind)
-        ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
-        out = torch.empty((37, 53), dtype=torch.float32, device="cuda")
-        aa._lib.check(aa._lib.lib().artea_distance_matrix_device(metric, ta.data_ptr(), 37, tb.data_ptr(), 53,
-                                                                 a.shape[1], out.data_ptr(), None))
-        torch.cuda.synchronize()
+        da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+        out = DeviceBuffer.empty((37, 53), np.float32)
+        aa._lib.check(aa._lib.lib().artea_distance_matrix_device(metric, da.ptr, 37, db.ptr, 53, a.shape[1], out.ptr,
+                                                                 s.h))
+        s.synchronize()
         ref = orc.distance_matrix(metric, a, b)
-        assert np.array_equal(out.cpu().numpy(), ref), "metric %d" % metric
+        assert np.array_equal(out.numpy(), ref), "metric %d" % metric
+
+
+@pytest.mark.parametrize("metric,kind,dim", [(0, 0, 128), (2, 1, 100), (1, 2, 96)])
+def test_bruteforce_knn_matches_oracle(orc, metric, kind, dim):
+    from artea_amd.device import DeviceBuffer, bruteforce_knn
+
+    base = gmm(2000, dim=dim, seed=31, kind=kind)
+    qs = gmm(50, dim=dim, seed=31, which=1, kind=kind)
+    ids, d = bruteforce_knn(metric, DeviceBuffer.from_numpy(base), 2000, DeviceBuffer.from_numpy(qs), 50,
+                            base.shape[1], 100)
+    oi, od = orc.bruteforce_knn(metric, base, qs, 100)
+    if metric == 0:  # integer data: FP64 distances are exact, ties broken by vid
+        assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    else:
+        assert np.mean(ids == oi) > 0.999 and np.allclose(d, od, rtol=1e-12, atol=1e-12)
 
 
 def test_batched_router_mixed_work_matches_plain(orc, small_l2):
-    import torch
+    from artea_amd.device import DeviceBuffer
 
     g, vecs, qs = small_l2
     idx = device_index(orc, g, vecs, 0)
@@ -179,15 +195,16 @@ def test_batched_router_mixed_work_matches_plain(orc, small_l2):
     topk = rng.choice([10, 100], nq).astype(np.uint32)
     queue = np.maximum(topk, rng.choice([100, 200, 400], nq)).astype(np.uint32)
     br = aa.BatchedRouter(idx, 100, 400)
-    tq = torch.from_numpy(qs).cuda()
-    tk = torch.from_numpy(topk.view(np.int32)).cuda()
-    tl = torch.from_numpy(queue.view(np.int32)).cuda()
-    ov = torch.empty((nq, 100), dtype=torch.int32, device="cuda")
-    od = torch.empty((nq, 100), dtype=torch.float32, device="cuda")
-    br.query_device(tq.data_ptr(), nq, tk.data_ptr(), tl.data_ptr(), ov.data_ptr(), od.data_ptr())
-    torch.cuda.synchronize()
-    bv = ov.cpu().numpy().view(np.uint32)
-    bd = od.cpu().numpy()
+    tq, tk, tl = DeviceBuffer.from_numpy(qs), DeviceBuffer.from_numpy(topk), DeviceBuffer.from_numpy(queue)
+    ov = DeviceBuffer.empty((nq, 100), np.uint32)
+    od = DeviceBuffer.empty((nq, 100), np.float32)
+    from artea_amd.device import Stream
+
+    s = Stream()
+    br.query_device(tq.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr, stream=s.h)
+    s.synchronize()
+    bv = ov.numpy()
+    bd = od.numpy()
     for k in (10, 100):
         for L in (100, 200, 400):
             sel = np.where((topk == k) & (queue == L))[0]
@@ -199,16 +216,16 @@ def test_batched_router_mixed_work_matches_plain(orc, small_l2):
 
 
 def test_device_entry_point_matches_host_buffers(orc, small_l2):
-    import torch
+    from artea_amd.device import DeviceBuffer, Stream
 
     g, vecs, qs = small_l2
     idx = device_index(orc, g, vecs, 0)
     r = aa.HierarchicalGraphRouter(idx, None, 10, 64)
     hv, hd = r.batch_query(qs)
-    tq = torch.from_numpy(qs).cuda()
-    ov = torch.empty((qs.shape[0], 10), dtype=torch.int32, device="cuda")
-    od = torch.empty((qs.shape[0], 10), dtype=torch.float32, device="cuda")
-    r.batch_query_device(tq.data_ptr(), qs.shape[0], ov.data_ptr(), od.data_ptr(),
-                         stream=torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    assert_same(ov.cpu().numpy().view(np.uint32), od.cpu().numpy(), hv, hd, "device api")
+    s = Stream()
+    tq = DeviceBuffer.from_numpy(qs)
+    ov = DeviceBuffer.empty((qs.shape[0], 10), np.uint32)
+    od = DeviceBuffer.empty((qs.shape[0], 10), np.float32)
+    r.batch_query_device(tq.ptr, qs.shape[0], ov.ptr, od.ptr, stream=s.h)
+    s.synchronize()
+    assert_same(ov.numpy(), od.numpy(), hv, hd, "device api")
