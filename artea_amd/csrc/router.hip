@@ -44,7 +44,8 @@ namespace {
 constexpr uint32_t kExplored = 0x80000000u;
 constexpr uint32_t kVidMask = 0x7FFFFFFFu;
 constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kVisSlots = 2048;  // lossy visited cache entries per wave (8 KiB)
+constexpr uint32_t kVisLog = 12;
+constexpr uint32_t kVisSlots = 1u << kVisLog;  // lossy visited cache entries per wave (8 KiB)
 
 struct RouterArgs {
     const float* vecs;
@@ -67,26 +68,34 @@ struct RouterArgs {
     uint32_t* out_vids;
     float* out_dists;
     unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
+    uint32_t hmask;                               // 2^B - 1, B = max(12, ceil(log2 n))
     uint32_t* work_counter;                       // persistent mode when non-null
 };
 
-__device__ __forceinline__ uint32_t vis_slot(uint32_t v) { return (v * 0x9E3779B1u) >> (32 - 11); }
+// Visited-cache key: h = v * odd mod 2^B is a bijection of [0, 2^B), so the
+// pair (slot = low 12 bits of h, tag = 1 + the rest) identifies v exactly
+// and a 16-bit tag suffices for n <= 2^28 (0 marks an empty slot).
+__device__ __forceinline__ uint32_t vis_hash(uint32_t v, uint32_t hmask) { return (v * 0x9E3779B1u) & hmask; }
+__device__ __forceinline__ uint32_t vis_slot(uint32_t h) { return h & (kVisSlots - 1); }
+__device__ __forceinline__ uint16_t vis_tag(uint32_t h) { return uint16_t((h >> kVisLog) + 1); }
 
 struct WaveLds {
     float* q;         // D floats
-    uint32_t* vis;    // kVisSlots
+    uint16_t* vis;    // kVisSlots
     uint32_t* cand;   // 64
     float* cdist;     // 64
 };
 
 // Collect the unvisited, valid ids of a 64-id chunk in row order into
 // cand[0..cnt), inserting them into the visited cache; returns cnt.
-__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n) {
+__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask) {
     const bool valid = id < n;
-    const uint32_t slot = vis_slot(id);
-    const bool fresh = valid && w.vis[slot] != id;
+    const uint32_t h = vis_hash(id, hmask);
+    const uint32_t slot = vis_slot(h);
+    const uint16_t tag = vis_tag(h);
+    const bool fresh = valid && w.vis[slot] != tag;
     wave_sync();
-    if (fresh) w.vis[slot] = id;
+    if (fresh) w.vis[slot] = tag;
     const uint64_t m = ballot(fresh);
     if (fresh) w.cand[lanes_below(m)] = id;
     wave_sync();
@@ -221,11 +230,25 @@ struct CandList {
         }
         return false;
     }
+
+    // The entry the next pop would return if no insertion precedes it
+    // (first unexplored entry), or 0xFFFFFFFF.
+    __device__ uint32_t peek() const {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
+            if (b) return readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
+        }
+        return 0xFFFFFFFFu;
+    }
+};
+
+struct QueryStats {
+    uint32_t ndc, expansions, overflow;
 };
 
 template <int K, int CT>
-__device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, unsigned long long* ndc_acc,
-                          unsigned long long* exp_acc, unsigned long long* ovf_acc) {
+__device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi) {
     const uint32_t lane = lane_id();
     // Per-query work items are clamped to what this launch was sized for:
     // topk <= row stride, topk <= L <= the router's maximum queue.
@@ -237,7 +260,7 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
     // Query row into LDS.
     const float* qg = a.queries + size_t(qi) * a.D;
     for (uint32_t i = lane; i < a.D; i += 64) w.q[i] = qg[i];
-    for (uint32_t i = lane; i < kVisSlots; i += 64) w.vis[i] = 0xFFFFFFFFu;
+    for (uint32_t i = lane; i < kVisSlots / 2; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
     wave_sync();
     const float qn2 = a.metric == kCosine ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
 
@@ -245,12 +268,12 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
     uint32_t cur = a.entry;
     float cd = quad_distance<CT>(w.q, a.vecs + size_t(cur) * a.D, int(a.C), a.metric, lane & 3u, qn2);
     cd = readlane_f(cd, 0);
-    unsigned long long ndc = 1, expansions = 0;
+    uint32_t ndc = 1, expansions = 0;
 
     if (!(a.flags & ARTEA_L0_ONLY)) {
         for (uint32_t level = a.top; level >= 1; --level) {
             // Greedy walk on `level` (greedy_loop_body); seed marked visited.
-            if (lane == 0) w.vis[vis_slot(cur)] = cur;
+            if (lane == 0) w.vis[vis_slot(vis_hash(cur, a.hmask))] = vis_tag(vis_hash(cur, a.hmask));
             wave_sync();
             const bool early = (a.flags & ARTEA_EARLY_STOP) != 0;
             const float thr = early ? a.thresholds[level] : 0.f;
@@ -264,7 +287,7 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
                 for (uint32_t base = 0; base < a.strideU; base += 64) {
                     const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
                     if (ballot(id != 0xFFFFFFFFu) == 0) break;
-                    const uint32_t cnt = collect(w, id, a.n);
+                    const uint32_t cnt = collect(w, id, a.n, a.hmask);
                     if (cnt == 0) continue;
                     distances<CT>(a, w, cnt, qn2);
                     ndc += cnt;
@@ -293,7 +316,7 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
                 cd = best_d;
             }
             // visited.clear() after every level (:228-236).
-            for (uint32_t i = lane; i < kVisSlots; i += 64) w.vis[i] = 0xFFFFFFFFu;
+            for (uint32_t i = lane; i < kVisSlots / 2; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
             wave_sync();
         }
     }
@@ -302,16 +325,24 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
     CandList<K> W;
     W.init(L);
     W.insert(cd, cur);
-    if (lane == 0) w.vis[vis_slot(cur)] = cur;
+    if (lane == 0) w.vis[vis_slot(vis_hash(cur, a.hmask))] = vis_tag(vis_hash(cur, a.hmask));
     wave_sync();
     uint32_t pv;
+    // Speculative read of the first 64 ids of the row the next pop will most
+    // likely return, issued before this expansion's distance gathers so the
+    // next expansion usually starts without a dependent row load.
+    uint32_t pre_v = 0xFFFFFFFFu, pre_id = 0xFFFFFFFFu;
     while (W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
+        const bool hit = pv == pre_v;
+        const uint32_t first = hit ? pre_id : (lane < a.stride0 ? row[lane] : 0xFFFFFFFFu);
+        pre_v = W.peek();
+        pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
         for (uint32_t base = 0; base < a.stride0; base += 64) {
-            const uint32_t id = (base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu;
+            const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
             if (ballot(id != 0xFFFFFFFFu) == 0) break;
-            const uint32_t cnt = collect(w, id, a.n);
+            const uint32_t cnt = collect(w, id, a.n, a.hmask);
             if (cnt == 0) continue;
             distances<CT>(a, w, cnt, qn2);
             ndc += cnt;
@@ -346,21 +377,19 @@ __device__ void route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi, un
         ov[i] = kInvalidVid;
         od[i] = FLT_MAX;
     }
-    *ndc_acc += ndc;
-    *exp_acc += expansions;
-    *ovf_acc += W.overflow;
+    return QueryStats{ndc, expansions, W.overflow};
 }
 
 template <int K, int CT>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wid = threadIdx.x / 64;
-    const size_t per_wave = size_t(a.D) * 4 + kVisSlots * 4 + 512;
+    const size_t per_wave = size_t(a.D) * 4 + kVisSlots * 2 + 512;
     unsigned char* base = smem + wid * per_wave;
     WaveLds w;
     w.q = reinterpret_cast<float*>(base);
-    w.vis = reinterpret_cast<uint32_t*>(base + size_t(a.D) * 4);
-    w.cand = w.vis + kVisSlots;
+    w.vis = reinterpret_cast<uint16_t*>(base + size_t(a.D) * 4);
+    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisSlots);
     w.cdist = reinterpret_cast<float*>(w.cand + 64);
     unsigned long long ndc = 0, exps = 0, ovf = 0;
     if (a.work_counter) {
@@ -369,11 +398,19 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
             if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
             qi = readfirst_u(qi);
             if (qi >= a.nq) break;
-            route_one<K, CT>(a, w, qi, &ndc, &exps, &ovf);
+            const QueryStats st = route_one<K, CT>(a, w, qi);
+            ndc += st.ndc;
+            exps += st.expansions;
+            ovf += st.overflow;
         }
     } else {
         const uint32_t qi = blockIdx.x * kWavesPerBlock + wid;
-        if (qi < a.nq) route_one<K, CT>(a, w, qi, &ndc, &exps, &ovf);
+        if (qi < a.nq) {
+            const QueryStats st = route_one<K, CT>(a, w, qi);
+            ndc = st.ndc;
+            exps = st.expansions;
+            ovf = st.overflow;
+        }
     }
     if (lane_id() == 0 && a.stats) {
         atomicAdd(a.stats + 0, ndc);
@@ -420,7 +457,7 @@ int pick_kernel(uint32_t L, uint32_t C, KernelFn* fn, uint32_t* K) {
     return ARTEA_OK;
 }
 
-size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisSlots * 4 + 512); }
+size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisSlots * 2 + 512); }
 
 }  // namespace
 
@@ -514,6 +551,12 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     a.out_dists = d_od;
     a.stats = r->d_stats;
     a.work_counter = nullptr;
+    {
+        uint32_t B = kVisLog;
+        while (B < 32 && (uint64_t(1) << B) < idx->n) ++B;
+        ARTEA_CHECK_ARG(B <= 28, "router: more than 2^28 vectors");
+        a.hmask = uint32_t((uint64_t(1) << B) - 1);
+    }
     KernelFn fn;
     uint32_t K;
     int rc = pick_kernel(queue, a.C, &fn, &K);

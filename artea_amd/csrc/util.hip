@@ -126,9 +126,26 @@ __global__ __launch_bounds__(256) void bruteforce_kernel(int metric, const float
         }
 }
 
+// Plain vectorised copy (16 B per lane per step): the reference kernel that
+// calibrates HBM byte counters (a copy of B bytes moves exactly B in, B out).
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int artea_copy_device(void* d_dst, const void* d_src, size_t bytes, void* stream) {
+    clear_error();
+    ARTEA_CHECK_ARG(bytes % 16 == 0, "artea_copy_device: bytes must be a multiple of 16");
+    if (!bytes) return ARTEA_OK;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const uint4*>(d_src), static_cast<uint4*>(d_dst), bytes / 16);
+    ARTEA_HIP_TRY(hipGetLastError());
+    return ARTEA_OK;
+}
 
 int artea_bruteforce_knn_device(int metric, const float* d_base, uint32_t n, const float* d_queries, uint32_t nq,
                                 uint32_t D, uint32_t k, uint32_t* d_out_vids, double* d_out_dists, void* stream) {

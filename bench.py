@@ -51,6 +51,10 @@ def main():
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after timing, run one plain copy kernel of the base vectors (counter calibration)")
+    ap.add_argument("--graph-cache", type=str, default="",
+                    help="HGRA path: load the index graph from it if present, else build and save it there")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path uses no collectives; "
@@ -74,8 +78,15 @@ def main():
 
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
     t0 = time.time()
-    secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                      l0_min_distance=1.0, seed=seed)
+    if args.graph_cache and os.path.exists(args.graph_cache):
+        idx.load_hgra(args.graph_cache)
+        secs = np.zeros(0)
+        log("loaded the index graph from %s (build not timed in this run)" % args.graph_cache)
+    else:
+        secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
+                          shift=1.5, l0_min_distance=1.0, seed=seed)
+        if args.graph_cache:
+            idx.snapshot_hgra(args.graph_cache)
     build_s = time.time() - t0
     info = idx.info()
     log("GPU build %.1fs (levels %s), top %d entry %d" % (build_s, np.round(secs, 2).tolist(), info["top"],
@@ -159,6 +170,15 @@ def main():
     qps_plain = nq / t_plain
     log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
     pred = ov.numpy(stream)
+    if args.calibrate_copy:
+        # One plain copy of the base vectors (B bytes in, B bytes out): the
+        # counter calibration dispatch for the HBM traffic figure.
+        from artea_amd._lib import check, lib
+
+        dst = DeviceBuffer(base.nbytes)
+        check(lib().artea_copy_device(dst.ptr, idx.device_vectors(), base.nbytes, stream.h))
+        stream.synchronize()
+        log("calibration copy of %d bytes done" % base.nbytes)
 
     # Algorithmic bytes of one launch: per query the query row and the result
     # rows, per distance evaluation one base row (4*D bytes), per beam

@@ -235,6 +235,9 @@ int artea_device_free(void* d_ptr);
 int artea_memcpy_htod(void* d_dst, const void* src, size_t bytes, void* stream);
 int artea_memcpy_dtoh(void* dst, const void* d_src, size_t bytes, void* stream);
 int artea_memset_device(void* d_dst, int value, size_t bytes, void* stream);
+/* Device-to-device copy by a plain kernel (bytes % 16 == 0); also the
+ * calibration kernel for HBM byte counters. */
+int artea_copy_device(void* d_dst, const void* d_src, size_t bytes, void* stream);
 int artea_stream_create(void** stream);
 int artea_stream_destroy(void* stream);
 int artea_stream_synchronize(void* stream);
