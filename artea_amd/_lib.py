@@ -56,6 +56,9 @@ SIGNATURES = {
     "artea_memcpy_dtoh": (i32, [P, P, C.c_size_t, P]),
     "artea_memset_device": (i32, [P, i32, C.c_size_t, P]),
     "artea_copy_device": (i32, [P, P, C.c_size_t, P]),
+    "artea_event_query": (i32, [P, C.POINTER(i32)]),
+    "artea_host_alloc": (i32, [C.c_size_t, PP]),
+    "artea_host_free": (i32, [P]),
     "artea_stream_create": (i32, [PP]),
     "artea_stream_destroy": (i32, [P]),
     "artea_stream_synchronize": (i32, [P]),

@@ -134,7 +134,7 @@ struct CandList {
     float lb;     // distance of W[L-1] when n >= L, else +inf
     uint32_t overflow;
 
-    __device__ void init(uint32_t cap) {
+    __device__ __forceinline__ void init(uint32_t cap) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             d[k] = __int_as_float(0x7F800000);
@@ -157,7 +157,7 @@ struct CandList {
 
     // try_push with the reference's rule already decided by the caller;
     // inserts (dist, vid) unless the same key is present.
-    __device__ void insert(float dist, uint32_t vid) {
+    __device__ __forceinline__ void insert(float dist, uint32_t vid) {
         const uint32_t lane = lane_id();
         uint32_t p = 0;
         bool dup = false;
@@ -216,30 +216,39 @@ struct CandList {
     }
 
     // pop_best_unexplored_entry: first unexplored entry; marks it explored.
-    __device__ bool pop(uint32_t* vid) {
+    // Every register block is indexed with compile-time constants only
+    // (a data-dependent block index would move the list to scratch memory).
+    __device__ __forceinline__ bool pop(uint32_t* vid) {
         const uint32_t lane = lane_id();
+        int kk = -1, l = 0;
+        uint32_t found = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
-            if (b) {
-                const int l = __ffsll((unsigned long long)b) - 1;
-                *vid = readlane_u(v[k], l) & kVidMask;
-                if (int(lane) == l) v[k] |= kExplored;
-                return true;
+            if (kk < 0 && b) {
+                kk = k;
+                l = __ffsll((unsigned long long)b) - 1;
+                found = readlane_u(v[k], l);
             }
         }
-        return false;
+        if (kk < 0) return false;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k == kk && int(lane) == l) v[k] |= kExplored;
+        *vid = found & kVidMask;
+        return true;
     }
 
     // The entry the next pop would return if no insertion precedes it
     // (first unexplored entry), or 0xFFFFFFFF.
-    __device__ uint32_t peek() const {
+    __device__ __forceinline__ uint32_t peek() const {
+        uint32_t found = 0xFFFFFFFFu;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
-            if (b) return readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
+            if (found == 0xFFFFFFFFu && b) found = readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
         }
-        return 0xFFFFFFFFu;
+        return found;
     }
 };
 
@@ -380,8 +389,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     return QueryStats{ndc, expansions, W.overflow};
 }
 
-template <int K, int CT>
-__global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
+__device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wid = threadIdx.x / 64;
     const size_t per_wave = size_t(a.D) * 4 + kVisSlots * 2 + 512;
@@ -391,32 +399,54 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
     w.vis = reinterpret_cast<uint16_t*>(base + size_t(a.D) * 4);
     w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisSlots);
     w.cdist = reinterpret_cast<float*>(w.cand + 64);
-    unsigned long long ndc = 0, exps = 0, ovf = 0;
-    if (a.work_counter) {
-        while (true) {
-            uint32_t qi = 0;
-            if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
-            qi = readfirst_u(qi);
-            if (qi >= a.nq) break;
-            const QueryStats st = route_one<K, CT>(a, w, qi);
-            ndc += st.ndc;
-            exps += st.expansions;
-            ovf += st.overflow;
-        }
-    } else {
-        const uint32_t qi = blockIdx.x * kWavesPerBlock + wid;
-        if (qi < a.nq) {
-            const QueryStats st = route_one<K, CT>(a, w, qi);
-            ndc = st.ndc;
-            exps = st.expansions;
-            ovf = st.overflow;
-        }
-    }
+    return w;
+}
+
+__device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long long ndc, unsigned long long exps,
+                                          unsigned long long ovf) {
     if (lane_id() == 0 && a.stats) {
         atomicAdd(a.stats + 0, ndc);
         atomicAdd(a.stats + 1, exps);
         if (ovf) atomicAdd(a.stats + 2, ovf);
     }
+}
+
+// Plain router: one wavefront per query, the grid covers the batch.
+template <int K, int CT>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
+    const WaveLds w = wave_lds(a);
+    const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
+    if (qi >= a.nq) return;
+    const QueryStats st = route_one<K, CT>(a, w, qi);
+    add_stats(a, st.ndc, st.expansions, st.overflow);
+}
+
+// Batched router: a persistent grid sized to the resident wave slots pulls
+// query indices from a device work counter until the batch is drained, so
+// queries of uneven work (mixed topk / queue, hop counts) balance across
+// waves and no wave idles at a launch tail.
+template <int K, int CT>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void router_persistent_kernel(RouterArgs a) {
+    const WaveLds w = wave_lds(a);
+    unsigned long long ndc = 0, exps = 0, ovf = 0;
+    while (true) {
+        uint32_t qi = 0;
+        if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
+        qi = readfirst_u(qi);
+        if (qi >= a.nq) break;
+        const QueryStats st = route_one<K, CT>(a, w, qi);
+        ndc += st.ndc;
+        exps += st.expansions;
+        ovf += st.overflow;
+    }
+    add_stats(a, ndc, exps, ovf);
+}
+
+// Clears the launch counters with a kernel rather than a memset node, so a
+// captured launch sequence replays correctly.
+__global__ void reset_kernel(unsigned long long* stats, uint32_t* counter) {
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+    if (threadIdx.x == 0 && counter) *counter = 0;
 }
 
 // Empty-graph launch: every slot gets the invalid entry.
@@ -431,25 +461,25 @@ __global__ void fill_invalid_kernel(uint32_t* vids, float* dists, size_t count) 
 using KernelFn = void (*)(RouterArgs);
 
 template <int K>
-KernelFn pick_ct(uint32_t C) {
+KernelFn pick_ct(uint32_t C, bool persistent) {
     switch (C) {
-    case 6: return router_kernel<K, 6>;
-    case 7: return router_kernel<K, 7>;
-    case 8: return router_kernel<K, 8>;
-    default: return router_kernel<K, 0>;
+    case 6: return persistent ? router_persistent_kernel<K, 6> : router_kernel<K, 6>;
+    case 7: return persistent ? router_persistent_kernel<K, 7> : router_kernel<K, 7>;
+    case 8: return persistent ? router_persistent_kernel<K, 8> : router_kernel<K, 8>;
+    default: return persistent ? router_persistent_kernel<K, 0> : router_kernel<K, 0>;
     }
 }
 
 constexpr uint32_t kMinTail = 16;
 
-int pick_kernel(uint32_t L, uint32_t C, KernelFn* fn, uint32_t* K) {
+int pick_kernel(uint32_t L, uint32_t C, KernelFn* fn, uint32_t* K, bool persistent = false) {
     const uint32_t need = L + kMinTail;
-    if (need <= 64) { *K = 1; *fn = pick_ct<1>(C); }
-    else if (need <= 128) { *K = 2; *fn = pick_ct<2>(C); }
-    else if (need <= 256) { *K = 4; *fn = pick_ct<4>(C); }
-    else if (need <= 512) { *K = 8; *fn = pick_ct<8>(C); }
-    else if (need <= 832) { *K = 13; *fn = pick_ct<13>(C); }
-    else if (need <= 1024) { *K = 16; *fn = pick_ct<16>(C); }
+    if (need <= 64) { *K = 1; *fn = pick_ct<1>(C, persistent); }
+    else if (need <= 128) { *K = 2; *fn = pick_ct<2>(C, persistent); }
+    else if (need <= 256) { *K = 4; *fn = pick_ct<4>(C, persistent); }
+    else if (need <= 512) { *K = 8; *fn = pick_ct<8>(C, persistent); }
+    else if (need <= 832) { *K = 13; *fn = pick_ct<13>(C, persistent); }
+    else if (need <= 1024) { *K = 16; *fn = pick_ct<16>(C, persistent); }
     else {
         set_error("candidate_queue_size " + std::to_string(L) + " exceeds the device maximum of 1008");
         return ARTEA_EINVAL;
@@ -469,6 +499,9 @@ struct artea_router {
     float* d_thr = nullptr;
     uint32_t* d_counter = nullptr;
     bool persistent = false;
+    const void* occ_fn = nullptr;  // cached persistent grid size for (kernel, LDS bytes)
+    size_t occ_lds = 0;
+    uint32_t occ_blocks = 0;
 };
 
 struct artea_batched_router {
@@ -503,7 +536,8 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     artea_index* idx = r->idx;
     ARTEA_CHECK_ARG(idx->dg.attached, "router: no graph attached to the index");
     ARTEA_CHECK_ARG((flags & ~uint32_t(ARTEA_EARLY_STOP | ARTEA_L0_ONLY)) == 0, "router: unknown query flags");
-    ARTEA_HIP_TRY(hipMemsetAsync(r->d_stats, 0, 4 * sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, r->d_stats, persistent ? r->d_counter : nullptr);
+    ARTEA_HIP_TRY(hipGetLastError());
     if (nq == 0) return ARTEA_OK;
     if (idx->n == 0) {
         const size_t cnt = size_t(nq) * out_stride;
@@ -559,17 +593,21 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     }
     KernelFn fn;
     uint32_t K;
-    int rc = pick_kernel(queue, a.C, &fn, &K);
+    int rc = pick_kernel(queue, a.C, &fn, &K, persistent);
     if (rc) return rc;
     const size_t lds = lds_bytes(a.D);
     if (persistent) {
-        ARTEA_HIP_TRY(hipMemsetAsync(r->d_counter, 0, 4, s));
         a.work_counter = r->d_counter;
-        int dev = 0, cus = 0, per_cu = 0;
-        ARTEA_HIP_TRY(hipGetDevice(&dev));
-        ARTEA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        ARTEA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWavesPerBlock * 64, lds));
-        uint32_t blocks = uint32_t(std::max(1, cus * std::max(per_cu, 1)));
+        if (r->occ_fn != reinterpret_cast<const void*>(fn) || r->occ_lds != lds) {
+            int dev = 0, cus = 0, per_cu = 0;
+            ARTEA_HIP_TRY(hipGetDevice(&dev));
+            ARTEA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            ARTEA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWavesPerBlock * 64, lds));
+            r->occ_fn = reinterpret_cast<const void*>(fn);
+            r->occ_lds = lds;
+            r->occ_blocks = uint32_t(std::max(1, cus * std::max(per_cu, 1)));
+        }
+        uint32_t blocks = r->occ_blocks;
         blocks = std::min(blocks, (nq + kWavesPerBlock - 1) / kWavesPerBlock);
         hipLaunchKernelGGL(fn, dim3(blocks), dim3(kWavesPerBlock * 64), lds, s, a);
     } else {

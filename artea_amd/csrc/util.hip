@@ -251,6 +251,32 @@ int artea_event_elapsed_ms(void* start, void* end, float* ms) {
     ARTEA_HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)));
     return ARTEA_OK;
 }
+int artea_event_query(void* event, int* done) {
+    clear_error();
+    ARTEA_CHECK_ARG(event && done, "artea_event_query: null argument");
+    const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipSuccess) {
+        *done = 1;
+        return ARTEA_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return ARTEA_OK;
+    }
+    set_error(std::string("hipEventQuery failed: ") + hipGetErrorString(e));
+    return ARTEA_EHIP;
+}
+int artea_host_alloc(size_t bytes, void** h_ptr) {
+    clear_error();
+    ARTEA_CHECK_ARG(h_ptr, "artea_host_alloc: null output");
+    ARTEA_HIP_TRY(hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return ARTEA_OK;
+}
+int artea_host_free(void* h_ptr) {
+    clear_error();
+    if (h_ptr) ARTEA_HIP_TRY(hipHostFree(h_ptr));
+    return ARTEA_OK;
+}
 int artea_device_info(char* name, size_t len, int* cus, size_t* total) {
     clear_error();
     int dev = 0;

@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40000,
+                    help="requests per offered-load level of the serving leg (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one plain copy kernel of the base vectors (counter calibration)")
     ap.add_argument("--graph-cache", type=str, default="",
@@ -215,6 +217,30 @@ def main():
                "slot_match_vs_gpu": round(match, 6), "ndc_per_query": round(float(cndc.mean()), 1)}
         log("cpu baseline %.0f q/s on %d cores, GPU slot match %.6f" % (cpu["value"], cores, match))
 
+    serving = None
+    if args.serve_requests > 0:
+        # Open-loop Poisson arrivals of the same queries at fractions of the
+        # batched peak; continuous batching over 4 overlapping lanes.
+        from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
+
+        srv = Server(idx, max_topk=100, max_queue=chosen, lanes=4, budget=2048)
+        ns = args.serve_requests
+        pick = np.arange(ns) % nq
+        sq, stopk, squeue = queries[pick], topk_q[pick], np.full(ns, chosen, np.uint32)
+        loads = []
+        same = True
+        for i, frac in enumerate((0.25, 0.5, 0.75, 0.9)):
+            rate = frac * qps
+            v, d, lat, wall = srv.run(sq, stopk, squeue, poisson_arrivals(ns, rate, seed=seed + i))
+            same &= bool(np.array_equal(v[:, :100][stopk == 100], pred[pick][stopk == 100]))
+            loads.append({"offered_qps": round(rate), "achieved_qps": round(ns / wall),
+                          "p50_ms": round(percentile_nearest_rank(lat, 50) * 1e3, 3),
+                          "p99_ms": round(percentile_nearest_rank(lat, 99) * 1e3, 3)})
+            log("serving at %.0f q/s offered: %.0f q/s achieved, p50 %.2f ms, p99 %.2f ms" % (
+                rate, ns / wall, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
+        serving = {"requests_per_load": ns, "lanes": 4, "lane_budget": 2048, "queue": chosen,
+                   "results_equal_batch": same, "loads": loads}
+
     out = {
         "metric": "QPS @ recall@100>=0.95 (1M x 128 L2), batched vs plain GPU; p99 ms",
         "value": round(qps, 1),
@@ -238,6 +264,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "p99_ms": serving["loads"][2]["p99_ms"] if serving else None,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -245,6 +245,11 @@ int artea_event_create(void** event);
 int artea_event_destroy(void* event);
 int artea_event_record(void* event, void* stream);
 int artea_event_elapsed_ms(void* start, void* end, float* ms);
+/* 1 when all work before the event has completed, 0 when not yet. */
+int artea_event_query(void* event, int* done);
+/* Page-locked host staging memory (hipHostMalloc). */
+int artea_host_alloc(size_t bytes, void** h_ptr);
+int artea_host_free(void* h_ptr);
 int artea_device_info(char* name, size_t name_len, int* compute_units, size_t* total_bytes);
 
 /* Seeded synthetic workloads, host side, identical on every machine:

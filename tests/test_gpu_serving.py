@@ -1,0 +1,30 @@
+"""Continuous-batching server: each request's result is the same alone and
+in the mix (bit-exact), and equals the plain router's for its (topk, queue)."""
+import numpy as np
+import pytest
+
+import artea_amd as aa
+
+pytestmark = pytest.mark.gpu
+
+
+def test_results_same_alone_and_in_the_mix(orc):
+    from artea_amd.serving import Server, poisson_arrivals
+
+    vecs = aa.generate(0, 41, 0, 5000, 128)
+    idx = aa.DeviceIndex(vecs, 0)
+    aa.build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=3)
+    n = 600
+    qs = aa.generate(0, 41, 1, n, 128)
+    rng = np.random.default_rng(9)
+    topk = rng.choice([10, 50, 100], n).astype(np.uint32)
+    queue = np.maximum(topk, rng.choice([64, 128, 256, 512, 800], n)).astype(np.uint32)
+    srv = Server(idx, max_topk=100, max_queue=800, lanes=3, budget=128)
+    v, d, lat, wall = srv.run(qs, topk, queue, poisson_arrivals(n, 20000.0, seed=1))
+    assert np.all(lat > 0)
+    for i in range(0, n, 12):
+        va, da, _, _ = srv.run(qs[i:i + 1], topk[i:i + 1], queue[i:i + 1], np.zeros(1))
+        k = int(topk[i])
+        assert np.array_equal(va[0, :k], v[i, :k]) and np.array_equal(da[0, :k], d[i, :k]), i
+        pv, pd = aa.HierarchicalGraphRouter(idx, None, k, int(queue[i])).batch_query(qs[i:i + 1])
+        assert np.array_equal(pv[0], v[i, :k]) and np.array_equal(pd[0], d[i, :k]), i
