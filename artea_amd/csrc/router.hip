@@ -44,8 +44,7 @@ namespace {
 constexpr uint32_t kExplored = 0x80000000u;
 constexpr uint32_t kVidMask = 0x7FFFFFFFu;
 constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kVisLog = 12;
-constexpr uint32_t kVisSlots = 1u << kVisLog;  // lossy visited cache entries per wave (8 KiB)
+constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit tags
 
 struct RouterArgs {
     const float* vecs;
@@ -68,34 +67,46 @@ struct RouterArgs {
     uint32_t* out_vids;
     float* out_dists;
     unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
-    uint32_t hmask;                               // 2^B - 1, B = max(12, ceil(log2 n))
+    uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
+    uint32_t vis_wide;                            // 1: 4096 x 16-bit tags (B > 20), 0: 8192 x 8-bit
     uint32_t* work_counter;                       // persistent mode when non-null
 };
 
 // Visited-cache key: h = v * odd mod 2^B is a bijection of [0, 2^B), so the
-// pair (slot = low 12 bits of h, tag = 1 + the rest) identifies v exactly
-// and a 16-bit tag suffices for n <= 2^28 (0 marks an empty slot).
-__device__ __forceinline__ uint32_t vis_hash(uint32_t v, uint32_t hmask) { return (v * 0x9E3779B1u) & hmask; }
-__device__ __forceinline__ uint32_t vis_slot(uint32_t h) { return h & (kVisSlots - 1); }
-__device__ __forceinline__ uint16_t vis_tag(uint32_t h) { return uint16_t((h >> kVisLog) + 1); }
+// pair (slot = low bits of h, tag = 1 + the remaining bits) identifies v
+// exactly; tag 0 marks an empty slot.  With n <= 2^20 the 8 KiB hold 8192
+// slots of 8-bit tags, otherwise 4096 slots of 16-bit tags (n <= 2^28).
+struct VisKey {
+    uint32_t slot, tag;
+};
+__device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t wide) {
+    const uint32_t h = (v * 0x9E3779B1u) & hmask;
+    return wide ? VisKey{h & 4095u, (h >> 12) + 1} : VisKey{h & 8191u, (h >> 13) + 1};
+}
+__device__ __forceinline__ uint32_t vis_get(const unsigned char* vis, VisKey k, uint32_t wide) {
+    return wide ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[k.slot]) : uint32_t(vis[k.slot]);
+}
+__device__ __forceinline__ void vis_put(unsigned char* vis, VisKey k, uint32_t wide) {
+    if (wide) reinterpret_cast<uint16_t*>(vis)[k.slot] = uint16_t(k.tag);
+    else vis[k.slot] = uint8_t(k.tag);
+}
 
 struct WaveLds {
     float* q;         // D floats
-    uint16_t* vis;    // kVisSlots
+    unsigned char* vis;  // kVisBytes
     uint32_t* cand;   // 64
     float* cdist;     // 64
 };
 
 // Collect the unvisited, valid ids of a 64-id chunk in row order into
 // cand[0..cnt), inserting them into the visited cache; returns cnt.
-__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask) {
+__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask,
+                                            uint32_t wide) {
     const bool valid = id < n;
-    const uint32_t h = vis_hash(id, hmask);
-    const uint32_t slot = vis_slot(h);
-    const uint16_t tag = vis_tag(h);
-    const bool fresh = valid && w.vis[slot] != tag;
+    const VisKey key = vis_key(id, hmask, wide);
+    const bool fresh = valid && vis_get(w.vis, key, wide) != key.tag;
     wave_sync();
-    if (fresh) w.vis[slot] = tag;
+    if (fresh) vis_put(w.vis, key, wide);
     const uint64_t m = ballot(fresh);
     if (fresh) w.cand[lanes_below(m)] = id;
     wave_sync();
@@ -269,7 +280,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // Query row into LDS.
     const float* qg = a.queries + size_t(qi) * a.D;
     for (uint32_t i = lane; i < a.D; i += 64) w.q[i] = qg[i];
-    for (uint32_t i = lane; i < kVisSlots / 2; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+    for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
     wave_sync();
     const float qn2 = a.metric == kCosine ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
 
@@ -282,7 +293,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     if (!(a.flags & ARTEA_L0_ONLY)) {
         for (uint32_t level = a.top; level >= 1; --level) {
             // Greedy walk on `level` (greedy_loop_body); seed marked visited.
-            if (lane == 0) w.vis[vis_slot(vis_hash(cur, a.hmask))] = vis_tag(vis_hash(cur, a.hmask));
+            if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
             wave_sync();
             const bool early = (a.flags & ARTEA_EARLY_STOP) != 0;
             const float thr = early ? a.thresholds[level] : 0.f;
@@ -296,7 +307,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 for (uint32_t base = 0; base < a.strideU; base += 64) {
                     const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
                     if (ballot(id != 0xFFFFFFFFu) == 0) break;
-                    const uint32_t cnt = collect(w, id, a.n, a.hmask);
+                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
                     if (cnt == 0) continue;
                     distances<CT>(a, w, cnt, qn2);
                     ndc += cnt;
@@ -325,7 +336,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 cd = best_d;
             }
             // visited.clear() after every level (:228-236).
-            for (uint32_t i = lane; i < kVisSlots / 2; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+            for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
             wave_sync();
         }
     }
@@ -334,7 +345,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     CandList<K> W;
     W.init(L);
     W.insert(cd, cur);
-    if (lane == 0) w.vis[vis_slot(vis_hash(cur, a.hmask))] = vis_tag(vis_hash(cur, a.hmask));
+    if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
     wave_sync();
     uint32_t pv;
     // Speculative read of the first 64 ids of the row the next pop will most
@@ -351,7 +362,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
         for (uint32_t base = 0; base < a.stride0; base += 64) {
             const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
             if (ballot(id != 0xFFFFFFFFu) == 0) break;
-            const uint32_t cnt = collect(w, id, a.n, a.hmask);
+            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
             if (cnt == 0) continue;
             distances<CT>(a, w, cnt, qn2);
             ndc += cnt;
@@ -392,12 +403,12 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
 __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wid = threadIdx.x / 64;
-    const size_t per_wave = size_t(a.D) * 4 + kVisSlots * 2 + 512;
+    const size_t per_wave = size_t(a.D) * 4 + kVisBytes + 512;
     unsigned char* base = smem + wid * per_wave;
     WaveLds w;
     w.q = reinterpret_cast<float*>(base);
-    w.vis = reinterpret_cast<uint16_t*>(base + size_t(a.D) * 4);
-    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisSlots);
+    w.vis = base + size_t(a.D) * 4;
+    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisBytes);
     w.cdist = reinterpret_cast<float*>(w.cand + 64);
     return w;
 }
@@ -487,7 +498,7 @@ int pick_kernel(uint32_t L, uint32_t C, KernelFn* fn, uint32_t* K, bool persiste
     return ARTEA_OK;
 }
 
-size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisSlots * 2 + 512); }
+size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisBytes + 512); }
 
 }  // namespace
 
@@ -586,10 +597,11 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     a.stats = r->d_stats;
     a.work_counter = nullptr;
     {
-        uint32_t B = kVisLog;
+        uint32_t B = 13;
         while (B < 32 && (uint64_t(1) << B) < idx->n) ++B;
         ARTEA_CHECK_ARG(B <= 28, "router: more than 2^28 vectors");
         a.hmask = uint32_t((uint64_t(1) << B) - 1);
+        a.vis_wide = B > 20 ? 1u : 0u;
     }
     KernelFn fn;
     uint32_t K;

@@ -78,9 +78,13 @@ class _Lane:
         self.max_topk = max_topk
         self.inflight: Optional[_Launch] = None
 
+    def cls(self, q: int) -> int:
+        # smallest router class that holds the queue and the row stride
+        return queue_class(max(int(q), self.max_topk))
+
     def launch(self, ids, queries, topk, queue):
         m = len(ids)
-        order = np.argsort(np.array([queue_class(int(q)) for q in queue[ids]]), kind="stable")
+        order = np.argsort(np.array([self.cls(q) for q in queue[ids]]), kind="stable")
         ids = ids[order]
         self.hq.array[:m] = queries[ids]
         self.hk.array[:m] = topk[ids]
@@ -90,7 +94,7 @@ class _Lane:
         check(lib().artea_memcpy_htod(C.c_void_p(self.dq.ptr), C.c_void_p(self.hq.ptr), m * d * 4, C.c_void_p(s)))
         check(lib().artea_memcpy_htod(C.c_void_p(self.dk.ptr), C.c_void_p(self.hk.ptr), m * 4, C.c_void_p(s)))
         check(lib().artea_memcpy_htod(C.c_void_p(self.dl.ptr), C.c_void_p(self.hl.ptr), m * 4, C.c_void_p(s)))
-        classes = np.array([queue_class(int(q)) for q in self.hl.array[:m]])
+        classes = np.array([self.cls(q) for q in self.hl.array[:m]])
         start = 0
         while start < m:
             c = classes[start]
@@ -120,7 +124,8 @@ class Server:
     how many launches overlap."""
 
     def __init__(self, index, max_topk: int = 100, max_queue: int = 800, lanes: int = 4, budget: int = 2048):
-        classes = [c for c in QUEUE_CLASSES if c <= queue_class(max_queue)]
+        top = queue_class(max(max_queue, max_topk))
+        classes = [c for c in QUEUE_CLASSES if queue_class(max_topk) <= c <= top]
         self.lanes = [_Lane(index, max_topk, budget, classes) for _ in range(lanes)]
         self.max_topk = max_topk
         self.budget = budget
