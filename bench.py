@@ -188,6 +188,12 @@ def main():
     row_bytes = 4 * 64
     alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
     achieved = alg_bytes / t_step / 1e9
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # this workload (calibrated on a copy kernel), committed under profiles/.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "router_c1_traffic.json")
+    if os.path.exists(tpath) and args.n == 1_000_000 and nq == 10_000 and chosen == 200:
+        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
 
     cpu = None
     if not args.no_cpu:
@@ -262,7 +268,9 @@ def main():
         "build_seconds": round(build_s, 2),
         "sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "router_persistent_kernel (batched router), one launch per step",
+                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_step * 1e3, 4)},
         "cpu_baseline": cpu,
         "p99_ms": serving["loads"][2]["p99_ms"] if serving else None,
         "serving": serving,
