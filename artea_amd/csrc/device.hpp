@@ -47,6 +47,25 @@ __device__ __forceinline__ float quad_xor1(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
 }
 
+// One DPP min step; lanes without a source keep their own value.
+template <int CTRL>
+__device__ __forceinline__ float dpp_min_step(float x) {
+    const int y = __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, 0xF, 0xF, false);
+    return fminf(x, __int_as_float(y));
+}
+
+// Minimum over the 64 lanes (quad permutes, row shifts, row broadcasts; the
+// full minimum lands in lane 63), returned to every lane.
+__device__ __forceinline__ float wave_min(float x) {
+    x = dpp_min_step<0xB1>(x);   // quad_perm [1,0,3,2]
+    x = dpp_min_step<0x4E>(x);   // quad_perm [2,3,0,1]
+    x = dpp_min_step<0x114>(x);  // row_shr:4
+    x = dpp_min_step<0x118>(x);  // row_shr:8
+    x = dpp_min_step<0x142>(x);  // row_bcast:15
+    x = dpp_min_step<0x143>(x);  // row_bcast:31
+    return readlane_f(x, 63);
+}
+
 // Sum of the 16 lane accumulators of one AVX-512 register in the order of
 // GCC's _mm512_reduce_add_ps: t8[i] = a[i+8] + a[i], t4[i] = t8[i+4] + t8[i],
 // (t4[0]+t4[2]) + (t4[1]+t4[3]).  A quad of lanes holds the register:

@@ -270,67 +270,55 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
         }
         __syncthreads();
         if (wid == 0) {
+            // Retained set in registers: lane l holds retained positions l and
+            // l + 64 (row index and new flag); it only grows by appending.
             uint32_t nR = 1, ne = 0;
-            if (lane == 0) s.R[0] = 0;
-            wave_sync();
+            uint32_t r_lo = lane == 0 ? 0u : 0xFFFFFFFFu, r_hi = 0xFFFFFFFFu;
+            bool n_lo = lane == 0 && s.isnew[0] != 0, n_hi = false;
             for (uint32_t i = 1; i < n; ++i) {
                 const bool cnew = s.isnew[i] != 0;
                 const float di = s.dist[i];
                 const float prune = di * inv_scale - shift_term;
-                float t0 = 0.f, t1 = 0.f;
-                bool v0 = false, v1 = false;
-                if (lane < nR) {
-                    const uint32_t j = s.R[lane];
-                    v0 = cnew || s.isnew[j];
-                    if (v0) t0 = s.P[tri_index(i, j)];
-                }
-                if (lane + 64 < nR) {
-                    const uint32_t j = s.R[lane + 64];
-                    v1 = cnew || s.isnew[j];
-                    if (v1) t1 = s.P[tri_index(i, j)];
-                }
+                const bool v0 = lane < nR && (cnew || n_lo);
+                const bool v1 = lane + 64 < nR && (cnew || n_hi);
+                const float t0 = v0 ? s.P[tri_index(i, r_lo)] : 0.f;
+                const float t1 = v1 ? s.P[tri_index(i, r_hi)] : 0.f;
                 const uint64_t b0 = ballot(v0 && t0 < prune), b1 = ballot(v1 && t1 < prune);
                 // Scan runs from the farthest retained entry; the first prune hit stops it.
                 const int jstar = b1 ? 64 + 63 - __clzll(b1) : (b0 ? 63 - __clzll(b0) : -1);
                 const bool r0 = v0 && t0 < di && int(lane) >= jstar;
                 const bool r1 = v1 && t1 < di && int(lane) + 64 >= jstar;
-                float bt = FLT_MAX;
-                int bk = -1;
-                if (r0 && t0 < bt) {
-                    bt = t0;
-                    bk = int(lane);
-                }
-                if (r1 && t1 <= bt) {  // later position wins equal distances
-                    bt = t1;
-                    bk = int(lane) + 64;
-                }
-                for (int off = 32; off >= 1; off >>= 1) {
-                    const float ot = __shfl_xor(bt, off);
-                    const int ok = __shfl_xor(bk, off);
-                    if (ok >= 0 && (bk < 0 || ot < bt || (ot == bt && ok > bk))) {
-                        bt = ot;
-                        bk = ok;
+                if (ballot(r0 || r1)) {
+                    // Closest recommendation target; equal distances go to the
+                    // later position (the scan meets it first).
+                    const float m = wave_min(fminf(r0 ? t0 : FLT_MAX, r1 ? t1 : FLT_MAX));
+                    const uint64_t e1 = ballot(r1 && t1 == m), e0 = ballot(r0 && t0 == m);
+                    const int bk = e1 ? 64 + 63 - __clzll(e1) : 63 - __clzll(e0);
+                    const uint32_t tv = bk >= 64 ? readlane_u(r_hi, bk - 64) : readlane_u(r_lo, bk);
+                    const uint32_t target = local_of(c, s.vid[tv]);
+                    if (target != 0xFFFFFFFFu) {
+                        if (lane == 0) emit(c, row, ne, target, s.vid[i], m);
+                        ne++;
                     }
                 }
-                bk = __builtin_amdgcn_readfirstlane(bk);
-                bt = readlane_f(bt, 0);
-                if (bk >= 0) {
-                    const uint32_t tv = s.R[bk];
-                    uint32_t target = local_of(c, s.vid[tv]);
-                    if (lane == 0 && target != 0xFFFFFFFFu) emit(c, row, ne, target, s.vid[i], bt);
-                    if (target != 0xFFFFFFFFu) ne++;
-                }
                 if (jstar < 0) {
-                    if (lane == 0) s.R[nR] = i;
+                    if (nR < 64) {
+                        if (lane == nR) {
+                            r_lo = i;
+                            n_lo = cnew;
+                        }
+                    } else if (lane == nR - 64) {
+                        r_hi = i;
+                        n_hi = cnew;
+                    }
                     nR++;
                 }
-                wave_sync();
             }
             // Retained entries, all marked old (mark_as_old before the swap).
-            for (uint32_t k = lane; k < nR; k += 64) {
-                const uint32_t pos = s.R[k];
-                c.rows[size_t(row) * c.cap + k] = make_uint2(s.vid[pos], __float_as_uint(s.dist[pos]));
-            }
+            if (lane < nR)
+                c.rows[size_t(row) * c.cap + lane] = make_uint2(s.vid[r_lo], __float_as_uint(s.dist[r_lo]));
+            if (lane + 64 < nR)
+                c.rows[size_t(row) * c.cap + lane + 64] = make_uint2(s.vid[r_hi], __float_as_uint(s.dist[r_hi]));
             if (lane == 0) {
                 c.cnt[row] = nR;
                 c.e_cnt[row] = ne;
