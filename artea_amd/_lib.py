@@ -76,8 +76,10 @@ SIGNATURES = {
     "artea_refine_truncate": (i32, [P]),
     "artea_refine_reverse": (i32, [P]),
     "artea_refine_schedule": (i32, [P, u32, u32, f32, f32]),
-    "artea_refine_last_stats": (i32, [P, P, P]),
-    "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, P]),
+    "artea_refine_set_join": (i32, [P, i32]),
+    "artea_refine_set_recheck_queue": (i32, [P, u32]),
+    "artea_refine_last_stats": (i32, [P, P, P, P]),
+    "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, P]),
 }
 
 STATUS = {0: "ARTEA_OK", 1: "ARTEA_EINVAL", 2: "ARTEA_EIO", 3: "ARTEA_EHIP", 4: "ARTEA_ENOMEM", 5: "ARTEA_EOVERFLOW"}

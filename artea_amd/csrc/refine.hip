@@ -53,7 +53,8 @@ struct Ctx {
     uint32_t* t_off;      // n + 1
     uint32_t* t_fill;     // n
     uint2* seg;           // n * cap
-    unsigned long long* stats;  // [0] logs, [1] pair distances
+    unsigned long long* stats;  // [0] logs, [1] FP32 pair distances, [2] BF16 pair distances
+    uint32_t list_cap;          // triangle recheck queue entries per row (<= kListCap)
 };
 
 __device__ __forceinline__ uint32_t storage_of(const Ctx& c, uint32_t local) { return c.l2g ? c.l2g[local] : local; }
@@ -206,133 +207,537 @@ __global__ void reverse_kernel(Ctx c) {
 // --------------------------------------------------------------- triangle
 // TriangleUpdater::update_impl / _internal_check
 // (refiner/updaters/triangle_updater.hpp:114-157, 200-237), one 256-thread
-// block per row.  The row's vectors are staged in LDS (padded stride), all
-// needed pair distances (not old-old) are computed in parallel, then wave 0
-// replays the sequential retain/recommend decisions.
-struct TriLds {
-    uint32_t* vid;   // kMaxCap
-    float* dist;     // kMaxCap
-    uint32_t* isnew; // kMaxCap
-    uint32_t* R;     // kMaxCap retained positions
-    float* P;        // cap*(cap-1)/2
-    float* vec;      // cap * vstride (staged mode)
+// block per row, rows sorted by (distance, vid).  The reference walks the
+// entries in order: entry i is compared with the retained entries from the
+// farthest down (pairs where both are old are skipped); the first retained r
+// with dist(i, r) < d_i / scale - shift_term prunes i, and the closest
+// retained r at or beyond that point with dist(i, r) < d_i receives i as a
+// recommendation (equal distances: the later position).  Those decisions
+// only depend on two per-pair predicates, so the block evaluates them for all
+// pairs at once as bitmasks over positions:
+//   H_i = { r < i : pair needed, dist(i, r) < prune_i }   (prune hits)
+//   C_i = { r < i : pair needed, dist(i, r) < d_i }       (recommendation candidates)
+// The retained set R is then the fixed point of "i in R iff H_i and R are
+// disjoint" (evaluated in rounds of a wave, each round deciding at least the
+// lowest open entry), and entry i recommends to the closest member of
+// C_i & R & [jstar_i, i) with jstar_i = the highest member of H_i & R.
+//
+// Pair distances come from one of two paths:
+//  * FP32: every needed pair by the FP32 chain of dist_func_t;
+//  * BF16: the row's Gram matrix in 32x32 tiles on MFMA (two-term BF16 split
+//    of each coordinate, FP32 accumulation; L2 vectors are first centred on
+//    the row's own vertex) gives every pair's distance inside a proven
+//    interval.  A pair whose interval straddles prune_i or d_i, and every
+//    candidate that may be the recommendation target, is recomputed by the
+//    FP32 chain, so every decision and every emitted distance equals the
+//    FP32 path's.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kListCap = 1024;  // FP32 recheck queue per row; overflow goes to a bitmap
+
+// Half-widths of the BF16 intervals, per metric (derived on the host from D).
+struct JoinBounds {
+    float l2;   // x (N_i + N_j), centred squared norms, on the squared distance
+    float dot;  // x sqrt(N_i N_j)
+    float cos;  // absolute
 };
+
+struct TriLayout {
+    uint32_t vid, dist, norm, H, Cw, X, pend, list, center, P, vec, total;
+};
+
+__host__ __device__ inline uint32_t align16u(uint32_t x) { return (x + 15u) & ~15u; }
+
+__host__ __device__ inline TriLayout tri_layout(uint32_t cap, uint32_t D, bool staged) {
+    TriLayout L;
+    const uint32_t np = cap * (cap - 1) / 2;
+    uint32_t o = 0;
+    L.vid = o;
+    o += kMaxCap * 4;
+    L.dist = o;
+    o += kMaxCap * 4;
+    L.norm = o;
+    o += kMaxCap * 4;
+    L.H = o;
+    o += kMaxCap * 16;
+    L.Cw = o;
+    o += kMaxCap * 16;
+    L.X = o;
+    o += align16u((np + 31) / 32 * 4);
+    L.pend = o;
+    o += align16u((np + 31) / 32 * 4);
+    L.list = o;
+    o += kListCap * 2;
+    L.center = o;
+    o += align16u(D * 4);
+    L.P = o;
+    o += align16u(np * 4);
+    L.vec = o;
+    if (staged) o += (cap + 31) / 32 * 32 * (D + 4) * 4;
+    L.total = o;
+    return L;
+}
 
 __device__ __forceinline__ uint32_t tri_index(uint32_t i, uint32_t j) { return i * (i - 1) / 2 + j; }  // i > j
 
+__device__ __forceinline__ uint32_t word_below(uint32_t i, uint32_t w) {  // bits [0, i) of word w
+    const uint32_t b = 32u * w;
+    return i >= b + 32u ? 0xFFFFFFFFu : (i > b ? (1u << (i - b)) - 1u : 0u);
+}
+
+// Two-term BF16 split of 8 coordinates: x = hi + lo + O(2^-16 |x|).
+__device__ __forceinline__ void split8(float4 x0, float4 x1, bf16x8& hi, bf16x8& lo) {
+    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const f32x2 a = {v[2 * p], v[2 * p + 1]};
+        const bf16x2 h = __builtin_convertvector(a, bf16x2);
+        const f32x2 hf = __builtin_convertvector(h, f32x2);
+        const f32x2 r = {a[0] - hf[0], a[1] - hf[1]};
+        const bf16x2 l = __builtin_convertvector(r, bf16x2);
+        hi[2 * p] = h[0];
+        hi[2 * p + 1] = h[1];
+        lo[2 * p] = l[0];
+        lo[2 * p + 1] = l[1];
+    }
+}
+
+// Metric value from the approximate Gram entry (centred norms for L2).
+__device__ __forceinline__ float approx_metric(int metric, float g, float ni, float nj) {
+    if (metric == kL2Sqr) return (ni + nj) - 2.0f * g;
+    if (metric == kL2) return __builtin_sqrtf(fmaxf((ni + nj) - 2.0f * g, 0.0f));
+    if (metric == kDot) return -g;
+    const float den = __builtin_sqrtf(ni * nj);
+    return den > 0.0f ? 1.0f - g / den : 1.0f;
+}
+
+// Interval certain to hold the FP32-chain value of a pair whose approximate
+// value is t.  Non-finite inputs give NaN or infinite ends, which make every
+// decision on the pair ambiguous.
+__device__ __forceinline__ void approx_interval(int metric, const JoinBounds& kb, float t, float ni, float nj,
+                                                float& lo, float& hi) {
+    if (metric == kL2Sqr) {
+        const float e = kb.l2 * (ni + nj);
+        lo = t - e;
+        hi = t + e;
+    } else if (metric == kL2) {
+        const float t2 = t * t;
+        const float e = kb.l2 * (ni + nj) + 0x1p-21f * t2;
+        lo = __builtin_sqrtf(fmaxf(t2 - e, 0.0f)) * (1.0f - 0x1p-21f);
+        hi = __builtin_sqrtf(t2 + e) * (1.0f + 0x1p-21f);
+    } else if (metric == kDot) {
+        const float e = kb.dot * __builtin_sqrtf(ni * nj);
+        lo = t - e;
+        hi = t + e;
+    } else {
+        if (!(ni * nj > 0.0f)) {
+            lo = -INFINITY;
+            hi = INFINITY;
+        } else {
+            lo = t - kb.cos;
+            hi = t + kb.cos;
+        }
+    }
+}
+
+template <bool MFMA>
 __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, float shift_term, int staged,
-                                                       uint32_t vstride) {
+                                                       uint32_t vstride, JoinBounds kb) {
     extern __shared__ __align__(16) unsigned char smem[];
-    TriLds s;
-    s.vid = reinterpret_cast<uint32_t*>(smem);
-    s.dist = reinterpret_cast<float*>(s.vid + kMaxCap);
-    s.isnew = reinterpret_cast<uint32_t*>(s.dist + kMaxCap);
-    s.R = s.isnew + kMaxCap;
-    s.P = reinterpret_cast<float*>(s.R + kMaxCap);
-    const uint32_t npairs_max = c.cap * (c.cap - 1) / 2;
-    s.vec = s.P + ((npairs_max + 3) / 4 * 4);
+    const TriLayout L = tri_layout(c.cap, c.D, staged != 0);
+    uint32_t* s_vid = reinterpret_cast<uint32_t*>(smem + L.vid);
+    float* s_dist = reinterpret_cast<float*>(smem + L.dist);
+    float* s_norm = reinterpret_cast<float*>(smem + L.norm);
+    uint32_t* s_H = reinterpret_cast<uint32_t*>(smem + L.H);
+    uint32_t* s_C = reinterpret_cast<uint32_t*>(smem + L.Cw);
+    uint32_t* s_X = reinterpret_cast<uint32_t*>(smem + L.X);
+    uint32_t* s_pend = reinterpret_cast<uint32_t*>(smem + L.pend);
+    uint16_t* s_list = reinterpret_cast<uint16_t*>(smem + L.list);
+    float* s_cen = reinterpret_cast<float*>(smem + L.center);
+    float* s_P = reinterpret_cast<float*>(smem + L.P);
+    float* s_vec = reinterpret_cast<float*>(smem + L.vec);
+    __shared__ uint32_t s_newm[4], s_R[4], s_nl, s_ne;
+
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
-    unsigned long long pairs_done = 0;
+    const bool centred = MFMA && (c.metric == kL2Sqr || c.metric == kL2);
+    const uint32_t xwords = (c.cap * (c.cap - 1) / 2 + 31) / 32;
+    unsigned long long exact_pairs = 0, approx_pairs = 0;
+
+    auto vec_of = [&](uint32_t k) -> const float* {
+        return staged ? s_vec + k * vstride : c.vecs + size_t(s_vid[k]) * c.D;
+    };
+    auto is_new = [&](uint32_t k) -> bool { return (s_newm[k >> 5] >> (k & 31)) & 1u; };
+    auto prune_of = [&](float d) -> float { return d * inv_scale - shift_term; };
+    auto exact_pair = [&](uint32_t i, uint32_t j) -> float {
+        return thread_distance(vec_of(i), vec_of(j), int(c.C), c.metric);
+    };
+    auto mark_exact = [&](uint32_t p) { atomicOr(s_X + (p >> 5), 1u << (p & 31)); };
+    auto is_exact = [&](uint32_t p) -> bool { return (s_X[p >> 5] >> (p & 31)) & 1u; };
+
     for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
         const uint32_t n = c.cnt[row];
-        if (tid == 0) c.e_cnt[row] = 0;
         if (n == 0) {
-            __syncthreads();
-            continue;
+            if (tid == 0) c.e_cnt[row] = 0;
+            continue;  // uniform; LDS untouched
         }
-        for (uint32_t k = tid; k < n; k += 256) {
-            const uint2 e = c.rows[size_t(row) * c.cap + k];
-            s.vid[k] = e.x & kMask;
-            s.dist[k] = __uint_as_float(e.y);
-            s.isnew[k] = (e.x & kNew) ? 1u : 0u;
+        // ---- row metadata, masks, staging
+        if (tid < 128) {
+            bool nw = false;
+            if (tid < n) {
+                const uint2 e = c.rows[size_t(row) * c.cap + tid];
+                s_vid[tid] = e.x & kMask;
+                s_dist[tid] = __uint_as_float(e.y);
+                nw = (e.x & kNew) != 0;
+            }
+            const uint64_t b = ballot(nw);
+            if (lane == 0) {
+                s_newm[2 * wid] = uint32_t(b);
+                s_newm[2 * wid + 1] = uint32_t(b >> 32);
+            }
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                s_H[tid * 4 + w] = 0;
+                s_C[tid * 4 + w] = 0;
+            }
+        }
+        if (MFMA)
+            for (uint32_t k = tid; k < xwords; k += 256) {
+                s_X[k] = 0;
+                s_pend[k] = 0;
+            }
+        if (tid == 0) {
+            s_nl = 0;
+            s_ne = 0;
         }
         __syncthreads();
+        const uint32_t T = (n + 31) / 32;
+        const uint32_t npad = MFMA ? T * 32 : n;
         if (staged) {
             const uint32_t per = c.D / 4;  // float4 per vector
-            for (uint32_t t = tid; t < n * per; t += 256) {
+            for (uint32_t t = tid; t < npad * per; t += 256) {
                 const uint32_t k = t / per, q = t % per;
-                *reinterpret_cast<float4*>(s.vec + k * vstride + q * 4) =
-                    *reinterpret_cast<const float4*>(c.vecs + size_t(s.vid[k]) * c.D + q * 4);
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (k < n) v = *reinterpret_cast<const float4*>(c.vecs + size_t(s_vid[k]) * c.D + q * 4);
+                *reinterpret_cast<float4*>(s_vec + k * vstride + q * 4) = v;
+            }
+            if (centred)
+                for (uint32_t q = tid; q < per; q += 256)
+                    *reinterpret_cast<float4*>(s_cen + q * 4) =
+                        *reinterpret_cast<const float4*>(c.vecs + size_t(storage_of(c, row)) * c.D + q * 4);
+        }
+        __syncthreads();
+
+        // ---- pair values and the H / C bitmasks
+        if (MFMA) {
+            // Squared norms of the (centred) coordinates as the tiles see them.
+            for (uint32_t k = tid; k < n; k += 256) {
+                float acc = 0.f;
+                for (uint32_t d = 0; d < c.D; d += 4) {
+                    float4 x = *reinterpret_cast<const float4*>(s_vec + k * vstride + d);
+                    if (centred) {
+                        const float4 m = *reinterpret_cast<const float4*>(s_cen + d);
+                        x = make_float4(x.x - m.x, x.y - m.y, x.z - m.z, x.w - m.w);
+                    }
+                    acc = __builtin_fmaf(x.x, x.x, acc);
+                    acc = __builtin_fmaf(x.y, x.y, acc);
+                    acc = __builtin_fmaf(x.z, x.z, acc);
+                    acc = __builtin_fmaf(x.w, x.w, acc);
+                }
+                s_norm[k] = acc;
+            }
+            __syncthreads();
+        } else {
+            const uint32_t np = n * (n - 1) / 2;
+            for (uint32_t p = tid; p < np; p += 256) {
+                uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
+                while (i * (i - 1) / 2 > p) --i;
+                while ((i + 1) * i / 2 <= p) ++i;
+                const uint32_t j = p - i * (i - 1) / 2;
+                if (!is_new(i) && !is_new(j)) continue;
+                s_P[p] = exact_pair(i, j);
+                exact_pairs++;
             }
             __syncthreads();
         }
-        const uint32_t np = n * (n - 1) / 2;
-        for (uint32_t p = tid; p < np; p += 256) {
-            uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
-            while (i * (i - 1) / 2 > p) --i;
-            while ((i + 1) * i / 2 <= p) ++i;
-            const uint32_t j = p - i * (i - 1) / 2;
-            if (!s.isnew[i] && !s.isnew[j]) continue;
-            const float* a = staged ? s.vec + i * vstride : c.vecs + size_t(s.vid[i]) * c.D;
-            const float* b = staged ? s.vec + j * vstride : c.vecs + size_t(s.vid[j]) * c.D;
-            s.P[p] = thread_distance(a, b, int(c.C), c.metric);
-            pairs_done++;
+        const uint32_t ntiles = T * (T + 1) / 2;
+        for (uint32_t t = wid; t < ntiles; t += 4) {
+            uint32_t I = 0;
+            while ((I + 1) * (I + 2) / 2 <= t) ++I;
+            const uint32_t J = t - I * (I + 1) / 2;
+            const uint32_t j = 32 * J + (lane & 31);
+            const bool jnew = j < n && is_new(j);
+            float tv[16];
+            if (MFMA) {
+                f32x16 acc = {};
+                const uint32_t ko = 8 * (lane >> 5);
+                const float* pa = s_vec + (32 * I + (lane & 31)) * vstride + ko;
+                const float* pb = s_vec + (32 * J + (lane & 31)) * vstride + ko;
+                for (uint32_t k0 = 0; k0 < c.D; k0 += 16) {
+                    float4 a0 = *reinterpret_cast<const float4*>(pa + k0);
+                    float4 a1 = *reinterpret_cast<const float4*>(pa + k0 + 4);
+                    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
+                    if (centred) {
+                        c0 = *reinterpret_cast<const float4*>(s_cen + ko + k0);
+                        c1 = *reinterpret_cast<const float4*>(s_cen + ko + k0 + 4);
+                        a0 = make_float4(a0.x - c0.x, a0.y - c0.y, a0.z - c0.z, a0.w - c0.w);
+                        a1 = make_float4(a1.x - c1.x, a1.y - c1.y, a1.z - c1.z, a1.w - c1.w);
+                    }
+                    bf16x8 ahi, alo, bhi, blo;
+                    split8(a0, a1, ahi, alo);
+                    if (I != J) {
+                        float4 b0 = *reinterpret_cast<const float4*>(pb + k0);
+                        float4 b1 = *reinterpret_cast<const float4*>(pb + k0 + 4);
+                        if (centred) {
+                            b0 = make_float4(b0.x - c0.x, b0.y - c0.y, b0.z - c0.z, b0.w - c0.w);
+                            b1 = make_float4(b1.x - c1.x, b1.y - c1.y, b1.z - c1.z, b1.w - c1.w);
+                        }
+                        split8(b0, b1, bhi, blo);
+                    } else {
+                        bhi = ahi;
+                        blo = alo;
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+                }
+                const float nj = j < n ? s_norm[j] : 0.f;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    tv[q] = i < n ? approx_metric(c.metric, acc[q], s_norm[i], nj) : 0.f;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    tv[q] = (i < n && j < i && (jnew || is_new(i))) ? s_P[tri_index(i, j)] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const bool valid = i < n && j < i && (jnew || is_new(i));
+                const float di = i < n ? s_dist[i] : 0.f;
+                const float prune = prune_of(di);
+                bool hit = false, rec = false;
+                if (MFMA) {
+                    float lo = 0.f, hi = 0.f;
+                    if (valid) approx_interval(c.metric, kb, tv[q], s_norm[i], s_norm[j], lo, hi);
+                    const bool amb = valid && !((hi < prune || lo >= prune) && (hi < di || lo >= di));
+                    if (valid) s_P[tri_index(i, j)] = tv[q];
+                    approx_pairs += valid ? 1 : 0;
+                    const uint64_t am = ballot(amb);
+                    if (am) {
+                        uint32_t base = 0;
+                        if (lane == 0) base = atomicAdd(&s_nl, uint32_t(__popcll(am)));
+                        base = readlane_u(base, 0);
+                        const uint32_t slot = base + lanes_below(am);
+                        if (amb) {
+                            if (slot < c.list_cap) {
+                                s_list[slot] = uint16_t((i << 7) | j);
+                            } else {  // queue full: flagged in the overflow bitmap
+                                const uint32_t p = tri_index(i, j);
+                                atomicOr(s_pend + (p >> 5), 1u << (p & 31));
+                            }
+                        }
+                    }
+                    if (valid && !amb) {
+                        hit = hi < prune;
+                        rec = hi < di;
+                    }
+                } else {
+                    hit = valid && tv[q] < prune;
+                    rec = valid && tv[q] < di;
+                }
+                const uint64_t hb = ballot(hit), cb = ballot(rec);
+                if (lane == 0) {
+                    const uint32_t i0 = 32 * I + 8 * (q >> 2) + (q & 3);
+                    s_H[i0 * 4 + J] = uint32_t(hb);
+                    s_H[(i0 + 4) * 4 + J] = uint32_t(hb >> 32);
+                    s_C[i0 * 4 + J] = uint32_t(cb);
+                    s_C[(i0 + 4) * 4 + J] = uint32_t(cb >> 32);
+                }
+            }
         }
         __syncthreads();
+        if (MFMA) {
+            // Ambiguous predicates: FP32 chain, bits set here.
+            auto settle = [&](uint32_t i, uint32_t j) {
+                const float te = exact_pair(i, j);
+                const uint32_t p = tri_index(i, j);
+                s_P[p] = te;
+                mark_exact(p);
+                exact_pairs++;
+                const float di = s_dist[i];
+                if (te < prune_of(di)) atomicOr(s_H + i * 4 + (j >> 5), 1u << (j & 31));
+                if (te < di) atomicOr(s_C + i * 4 + (j >> 5), 1u << (j & 31));
+            };
+            const uint32_t nl = min(s_nl, c.list_cap);
+            for (uint32_t k = tid; k < nl; k += 256) settle(s_list[k] >> 7, s_list[k] & 127u);
+            if (s_nl > c.list_cap)
+                for (uint32_t w = tid; w < xwords; w += 256)
+                    for (uint32_t m = s_pend[w]; m; m &= m - 1) {
+                        const uint32_t p = 32 * w + __ffs(m) - 1;
+                        uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
+                        while (i * (i - 1) / 2 > p) --i;
+                        while ((i + 1) * i / 2 <= p) ++i;
+                        settle(i, p - i * (i - 1) / 2);
+                    }
+            __syncthreads();
+        }
+
+        // ---- retained set: rounds of "i in R iff H_i and R are disjoint"
         if (wid == 0) {
-            // Retained set in registers: lane l holds retained positions l and
-            // l + 64 (row index and new flag); it only grows by appending.
-            uint32_t nR = 1, ne = 0;
-            uint32_t r_lo = lane == 0 ? 0u : 0xFFFFFFFFu, r_hi = 0xFFFFFFFFu;
-            bool n_lo = lane == 0 && s.isnew[0] != 0, n_hi = false;
-            for (uint32_t i = 1; i < n; ++i) {
-                const bool cnew = s.isnew[i] != 0;
-                const float di = s.dist[i];
-                const float prune = di * inv_scale - shift_term;
-                const bool v0 = lane < nR && (cnew || n_lo);
-                const bool v1 = lane + 64 < nR && (cnew || n_hi);
-                const float t0 = v0 ? s.P[tri_index(i, r_lo)] : 0.f;
-                const float t1 = v1 ? s.P[tri_index(i, r_hi)] : 0.f;
-                const uint64_t b0 = ballot(v0 && t0 < prune), b1 = ballot(v1 && t1 < prune);
-                // Scan runs from the farthest retained entry; the first prune hit stops it.
-                const int jstar = b1 ? 64 + 63 - __clzll(b1) : (b0 ? 63 - __clzll(b0) : -1);
-                const bool r0 = v0 && t0 < di && int(lane) >= jstar;
-                const bool r1 = v1 && t1 < di && int(lane) + 64 >= jstar;
-                if (ballot(r0 || r1)) {
-                    // Closest recommendation target; equal distances go to the
-                    // later position (the scan meets it first).
-                    const float m = wave_min(fminf(r0 ? t0 : FLT_MAX, r1 ? t1 : FLT_MAX));
-                    const uint64_t e1 = ballot(r1 && t1 == m), e0 = ballot(r0 && t0 == m);
-                    const int bk = e1 ? 64 + 63 - __clzll(e1) : 63 - __clzll(e0);
-                    const uint32_t tv = bk >= 64 ? readlane_u(r_hi, bk - 64) : readlane_u(r_lo, bk);
-                    const uint32_t target = local_of(c, s.vid[tv]);
-                    if (target != 0xFFFFFFFFu) {
-                        if (lane == 0) emit(c, row, ne, target, s.vid[i], m);
-                        ne++;
-                    }
-                }
-                if (jstar < 0) {
-                    if (nR < 64) {
-                        if (lane == nR) {
-                            r_lo = i;
-                            n_lo = cnew;
-                        }
-                    } else if (lane == nR - 64) {
-                        r_hi = i;
-                        n_hi = cnew;
-                    }
-                    nR++;
-                }
+            uint32_t ha[4], hb[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                ha[w] = s_H[lane * 4 + w];
+                hb[w] = s_H[(lane + 64) * 4 + w];
             }
-            // Retained entries, all marked old (mark_as_old before the swap).
-            if (lane < nR)
-                c.rows[size_t(row) * c.cap + lane] = make_uint2(s.vid[r_lo], __float_as_uint(s.dist[r_lo]));
-            if (lane + 64 < nR)
-                c.rows[size_t(row) * c.cap + lane + 64] = make_uint2(s.vid[r_hi], __float_as_uint(s.dist[r_hi]));
+            uint32_t kr[4] = {0, 0, 0, 0}, kn[4] = {0, 0, 0, 0};
+            bool da = lane >= n, db = lane + 64 >= n;
+            for (uint32_t round = 0; round <= kMaxCap; ++round) {
+                bool ra = false, na = false, rb = false, nb = false;
+                if (!da) {
+                    const uint32_t hit = (ha[0] & kr[0]) | (ha[1] & kr[1]) | (ha[2] & kr[2]) | (ha[3] & kr[3]);
+                    const uint32_t open = (ha[0] & ~kn[0]) | (ha[1] & ~kn[1]) | (ha[2] & ~kn[2]) | (ha[3] & ~kn[3]);
+                    na = hit != 0;
+                    ra = !na && open == 0;
+                }
+                if (!db) {
+                    const uint32_t hit = (hb[0] & kr[0]) | (hb[1] & kr[1]) | (hb[2] & kr[2]) | (hb[3] & kr[3]);
+                    const uint32_t open = (hb[0] & ~kn[0]) | (hb[1] & ~kn[1]) | (hb[2] & ~kn[2]) | (hb[3] & ~kn[3]);
+                    nb = hit != 0;
+                    rb = !nb && open == 0;
+                }
+                const uint64_t bra = ballot(ra), bna = ballot(na), brb = ballot(rb), bnb = ballot(nb);
+                kr[0] |= uint32_t(bra);
+                kr[1] |= uint32_t(bra >> 32);
+                kr[2] |= uint32_t(brb);
+                kr[3] |= uint32_t(brb >> 32);
+                kn[0] |= uint32_t(bna);
+                kn[1] |= uint32_t(bna >> 32);
+                kn[2] |= uint32_t(bnb);
+                kn[3] |= uint32_t(bnb >> 32);
+                da = da || ra || na;
+                db = db || rb || nb;
+                if (ballot(!da || !db) == 0) break;
+            }
             if (lane == 0) {
-                c.cnt[row] = nR;
-                c.e_cnt[row] = ne;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) s_R[w] = kr[w];
+                s_nl = 0;
             }
+        }
+        __syncthreads();
+
+        // ---- recommendations: lane per entry
+        const uint32_t i = tid;
+        uint32_t rec[4] = {0, 0, 0, 0};
+        if (i < n && i > 0) {
+            uint32_t hr[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t rb = s_R[w] & word_below(i, w);
+                hr[w] = s_H[i * 4 + w] & rb;
+                rec[w] = s_C[i * 4 + w] & rb;
+            }
+            int jstar = -1;
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+                if (hr[w]) jstar = 32 * w + 31 - __clz(hr[w]);
+            if (jstar >= 0)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) rec[w] &= ~word_below(uint32_t(jstar), w);
+        }
+        const bool has_rec = (rec[0] | rec[1] | rec[2] | rec[3]) != 0;
+        if (MFMA) {
+            // Candidates that may be the closest: interval low end at or below
+            // the smallest high end.  Those not yet exact go to the FP32 chain.
+            if (has_rec) {
+                const float ni = s_norm[i];
+                float m_hi = INFINITY;
+                for (int w = 0; w < 4; ++w)
+                    for (uint32_t m = rec[w]; m; m &= m - 1) {
+                        const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
+                        float lo = s_P[p], hi = lo;
+                        if (!is_exact(p)) approx_interval(c.metric, kb, s_P[p], ni, s_norm[r], lo, hi);
+                        m_hi = fminf(m_hi, hi);
+                    }
+                for (int w = 0; w < 4; ++w)
+                    for (uint32_t m = rec[w]; m; m &= m - 1) {
+                        const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
+                        if (is_exact(p)) continue;
+                        float lo, hi;
+                        approx_interval(c.metric, kb, s_P[p], ni, s_norm[r], lo, hi);
+                        if (!(lo <= m_hi) && m_hi == m_hi && lo == lo) continue;
+                        const uint32_t slot = atomicAdd(&s_nl, 1u);
+                        if (slot < c.list_cap) {
+                            s_list[slot] = uint16_t((i << 7) | r);
+                        } else {
+                            s_P[p] = exact_pair(i, r);
+                            mark_exact(p);
+                            exact_pairs++;
+                        }
+                    }
+            }
+            __syncthreads();
+            const uint32_t nl = min(s_nl, c.list_cap);
+            for (uint32_t k = tid; k < nl; k += 256) {
+                const uint32_t a = s_list[k] >> 7, b = s_list[k] & 127u;
+                const uint32_t p = tri_index(a, b);
+                s_P[p] = exact_pair(a, b);
+                mark_exact(p);
+                exact_pairs++;
+            }
+            __syncthreads();
+        }
+        if (has_rec) {
+            // Closest candidate; equal distances go to the later position.  In
+            // the BF16 path only pairs settled by the FP32 chain can win: every
+            // other candidate's value is above the winner's.
+            float best = 0.f;
+            int br = -1;
+            for (int w = 0; w < 4; ++w)
+                for (uint32_t m = rec[w]; m; m &= m - 1) {
+                    const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
+                    if (MFMA && !is_exact(p)) continue;
+                    const float v = s_P[p];
+                    if (br < 0 || v <= best) {
+                        best = v;
+                        br = int(r);
+                    }
+                }
+            const uint32_t target = local_of(c, s_vid[br]);
+            if (target != 0xFFFFFFFFu) emit(c, row, atomicAdd(&s_ne, 1u), target, s_vid[i], best);
+        }
+        // ---- write back the retained entries, all marked old
+        if (i < n && ((s_R[i >> 5] >> (i & 31)) & 1u)) {
+            uint32_t pos = 0;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) pos += __popc(s_R[w] & word_below(i, w));
+            c.rows[size_t(row) * c.cap + pos] = make_uint2(s_vid[i], __float_as_uint(s_dist[i]));
+        }
+        __syncthreads();
+        if (tid == 0) {
+            c.cnt[row] = __popc(s_R[0]) + __popc(s_R[1]) + __popc(s_R[2]) + __popc(s_R[3]);
+            c.e_cnt[row] = s_ne;
         }
         __syncthreads();
     }
-    // Per-block pair count.
-    __shared__ unsigned long long s_pairs;
-    if (tid == 0) s_pairs = 0;
+    __shared__ unsigned long long s_pairs[2];
+    if (tid == 0) s_pairs[0] = s_pairs[1] = 0;
     __syncthreads();
-    atomicAdd(&s_pairs, pairs_done);
+    atomicAdd(&s_pairs[0], exact_pairs);
+    atomicAdd(&s_pairs[1], approx_pairs);
     __syncthreads();
-    if (tid == 0) atomicAdd(c.stats + 1, s_pairs);
+    if (tid == 0) {
+        atomicAdd(c.stats + 1, s_pairs[0]);
+        atomicAdd(c.stats + 2, s_pairs[1]);
+    }
 }
 
 // ------------------------------------------------------------ log bucketing
@@ -464,6 +869,7 @@ struct artea_refine {
     uint32_t* d_l2g = nullptr;
     uint32_t* d_g2l = nullptr;
     int dev_cus = 0;
+    int join = ARTEA_JOIN_FP32;
 };
 
 namespace {
@@ -511,10 +917,20 @@ int do_prefill(artea_refine* r, uint32_t gen, uint32_t lo, uint32_t hi, uint32_t
     return merge_logs(r);
 }
 
-size_t tri_lds(const Ctx& c, bool staged, uint32_t vstride) {
-    const size_t np = size_t(c.cap) * (c.cap - 1) / 2;
-    size_t b = kMaxCap * 16 + (np + 3) / 4 * 4 * 4;
-    if (staged) b += size_t(c.cap) * vstride * 4;
+// Interval half-widths of the BF16 join for dimension D (each a proven
+// bound times 2).  Per coordinate the split keeps |x - hi - lo| <= 2^-16 |x|,
+// so |ab - (ah bh + ah bl + al bh)| <= 3.1 * 2^-16 |a||b|; the 3D products are
+// summed in FP32 (<= 3.03 D u of the absolute sum, u = 2^-23 to allow any
+// rounding of the accumulator); |sum a_k b_k| <= sqrt(N_a N_b) <= (N_a + N_b) / 2.
+// The squared norms carry D u, forming N_a + N_b - 2G 6u, the centring
+// 2^-22, and the FP32 chain itself (D/16 + 6) u of the true value.
+JoinBounds join_bounds(uint32_t D) {
+    const double u = 0x1p-23, d = double(D);
+    const double kg = 3.1 * 0x1p-16 + 3.03 * d * u;
+    JoinBounds b;
+    b.l2 = float(2.0 * (kg + d * u + 6.0 * u + 0x1p-22 + 2.0 * (d / 16.0 + 6.0) * u));
+    b.dot = float(2.0 * (kg + (d / 16.0 + 4.0) * u + d * u));
+    b.cos = float(2.0 * (kg + (d / 16.0 + 4.0) * u + 2.0 * d * u + 8.0 * u));
     return b;
 }
 
@@ -526,14 +942,21 @@ int do_triangle(artea_refine* r, float scale, float shift) {
     int dev = 0, max_lds = 65536;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    bool staged = tri_lds(c, true, vstride) <= size_t(max_lds);
-    const size_t lds = tri_lds(c, staged, vstride);
+    const bool staged = tri_layout(c.cap, c.D, true).total <= uint32_t(max_lds);
+    const size_t lds = tri_layout(c.cap, c.D, staged).total;
     ARTEA_CHECK_ARG(lds <= size_t(max_lds), "triangle: row capacity too large for LDS");
-    if (lds > 65536)
-        ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(triangle_kernel),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL(triangle_kernel, dim3(grid_for(r, 1, 2)), dim3(256), lds, r->s, c, inv, shift_term,
-                       staged ? 1 : 0, vstride);
+    const bool mfma = staged && r->join == ARTEA_JOIN_BF16;
+    const void* fn = mfma ? reinterpret_cast<const void*>(triangle_kernel<true>)
+                          : reinterpret_cast<const void*>(triangle_kernel<false>);
+    if (lds > 65536) ARTEA_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    const JoinBounds kb = join_bounds(c.D);
+    const uint32_t per_cu = std::max<uint32_t>(1u, uint32_t(160u * 1024u / std::max<size_t>(lds, 1)));
+    const dim3 grid(grid_for(r, 1, std::min<uint32_t>(per_cu, 4u)));
+    if (mfma)
+        hipLaunchKernelGGL(triangle_kernel<true>, grid, dim3(256), lds, r->s, c, inv, shift_term, 1, vstride, kb);
+    else
+        hipLaunchKernelGGL(triangle_kernel<false>, grid, dim3(256), lds, r->s, c, inv, shift_term, staged ? 1 : 0,
+                           vstride, kb);
     ARTEA_HIP_TRY(hipGetLastError());
     return merge_logs(r);
 }
@@ -577,6 +1000,7 @@ int refine_make(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int me
     c.metric = metric;
     c.cap = cap;
     c.n = l2g ? n_local : n_global;
+    c.list_cap = kListCap;
     r->seed = seed;
     const size_t slots = std::max<size_t>(size_t(c.n) * cap, 1);
     auto fail = [&](hipError_t e) {
@@ -744,22 +1168,42 @@ int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_i
     return rc;
 }
 
-int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* pairs) {
+int artea_refine_set_join(artea_refine* r, int precision) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_set_join: null handle");
+    ARTEA_CHECK_ARG(precision == ARTEA_JOIN_FP32 || precision == ARTEA_JOIN_BF16,
+                    "artea_refine_set_join: precision must be ARTEA_JOIN_FP32 or ARTEA_JOIN_BF16");
+    r->join = precision;
+    return ARTEA_OK;
+}
+
+int artea_refine_set_recheck_queue(artea_refine* r, uint32_t entries) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_set_recheck_queue: null handle");
+    ARTEA_CHECK_ARG(entries <= kListCap, "artea_refine_set_recheck_queue: at most 1024 entries");
+    r->c.list_cap = entries;
+    return ARTEA_OK;
+}
+
+int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pairs, uint64_t* approx_pairs) {
     clear_error();
     ARTEA_CHECK_ARG(r, "artea_refine_last_stats: null handle");
     unsigned long long h[4];
     ARTEA_HIP_TRY(hipMemcpy(h, r->c.stats, sizeof h, hipMemcpyDeviceToHost));
     if (logs) *logs = h[0];
-    if (pairs) *pairs = h[1];
+    if (exact_pairs) *exact_pairs = h[1];
+    if (approx_pairs) *approx_pairs = h[2];
     ARTEA_HIP_TRY(hipMemset(r->c.stats, 0, sizeof h));
     return ARTEA_OK;
 }
 
 int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
-                     float l0_min_distance, uint64_t seed, double* seconds_per_level) {
+                     float l0_min_distance, uint64_t seed, int join_precision, double* seconds_per_level) {
     clear_error();
     ARTEA_CHECK_ARG(idx, "artea_build_full: null index");
+    ARTEA_CHECK_ARG(join_precision == ARTEA_JOIN_FP32 || join_precision == ARTEA_JOIN_BF16,
+                    "artea_build_full: join_precision must be ARTEA_JOIN_FP32 or ARTEA_JOIN_BF16");
     ARTEA_CHECK_ARG(max_level < 64, "artea_build_full: max_level must be < 64");
     ARTEA_CHECK_ARG(uint32_t(float(bl_max) * 1.5f) <= kMaxCap && uint32_t(float(ul_max) * 1.5f) <= kMaxCap,
                     "artea_build_full: refine caps (1.5 x max) must be <= 128");
@@ -784,6 +1228,7 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
         int rc = refine_make(idx->d_vecs, n, idx->dim_pad, bmetric, cap, nullptr, n, seed, &r);
         if (rc) return rc;
         std::unique_ptr<artea_refine, int (*)(artea_refine*)> guard(r, artea_refine_destroy);
+        r->join = join_precision;
         const uint32_t gen = uint32_t(float(cap) * prefill_ratio);
         if (gen > 0) {
             rc = do_prefill(r, gen, 0, n, 0, n, l);

@@ -11,6 +11,7 @@ import numpy as np
 from ._lib import check, lib
 
 NBR_NEW = 0x80000000
+JOIN_FP32, JOIN_BF16 = 0, 1
 
 
 def _ptr(a):
@@ -25,7 +26,7 @@ class PropagateEngine:
     by merge_logs (propagate_engine.hpp:200-225)."""
 
     def __init__(self, index, cap: int, metric: Optional[int] = None, local_to_global: Optional[Sequence[int]] = None,
-                 seed: int = 0):
+                 seed: int = 0, join: int = JOIN_FP32):
         m = index.metric if metric is None else metric
         h = C.c_void_p()
         l2g = None if local_to_global is None else np.ascontiguousarray(local_to_global, np.uint32)
@@ -36,6 +37,16 @@ class PropagateEngine:
                                         _ptr(l2g) if l2g is not None else None, self.n_local, seed, C.byref(h)))
         self.h = h
         self.index = index
+        self.set_join(join)
+
+    def set_join(self, precision: int):
+        """JOIN_FP32: every needed pair by the FP32 chain; JOIN_BF16: MFMA
+        Gram tiles with FP32 rechecks (identical rows and logs)."""
+        check(lib().artea_refine_set_join(self.h, precision))
+
+    def set_recheck_queue(self, entries: int):
+        """Per-row FP32 recheck queue of the BF16 join (0..1024)."""
+        check(lib().artea_refine_set_recheck_queue(self.h, entries))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -73,9 +84,9 @@ class PropagateEngine:
         check(lib().artea_refine_schedule(self.h, build_loops, triu_iters, scale_coeffs, effective_shift))
 
     def last_stats(self):
-        a, b = C.c_uint64(), C.c_uint64()
-        check(lib().artea_refine_last_stats(self.h, C.byref(a), C.byref(b)))
-        return {"logs": a.value, "pair_distances": b.value}
+        a, b, d = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().artea_refine_last_stats(self.h, C.byref(a), C.byref(b), C.byref(d)))
+        return {"logs": a.value, "pair_distances": b.value, "approx_pair_distances": d.value}
 
 
 def max_level_for(n: int) -> int:
@@ -89,11 +100,12 @@ def max_level_for(n: int) -> int:
 
 
 def build_full(index, ul_max=32, bl_max=64, max_level=None, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-               shift=1.5, l0_min_distance=1.0, seed=0):
+               shift=1.5, l0_min_distance=1.0, seed=0, join=JOIN_FP32):
     """Full-hierarchy build on the GPU; attaches the graph to ``index``.
-    Returns seconds per level."""
+    ``join`` picks the triangle join's pair-distance path (the graph is the
+    same either way).  Returns seconds per level."""
     ml = max_level_for(index.n) if max_level is None else max_level
     secs = np.zeros(ml + 1, np.float64)
     check(lib().artea_build_full(index.h, ul_max, bl_max, ml, build_loops, triu_iters, prefill, scale, shift,
-                                 l0_min_distance, seed, _ptr(secs)))
+                                 l0_min_distance, seed, join, _ptr(secs)))
     return secs

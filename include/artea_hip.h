@@ -196,8 +196,24 @@ int artea_refine_reverse(artea_refine* r);
  * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */
 int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_iters, float scale_coeffs,
                           float effective_shift);
-/* Log records merged and pair distances evaluated since the last call. */
-int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* pair_distances);
+/* Precision of the triangle join's pair distances (no reference
+ * counterpart; the reference's join is FP32 only).  ARTEA_JOIN_FP32 (the
+ * default): every needed pair by dist_func_t's FP32 chain.  ARTEA_JOIN_BF16:
+ * the row's Gram matrix on BF16 MFMA tiles (two-term split, FP32
+ * accumulation) places every pair's distance in a proven interval; pairs whose
+ * interval straddles a pruning or recommendation threshold, and candidates
+ * that may be the recommendation target, are recomputed by the FP32 chain, so
+ * rows and logs are identical to ARTEA_JOIN_FP32's.  Falls back to FP32 when
+ * the row's vectors do not fit in LDS. */
+enum artea_join_precision { ARTEA_JOIN_FP32 = 0, ARTEA_JOIN_BF16 = 1 };
+int artea_refine_set_join(artea_refine* r, int precision);
+/* Entries of the BF16 join's per-row queue of pairs to recompute in FP32
+ * (0..1024, default 1024).  Pairs beyond it are found again from a bitmap:
+ * slower, same result.  A tuning knob; no reference counterpart. */
+int artea_refine_set_recheck_queue(artea_refine* r, uint32_t entries);
+/* Since the last call: log records merged, pair distances evaluated by the
+ * FP32 chain, pair distances evaluated on the BF16 tiles. */
+int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pairs, uint64_t* approx_pairs);
 
 /* Full-hierarchy build on the GPU, attached to idx: every vertex on levels
  * 0..max_level (the r-net hierarchy when no vertex is absorbed,
@@ -207,10 +223,11 @@ int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* pair_dist
  * L0 and 0 above; rows written back truncated to bl_max / ul_max and
  * compacted with the FP64-centroid entry point
  * (compactor/hierarchical_graph_compactor.hpp:118-327).  Build metric is L2
- * for an L2SQR index.  seconds_per_level (optional) gets max_level+1 values. */
+ * for an L2SQR index; join_precision is an artea_join_precision.
+ * seconds_per_level (optional) gets max_level+1 values. */
 int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
-                     float l0_min_distance, uint64_t seed, double* seconds_per_level);
+                     float l0_min_distance, uint64_t seed, int join_precision, double* seconds_per_level);
 
 /* -------------------------------------------------------------- utilities */
 /* dist_func_t on the device (utils/simd_distance.hpp:64-79): dists[i*nb+j]

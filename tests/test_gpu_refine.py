@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import artea_amd as aa
-from artea_amd.refine import PropagateEngine, build_full
+from artea_amd.refine import JOIN_BF16, JOIN_FP32, PropagateEngine, build_full
 
 pytestmark = pytest.mark.gpu
 
@@ -47,17 +47,19 @@ def upload_single_row(eng, n, row, pairs):
     eng.upload(counts, ent)
 
 
-def test_triangle_kats_on_device(orc):
+@pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
+def test_triangle_kats_on_device(orc, join):
     # test_propagate_engine.cpp:235-318.
     v, idx = grid_index()
     dist = lambda a, b: float(orc.distance(orc.L2, v[a], v[b]))
-    eng = PropagateEngine(idx, cap=8)
+    PE = lambda index, cap: PropagateEngine(index, cap=cap, join=join)
+    eng = PE(idx, cap=8)
     upload_single_row(eng, 12, 0, [(u, dist(0, u)) for u in (1, 8, 10)])
     eng.next_triangle(1.0, 0.0)
     rows = rows_of(eng)
     assert [e[0] for e in rows[0]] == [1, 8]
     assert rows[8] == [(10 | 0x80000000, int(np.float32(2.0).view(np.uint32)))] and rows[1] == []
-    eng = PropagateEngine(idx, cap=8)
+    eng = PE(idx, cap=8)
     upload_single_row(eng, 12, 0, [(u, dist(0, u)) for u in (1, 8, 11)])
     eng.next_triangle(2.0, 0.0)
     rows = rows_of(eng)
@@ -65,7 +67,7 @@ def test_triangle_kats_on_device(orc):
     assert rows[1] == [(11 | 0x80000000, int(np.float32(np.sqrt(np.float32(8.0))).view(np.uint32)))]
     for alpha, shift, retain in [(2.0, 0.0, False), (2.0, 0.2, False), (2.0, 0.4, True), (2.0, 1.0 / 3.0, True),
                                  (1.0, 1.25, True), (1.0, 1.5, True), (1.0, 2.0, True)]:
-        eng = PropagateEngine(idx, cap=8)
+        eng = PE(idx, cap=8)
         upload_single_row(eng, 12, 0, [(u, dist(0, u)) for u in (2, 3)])
         eng.next_triangle(alpha, shift)
         rows = rows_of(eng)
@@ -73,13 +75,17 @@ def test_triangle_kats_on_device(orc):
         assert [e[0] & 0x7FFFFFFF for e in rows[2]] == [3]
 
 
-@pytest.mark.parametrize("metric,kind,dim,cap,shift", [(3, 0, 128, 96, 1.5), (3, 0, 128, 48, 0.0),
-                                                      (2, 1, 100, 96, 0.0), (3, 2, 96, 96, 1.5)])
-def test_layer_schedule_parity(orc, metric, kind, dim, cap, shift):
+LAYER_CASES = [(3, 0, 128, 96, 1.5), (3, 0, 128, 48, 0.0), (2, 1, 100, 96, 0.0), (3, 2, 96, 96, 1.5),
+               (0, 2, 96, 128, 0.0), (1, 1, 100, 64, 0.0)]
+
+
+@pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
+@pytest.mark.parametrize("metric,kind,dim,cap,shift", LAYER_CASES)
+def test_layer_schedule_parity(orc, metric, kind, dim, cap, shift, join):
     n = 2500
     vecs = aa.generate(kind, 7, 0, n, dim)
     idx = aa.DeviceIndex(vecs, metric)
-    eng = PropagateEngine(idx, cap=cap, seed=99)
+    eng = PropagateEngine(idx, cap=cap, seed=99, join=join)
     L = orc.Layer(vecs, metric, cap)
     g = int(np.float32(cap) * np.float32(0.4))
     eng.next_prefill(g, 0, n, stream=0)
@@ -99,11 +105,51 @@ def test_layer_schedule_parity(orc, metric, kind, dim, cap, shift):
         assert rows_of(eng) == orc_rows(L), "loop %d reverse" % b
 
 
-def test_full_build_parity(orc):
+def test_bf16_join_settles_few_pairs_in_fp32():
+    # Same rows as the FP32 join (checked above); here the work split: the
+    # tiles evaluate every needed pair, the FP32 chain a small share of them.
+    n = 3000
+    vecs = aa.generate(2, 5, 0, n, 96)
+    idx = aa.DeviceIndex(vecs, 3)
+    out = {}
+    for join in (JOIN_FP32, JOIN_BF16):
+        eng = PropagateEngine(idx, cap=96, seed=4, join=join)
+        eng.next_prefill(38, 0, n, stream=0)
+        eng.last_stats()
+        for _ in range(3):
+            eng.next_triangle(1.10, 1.5)
+        out[join] = (rows_of(eng), eng.last_stats())
+    assert out[JOIN_FP32][0] == out[JOIN_BF16][0]
+    fp, bf = out[JOIN_FP32][1], out[JOIN_BF16][1]
+    assert fp["approx_pair_distances"] == 0 and bf["approx_pair_distances"] == fp["pair_distances"]
+    assert bf["logs"] == fp["logs"]
+    assert bf["pair_distances"] < 0.25 * fp["pair_distances"], (bf, fp)
+
+
+@pytest.mark.parametrize("queue", [0, 7])
+def test_bf16_join_recheck_queue_overflow(orc, queue):
+    # Pairs beyond the recheck queue are settled from the overflow bitmap.
+    n = 2000
+    vecs = aa.generate(0, 8, 0, n, 128)
+    idx = aa.DeviceIndex(vecs, 3)
+    eng = PropagateEngine(idx, cap=96, seed=6, join=JOIN_BF16)
+    eng.set_recheck_queue(queue)
+    L = orc.Layer(vecs, 3, 96)
+    eng.next_prefill(38, 0, n, stream=0)
+    L.prefill(38, 0, n, seed=6, stream=0, nthreads=8)
+    for _ in range(3):
+        eng.next_triangle(1.10, 1.5)
+        L.triangle(1.10, 1.5, nthreads=8)
+        L.truncate(nthreads=8)
+        assert rows_of(eng) == orc_rows(L)
+
+
+@pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
+def test_full_build_parity(orc, join):
     n = 3000
     vecs = aa.generate(0, 13, 0, n, 128)
     idx = aa.DeviceIndex(vecs, 0)
-    build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=17)
+    build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=17, join=join)
     g, _ = orc.build_full(vecs, metric=orc.L2, max_level_=2, build_loops=4, triu_iters=2, seed=17, nthreads=8)
     info = idx.info()
     assert info["top"] == g.top and info["entry"] == g.entry
