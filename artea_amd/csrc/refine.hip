@@ -83,60 +83,6 @@ __device__ __forceinline__ void emit(const Ctx& c, uint32_t src, uint32_t slot, 
     atomicAdd(c.t_cnt + target, 1u);
 }
 
-// Single-thread dist_func_t (all 16 lanes of the AVX-512 chain in registers).
-__device__ __forceinline__ float thread_distance(const float* __restrict__ a, const float* __restrict__ b, int C,
-                                                 int metric) {
-    float acc[16], nb[16], na[16];
-#pragma unroll
-    for (int l = 0; l < 16; ++l) acc[l] = nb[l] = na[l] = 0.f;
-    for (int ch = 0; ch < C; ++ch) {
-#pragma unroll
-        for (int l4 = 0; l4 < 4; ++l4) {
-            const float4 x = *reinterpret_cast<const float4*>(a + ch * 16 + l4 * 4);
-            const float4 y = *reinterpret_cast<const float4*>(b + ch * 16 + l4 * 4);
-            if (metric == kL2Sqr || metric == kL2) {
-                const float d0 = x.x - y.x, d1 = x.y - y.y, d2 = x.z - y.z, d3 = x.w - y.w;
-                acc[l4 * 4 + 0] = __builtin_fmaf(d0, d0, acc[l4 * 4 + 0]);
-                acc[l4 * 4 + 1] = __builtin_fmaf(d1, d1, acc[l4 * 4 + 1]);
-                acc[l4 * 4 + 2] = __builtin_fmaf(d2, d2, acc[l4 * 4 + 2]);
-                acc[l4 * 4 + 3] = __builtin_fmaf(d3, d3, acc[l4 * 4 + 3]);
-            } else {
-                acc[l4 * 4 + 0] = __builtin_fmaf(x.x, y.x, acc[l4 * 4 + 0]);
-                acc[l4 * 4 + 1] = __builtin_fmaf(x.y, y.y, acc[l4 * 4 + 1]);
-                acc[l4 * 4 + 2] = __builtin_fmaf(x.z, y.z, acc[l4 * 4 + 2]);
-                acc[l4 * 4 + 3] = __builtin_fmaf(x.w, y.w, acc[l4 * 4 + 3]);
-                if (metric == kCosine) {
-                    na[l4 * 4 + 0] = __builtin_fmaf(x.x, x.x, na[l4 * 4 + 0]);
-                    na[l4 * 4 + 1] = __builtin_fmaf(x.y, x.y, na[l4 * 4 + 1]);
-                    na[l4 * 4 + 2] = __builtin_fmaf(x.z, x.z, na[l4 * 4 + 2]);
-                    na[l4 * 4 + 3] = __builtin_fmaf(x.w, x.w, na[l4 * 4 + 3]);
-                    nb[l4 * 4 + 0] = __builtin_fmaf(y.x, y.x, nb[l4 * 4 + 0]);
-                    nb[l4 * 4 + 1] = __builtin_fmaf(y.y, y.y, nb[l4 * 4 + 1]);
-                    nb[l4 * 4 + 2] = __builtin_fmaf(y.z, y.z, nb[l4 * 4 + 2]);
-                    nb[l4 * 4 + 3] = __builtin_fmaf(y.w, y.w, nb[l4 * 4 + 3]);
-                }
-            }
-        }
-    }
-    auto red = [](const float* v) {
-        float t8[8], t4[4];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) t8[i] = v[i + 8] + v[i];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) t4[i] = t8[i + 4] + t8[i];
-        const float x0 = t4[0] + t4[2];
-        const float x1 = t4[1] + t4[3];
-        return x0 + x1;
-    };
-    const float s = red(acc);
-    if (metric == kL2Sqr) return s;
-    if (metric == kL2) return __builtin_sqrtf(s);
-    if (metric == kDot) return -s;
-    const float sq1 = red(na), sq2 = red(nb);
-    const float denom = __builtin_sqrtf(sq1 * sq2);
-    return denom > 0.0f ? 1.0f - s / denom : 1.0f;
-}
-
 // ---------------------------------------------------------------- prefill
 // RandomUpdater::update_impl (refiner/updaters/random_updater.hpp:97-142),
 // one wavefront per row of [row_begin, row_end).
@@ -226,57 +172,63 @@ __global__ void reverse_kernel(Ctx c) {
 //  * FP32: every needed pair by the FP32 chain of dist_func_t;
 //  * BF16: the row's Gram matrix in 32x32 tiles on MFMA (two-term BF16 split
 //    of each coordinate, FP32 accumulation; L2 vectors are first centred on
-//    the row's own vertex) gives every pair's distance inside a proven
-//    interval.  A pair whose interval straddles prune_i or d_i, and every
-//    candidate that may be the recommendation target, is recomputed by the
-//    FP32 chain, so every decision and every emitted distance equals the
-//    FP32 path's.
+//    the row's own vertex) gives every pair a "key" (the squared distance for
+//    L2, the metric value otherwise) inside a proven interval.  A pair whose
+//    interval straddles prune_i or d_i, and every candidate that may be the
+//    recommendation target, is recomputed by the FP32 chain, so every
+//    decision and every emitted distance equals the FP32 path's.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t kListCap = 1024;  // FP32 recheck queue per row; overflow goes to a bitmap
+constexpr int kStageBatch = 8;      // row-vector gathers in flight per thread
+constexpr uint32_t kListCap = 768;  // FP32 recheck queue per row; overflow goes to a bitmap
 
-// Half-widths of the BF16 intervals, per metric (derived on the host from D).
+// Half-widths of the BF16 key intervals (derived on the host from D).
 struct JoinBounds {
-    float l2;   // x (N_i + N_j), centred squared norms, on the squared distance
-    float dot;  // x sqrt(N_i N_j)
+    float l2;   // x (N_i + N_j) on the squared distance, N = centred squared norms
+    float dot;  // x sqrt(N_i) sqrt(N_j)
     float cos;  // absolute
 };
 
 struct TriLayout {
-    uint32_t vid, dist, norm, H, Cw, X, pend, list, center, P, vec, total;
+    uint32_t vid, dist, norm, rn, thr, best, H, Cw, X, pend, list, P, vec, vstride, total;
 };
 
 __host__ __device__ inline uint32_t align16u(uint32_t x) { return (x + 15u) & ~15u; }
 
-__host__ __device__ inline TriLayout tri_layout(uint32_t cap, uint32_t D, bool staged) {
+// LDS of one row: entries up to cap (rounded to the 32-row tile), the pair
+// triangle, the bitmasks, the BF16 path's per-entry data and recheck queue,
+// and the row's vectors (FP32, or BF16 hi/lo at one dword per coordinate).
+// Row stride: the MFMA operand reads (32 rows, 16 B each) want rows 4 banks
+// apart, the FP32 chain's quad reads (16 consecutive dwords per vector) want
+// rows 16 banks apart.
+__host__ __device__ inline TriLayout tri_layout(uint32_t cap, uint32_t D, bool staged, bool mfma) {
     TriLayout L;
+    const uint32_t capr = (cap + 31) / 32 * 32;
     const uint32_t np = cap * (cap - 1) / 2;
+    const uint32_t xw = align16u((np + 31) / 32 * 4);
+    L.vstride = mfma ? D + 4 : ((D % 64 == 16 || D % 64 == 48) ? D : D + 16);
     uint32_t o = 0;
-    L.vid = o;
-    o += kMaxCap * 4;
-    L.dist = o;
-    o += kMaxCap * 4;
-    L.norm = o;
-    o += kMaxCap * 4;
-    L.H = o;
-    o += kMaxCap * 16;
-    L.Cw = o;
-    o += kMaxCap * 16;
-    L.X = o;
-    o += align16u((np + 31) / 32 * 4);
-    L.pend = o;
-    o += align16u((np + 31) / 32 * 4);
-    L.list = o;
-    o += kListCap * 2;
-    L.center = o;
-    o += align16u(D * 4);
-    L.P = o;
-    o += align16u(np * 4);
-    L.vec = o;
-    if (staged) o += (cap + 31) / 32 * 32 * (D + 4) * 4;
+    auto take = [&](uint32_t& field, uint32_t bytes) {
+        field = o;
+        o += align16u(bytes);
+    };
+    take(L.vid, capr * 4);
+    take(L.dist, capr * 4);
+    take(L.H, capr * 16);
+    take(L.Cw, capr * 16);
+    take(L.P, np * 4);
+    const uint32_t m = mfma ? 1u : 0u;
+    take(L.norm, m * capr * 4);
+    take(L.rn, m * capr * 4);
+    take(L.thr, m * capr * 16);
+    take(L.best, m * capr * 8);
+    take(L.X, m * xw);
+    take(L.pend, m * xw);
+    take(L.list, m * kListCap * 2);
+    take(L.vec, staged ? capr * L.vstride * 4 : 0u);
     L.total = o;
     return L;
 }
@@ -305,80 +257,144 @@ __device__ __forceinline__ void split8(float4 x0, float4 x1, bf16x8& hi, bf16x8&
     }
 }
 
-// Metric value from the approximate Gram entry (centred norms for L2).
-__device__ __forceinline__ float approx_metric(int metric, float g, float ni, float nj) {
-    if (metric == kL2Sqr) return (ni + nj) - 2.0f * g;
-    if (metric == kL2) return __builtin_sqrtf(fmaxf((ni + nj) - 2.0f * g, 0.0f));
-    if (metric == kDot) return -g;
-    const float den = __builtin_sqrtf(ni * nj);
-    return den > 0.0f ? 1.0f - g / den : 1.0f;
+// Threshold x in the key domain as (certainly below, certainly at or above)
+// bounds: key_hi < first implies dist < x, key_lo >= second implies
+// dist >= x.  For L2 the FP32 chain's sqrt is correctly rounded, so squared
+// keys below x^2 (1 - 2^-20) give distances below x and keys at or above
+// x^2 (1 + 2^-20) give distances at or above x; distances are never negative.
+__device__ __forceinline__ float2 key_threshold(int metric, float x) {
+    if (metric != kL2) return make_float2(x, x);
+    if (!(x > 0.0f)) return make_float2(-INFINITY, -INFINITY);
+    const float x2 = x * x;
+    return make_float2(x2 * (1.0f - 0x1p-20f), x2 * (1.0f + 0x1p-20f));
 }
 
-// Interval certain to hold the FP32-chain value of a pair whose approximate
-// value is t.  Non-finite inputs give NaN or infinite ends, which make every
-// decision on the pair ambiguous.
-__device__ __forceinline__ void approx_interval(int metric, const JoinBounds& kb, float t, float ni, float nj,
-                                                float& lo, float& hi) {
-    if (metric == kL2Sqr) {
-        const float e = kb.l2 * (ni + nj);
-        lo = t - e;
-        hi = t + e;
-    } else if (metric == kL2) {
-        const float t2 = t * t;
-        const float e = kb.l2 * (ni + nj) + 0x1p-21f * t2;
-        lo = __builtin_sqrtf(fmaxf(t2 - e, 0.0f)) * (1.0f - 0x1p-21f);
-        hi = __builtin_sqrtf(t2 + e) * (1.0f + 0x1p-21f);
-    } else if (metric == kDot) {
-        const float e = kb.dot * __builtin_sqrtf(ni * nj);
-        lo = t - e;
-        hi = t + e;
+// Interval of the key of an FP32-chain distance v.
+__device__ __forceinline__ void exact_key(int metric, float v, float& lo, float& hi) {
+    if (metric == kL2) {
+        const float v2 = v * v;
+        lo = v2 * (1.0f - 0x1p-20f);
+        hi = v2 * (1.0f + 0x1p-20f);
     } else {
-        if (!(ni * nj > 0.0f)) {
-            lo = -INFINITY;
-            hi = INFINITY;
-        } else {
-            lo = t - kb.cos;
-            hi = t + kb.cos;
-        }
+        lo = hi = v;
     }
 }
 
-template <bool MFMA>
-__global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, float shift_term, int staged,
-                                                       uint32_t vstride, JoinBounds kb) {
+// Half-width of an approximate key; rn = sqrt(N) (dot) or 1/sqrt(N) (cosine).
+__device__ __forceinline__ float key_radius(int metric, const JoinBounds& kb, float ni, float nj, float rni,
+                                            float rnj) {
+    if (metric == kL2Sqr || metric == kL2) return kb.l2 * (ni + nj);
+    if (metric == kDot) return kb.dot * (rni * rnj);
+    return kb.cos;
+}
+
+// Recommendation target order: smaller distance first, then the later
+// position (-0 and +0 are equal, as the FP32 comparisons see them).
+__device__ __forceinline__ unsigned long long target_key(float d, uint32_t r) {
+    uint32_t u = __float_as_uint(d == 0.0f ? 0.0f : d);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return (static_cast<unsigned long long>(u) << 32) | (0xFFu - r);
+}
+
+template <bool MFMA, int M>
+__global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale, float shift_term, int staged,
+                                                          JoinBounds kb) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const TriLayout L = tri_layout(c.cap, c.D, staged != 0);
+    const TriLayout L = tri_layout(c.cap, c.D, staged != 0, MFMA);
+    const uint32_t vstride = L.vstride;
     uint32_t* s_vid = reinterpret_cast<uint32_t*>(smem + L.vid);
     float* s_dist = reinterpret_cast<float*>(smem + L.dist);
     float* s_norm = reinterpret_cast<float*>(smem + L.norm);
+    float* s_rn = reinterpret_cast<float*>(smem + L.rn);
+    float4* s_thr = reinterpret_cast<float4*>(smem + L.thr);
+    unsigned long long* s_best = reinterpret_cast<unsigned long long*>(smem + L.best);
     uint32_t* s_H = reinterpret_cast<uint32_t*>(smem + L.H);
     uint32_t* s_C = reinterpret_cast<uint32_t*>(smem + L.Cw);
     uint32_t* s_X = reinterpret_cast<uint32_t*>(smem + L.X);
     uint32_t* s_pend = reinterpret_cast<uint32_t*>(smem + L.pend);
     uint16_t* s_list = reinterpret_cast<uint16_t*>(smem + L.list);
-    float* s_cen = reinterpret_cast<float*>(smem + L.center);
     float* s_P = reinterpret_cast<float*>(smem + L.P);
-    float* s_vec = reinterpret_cast<float*>(smem + L.vec);
+    float* s_vec = reinterpret_cast<float*>(smem + L.vec);       // FP32 path: staged vectors
+    uint32_t* s_split = reinterpret_cast<uint32_t*>(smem + L.vec);  // BF16 path: split coordinates
     __shared__ uint32_t s_newm[4], s_R[4], s_nl, s_ne;
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
-    const bool centred = MFMA && (c.metric == kL2Sqr || c.metric == kL2);
+    constexpr int metric = M;  // the build metric, fixed per instantiation
+    const bool centred = MFMA && (metric == kL2Sqr || metric == kL2);
+    const uint32_t capr = (c.cap + 31) / 32 * 32;
     const uint32_t xwords = (c.cap * (c.cap - 1) / 2 + 31) / 32;
     unsigned long long exact_pairs = 0, approx_pairs = 0;
 
+    // FP32 coordinates of entry k: LDS when the FP32 path staged them, else
+    // global memory (the BF16 path's rechecks: a small share of the pairs).
     auto vec_of = [&](uint32_t k) -> const float* {
-        return staged ? s_vec + k * vstride : c.vecs + size_t(s_vid[k]) * c.D;
+        return (staged && !MFMA) ? s_vec + k * vstride : c.vecs + size_t(s_vid[k]) * c.D;
     };
-    auto is_new = [&](uint32_t k) -> bool { return (s_newm[k >> 5] >> (k & 31)) & 1u; };
+    uint32_t newm[4] = {0, 0, 0, 0};
+    auto is_new = [&](uint32_t k) -> bool { return (newm[k >> 5] >> (k & 31)) & 1u; };
     auto prune_of = [&](float d) -> float { return d * inv_scale - shift_term; };
-    auto exact_pair = [&](uint32_t i, uint32_t j) -> float {
-        return thread_distance(vec_of(i), vec_of(j), int(c.C), c.metric);
+    // dist_func_t's FP32 chain for one pair, by a quad of lanes (all four
+    // lanes active, same pair).
+    auto exact_pair_quad = [&](uint32_t i, uint32_t j) -> float {
+        const uint32_t sub = lane & 3u;
+        const float qn2 = metric == kCosine ? quad_norm2(vec_of(i), int(c.C), sub) : 0.f;
+        return quad_distance<0>(vec_of(i), vec_of(j), int(c.C), metric, sub, qn2);
     };
     auto mark_exact = [&](uint32_t p) { atomicOr(s_X + (p >> 5), 1u << (p & 31)); };
     auto is_exact = [&](uint32_t p) -> bool { return (s_X[p >> 5] >> (p & 31)) & 1u; };
+    // Queue of pairs for the FP32 chain; beyond list_cap entries, a bitmap.
+    auto push_recheck = [&](uint32_t i, uint32_t j) {
+        const uint32_t slot = atomicAdd(&s_nl, 1u);
+        if (slot < c.list_cap) {
+            s_list[slot] = uint16_t((i << 7) | j);
+        } else {
+            const uint32_t p = tri_index(i, j);
+            atomicOr(s_pend + (p >> 5), 1u << (p & 31));
+        }
+    };
+    // Runs the queued pairs through the FP32 chain, a quad of lanes per pair,
+    // and hands each result to settle(i, j, value) on the quad's first lane.
+    // The overflow bitmap is cleared as it is drained.
+    auto drain = [&](auto&& settle) {
+        const uint32_t total = s_nl, nl = min(total, c.list_cap);
+        for (uint32_t base = 0; base < nl; base += 64) {
+            const uint32_t k = base + (tid >> 2);
+            if (k < nl) {
+                const uint32_t i = s_list[k] >> 7, j = s_list[k] & 127u;
+                const float te = exact_pair_quad(i, j);
+                if ((tid & 3u) == 0) settle(i, j, te);
+            }
+        }
+        if (total > c.list_cap)
+            for (uint32_t w = tid >> 2; w < xwords; w += 64) {
+                const uint32_t bits = s_pend[w];
+                for (uint32_t m = bits; m; m &= m - 1) {
+                    const uint32_t p = 32 * w + __ffs(m) - 1;
+                    uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
+                    while (i * (i - 1) / 2 > p) --i;
+                    while ((i + 1) * i / 2 <= p) ++i;
+                    const uint32_t j = p - i * (i - 1) / 2;
+                    const float te = exact_pair_quad(i, j);
+                    if ((tid & 3u) == 0) settle(i, j, te);
+                }
+                if (bits && (tid & 3u) == 0) s_pend[w] = 0;
+            }
+    };
 
+    // The next row's count and entries are loaded while this row is processed.
+    uint32_t n_pf = 0;
+    uint2 e_pf = make_uint2(0u, 0u);
+    auto prefetch = [&](uint32_t r) {
+        if (r < c.n) {
+            n_pf = c.cnt[r];
+            if (tid < c.cap) e_pf = c.rows[size_t(r) * c.cap + tid];
+        }
+    };
+    prefetch(blockIdx.x);
     for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
-        const uint32_t n = c.cnt[row];
+        const uint32_t n = n_pf;
+        const uint2 e_row = e_pf;
+        prefetch(row + gridDim.x);
         if (n == 0) {
             if (tid == 0) c.e_cnt[row] = 0;
             continue;  // uniform; LDS untouched
@@ -387,21 +403,21 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
         if (tid < 128) {
             bool nw = false;
             if (tid < n) {
-                const uint2 e = c.rows[size_t(row) * c.cap + tid];
-                s_vid[tid] = e.x & kMask;
-                s_dist[tid] = __uint_as_float(e.y);
-                nw = (e.x & kNew) != 0;
+                s_vid[tid] = e_row.x & kMask;
+                s_dist[tid] = __uint_as_float(e_row.y);
+                nw = (e_row.x & kNew) != 0;
             }
             const uint64_t b = ballot(nw);
             if (lane == 0) {
                 s_newm[2 * wid] = uint32_t(b);
                 s_newm[2 * wid + 1] = uint32_t(b >> 32);
             }
+            if (tid < capr)
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                s_H[tid * 4 + w] = 0;
-                s_C[tid * 4 + w] = 0;
-            }
+                for (int w = 0; w < 4; ++w) {
+                    s_H[tid * 4 + w] = 0;
+                    s_C[tid * 4 + w] = 0;
+                }
         }
         if (MFMA)
             for (uint32_t k = tid; k < xwords; k += 256) {
@@ -415,50 +431,114 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
         __syncthreads();
         const uint32_t T = (n + 31) / 32;
         const uint32_t npad = MFMA ? T * 32 : n;
-        if (staged) {
-            const uint32_t per = c.D / 4;  // float4 per vector
-            for (uint32_t t = tid; t < npad * per; t += 256) {
-                const uint32_t k = t / per, q = t % per;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (k < n) v = *reinterpret_cast<const float4*>(c.vecs + size_t(s_vid[k]) * c.D + q * 4);
-                *reinterpret_cast<float4*>(s_vec + k * vstride + q * 4) = v;
-            }
-            if (centred)
-                for (uint32_t q = tid; q < per; q += 256)
-                    *reinterpret_cast<float4*>(s_cen + q * 4) =
-                        *reinterpret_cast<const float4*>(c.vecs + size_t(storage_of(c, row)) * c.D + q * 4);
-        }
-        __syncthreads();
-
-        // ---- pair values and the H / C bitmasks
+#pragma unroll
+        for (int w = 0; w < 4; ++w) newm[w] = s_newm[w];
         if (MFMA) {
-            // Squared norms of the (centred) coordinates as the tiles see them.
-            for (uint32_t k = tid; k < n; k += 256) {
-                float acc = 0.f;
-                for (uint32_t d = 0; d < c.D; d += 4) {
-                    float4 x = *reinterpret_cast<const float4*>(s_vec + k * vstride + d);
-                    if (centred) {
-                        const float4 m = *reinterpret_cast<const float4*>(s_cen + d);
-                        x = make_float4(x.x - m.x, x.y - m.y, x.z - m.z, x.w - m.w);
+            // Split staging: a group of 8 lanes per vector, one 16-float chunk
+            // per lane and pass, the row's four 32-vector passes in flight
+            // together.  Coordinates are centred (L2), summed into squared
+            // norm partials, and stored as BF16 [hi x8 | lo x8] per 8
+            // coordinates (one dword per coordinate).  Padding rows are zero.
+            // Per entry then: its norm, root or inverse root, and the two
+            // thresholds in the key domain.
+            const uint32_t C16 = c.D / 16, g = tid >> 3, sub = tid & 7u;
+            const float* cen = c.vecs + size_t(storage_of(c, row)) * c.D;
+            float nrm[4] = {0.f, 0.f, 0.f, 0.f};
+            for (uint32_t ch = sub; ch < C16; ch += 8) {
+                float4 x[4][4];
+                float4 m[4] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f),
+                               make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+                if (centred)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) m[u] = *reinterpret_cast<const float4*>(cen + ch * 16 + 4 * u);
+#pragma unroll
+                for (int ps = 0; ps < 4; ++ps) {
+                    const uint32_t k = 32u * ps + g;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        x[ps][u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (k < n)
+                            x[ps][u] = *reinterpret_cast<const float4*>(c.vecs + size_t(s_vid[k]) * c.D + ch * 16 + 4 * u);
                     }
-                    acc = __builtin_fmaf(x.x, x.x, acc);
-                    acc = __builtin_fmaf(x.y, x.y, acc);
-                    acc = __builtin_fmaf(x.z, x.z, acc);
-                    acc = __builtin_fmaf(x.w, x.w, acc);
                 }
-                s_norm[k] = acc;
+#pragma unroll
+                for (int ps = 0; ps < 4; ++ps) {
+                    const uint32_t k = 32u * ps + g;
+                    if (k >= npad) continue;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        float4& v = x[ps][u];
+                        if (centred && k < n) v = make_float4(v.x - m[u].x, v.y - m[u].y, v.z - m[u].z, v.w - m[u].w);
+                        nrm[ps] = __builtin_fmaf(v.x, v.x, nrm[ps]);
+                        nrm[ps] = __builtin_fmaf(v.y, v.y, nrm[ps]);
+                        nrm[ps] = __builtin_fmaf(v.z, v.z, nrm[ps]);
+                        nrm[ps] = __builtin_fmaf(v.w, v.w, nrm[ps]);
+                    }
+                    bf16x8 h0, l0, h1, l1;
+                    split8(x[ps][0], x[ps][1], h0, l0);
+                    split8(x[ps][2], x[ps][3], h1, l1);
+                    uint32_t* dst = s_split + k * vstride + ch * 16;
+                    *reinterpret_cast<bf16x8*>(dst) = h0;
+                    *reinterpret_cast<bf16x8*>(dst + 4) = l0;
+                    *reinterpret_cast<bf16x8*>(dst + 8) = h1;
+                    *reinterpret_cast<bf16x8*>(dst + 12) = l1;
+                }
+            }
+#pragma unroll
+            for (int ps = 0; ps < 4; ++ps) {
+                float v = nrm[ps];
+                v += __shfl_xor(v, 4);
+                v += __shfl_xor(v, 2);
+                v += __shfl_xor(v, 1);
+                const uint32_t k = 32u * ps + g;
+                if (k < n && sub == 0) {
+                    s_norm[k] = v;
+                    s_rn[k] = metric == kDot ? __builtin_sqrtf(v) : (metric == kCosine ? 1.0f / __builtin_sqrtf(v) : 0.f);
+                    const float dk = s_dist[k];
+                    const float2 tp = key_threshold(metric, prune_of(dk)), td = key_threshold(metric, dk);
+                    s_thr[k] = make_float4(tp.x, tp.y, td.x, td.y);
+                }
             }
             __syncthreads();
         } else {
+            if (staged) {
+                // Gathers issued kStageBatch at a time per thread before any LDS store.
+                const uint32_t per = c.D / 4;  // float4 per vector
+                const uint32_t total = npad * per;
+                for (uint32_t base = 0; base < total; base += 256 * kStageBatch) {
+                    float4 v[kStageBatch];
+#pragma unroll
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const uint32_t t = base + uint32_t(u) * 256 + tid;
+                        const uint32_t k = t / per;
+                        v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (t < total && k < n)
+                            v[u] = *reinterpret_cast<const float4*>(c.vecs + size_t(s_vid[k]) * c.D + (t - k * per) * 4);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kStageBatch; ++u) {
+                        const uint32_t t = base + uint32_t(u) * 256 + tid;
+                        const uint32_t k = t / per;
+                        if (t < total) *reinterpret_cast<float4*>(s_vec + k * vstride + (t - k * per) * 4) = v[u];
+                    }
+                }
+                __syncthreads();
+            }
+            // Every needed pair by the FP32 chain, a quad of lanes per pair.
             const uint32_t np = n * (n - 1) / 2;
-            for (uint32_t p = tid; p < np; p += 256) {
+            for (uint32_t p0 = 0; p0 < np; p0 += 64) {
+                const uint32_t p = p0 + (tid >> 2);
+                if (p >= np) break;
                 uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
                 while (i * (i - 1) / 2 > p) --i;
                 while ((i + 1) * i / 2 <= p) ++i;
                 const uint32_t j = p - i * (i - 1) / 2;
                 if (!is_new(i) && !is_new(j)) continue;
-                s_P[p] = exact_pair(i, j);
-                exact_pairs++;
+                const float d = exact_pair_quad(i, j);
+                if ((tid & 3u) == 0) {
+                    s_P[p] = d;
+                    exact_pairs++;
+                }
             }
             __syncthreads();
         }
@@ -469,89 +549,9 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
             const uint32_t J = t - I * (I + 1) / 2;
             const uint32_t j = 32 * J + (lane & 31);
             const bool jnew = j < n && is_new(j);
-            float tv[16];
-            if (MFMA) {
-                f32x16 acc = {};
-                const uint32_t ko = 8 * (lane >> 5);
-                const float* pa = s_vec + (32 * I + (lane & 31)) * vstride + ko;
-                const float* pb = s_vec + (32 * J + (lane & 31)) * vstride + ko;
-                for (uint32_t k0 = 0; k0 < c.D; k0 += 16) {
-                    float4 a0 = *reinterpret_cast<const float4*>(pa + k0);
-                    float4 a1 = *reinterpret_cast<const float4*>(pa + k0 + 4);
-                    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f), c1 = c0;
-                    if (centred) {
-                        c0 = *reinterpret_cast<const float4*>(s_cen + ko + k0);
-                        c1 = *reinterpret_cast<const float4*>(s_cen + ko + k0 + 4);
-                        a0 = make_float4(a0.x - c0.x, a0.y - c0.y, a0.z - c0.z, a0.w - c0.w);
-                        a1 = make_float4(a1.x - c1.x, a1.y - c1.y, a1.z - c1.z, a1.w - c1.w);
-                    }
-                    bf16x8 ahi, alo, bhi, blo;
-                    split8(a0, a1, ahi, alo);
-                    if (I != J) {
-                        float4 b0 = *reinterpret_cast<const float4*>(pb + k0);
-                        float4 b1 = *reinterpret_cast<const float4*>(pb + k0 + 4);
-                        if (centred) {
-                            b0 = make_float4(b0.x - c0.x, b0.y - c0.y, b0.z - c0.z, b0.w - c0.w);
-                            b1 = make_float4(b1.x - c1.x, b1.y - c1.y, b1.z - c1.z, b1.w - c1.w);
-                        }
-                        split8(b0, b1, bhi, blo);
-                    } else {
-                        bhi = ahi;
-                        blo = alo;
-                    }
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
-                }
-                const float nj = j < n ? s_norm[j] : 0.f;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                    tv[q] = i < n ? approx_metric(c.metric, acc[q], s_norm[i], nj) : 0.f;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                    tv[q] = (i < n && j < i && (jnew || is_new(i))) ? s_P[tri_index(i, j)] : 0.f;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const bool valid = i < n && j < i && (jnew || is_new(i));
-                const float di = i < n ? s_dist[i] : 0.f;
-                const float prune = prune_of(di);
-                bool hit = false, rec = false;
-                if (MFMA) {
-                    float lo = 0.f, hi = 0.f;
-                    if (valid) approx_interval(c.metric, kb, tv[q], s_norm[i], s_norm[j], lo, hi);
-                    const bool amb = valid && !((hi < prune || lo >= prune) && (hi < di || lo >= di));
-                    if (valid) s_P[tri_index(i, j)] = tv[q];
-                    approx_pairs += valid ? 1 : 0;
-                    const uint64_t am = ballot(amb);
-                    if (am) {
-                        uint32_t base = 0;
-                        if (lane == 0) base = atomicAdd(&s_nl, uint32_t(__popcll(am)));
-                        base = readlane_u(base, 0);
-                        const uint32_t slot = base + lanes_below(am);
-                        if (amb) {
-                            if (slot < c.list_cap) {
-                                s_list[slot] = uint16_t((i << 7) | j);
-                            } else {  // queue full: flagged in the overflow bitmap
-                                const uint32_t p = tri_index(i, j);
-                                atomicOr(s_pend + (p >> 5), 1u << (p & 31));
-                            }
-                        }
-                    }
-                    if (valid && !amb) {
-                        hit = hi < prune;
-                        rec = hi < di;
-                    }
-                } else {
-                    hit = valid && tv[q] < prune;
-                    rec = valid && tv[q] < di;
-                }
+            const uint32_t newI = I == 0 ? newm[0] : (I == 1 ? newm[1] : (I == 2 ? newm[2] : newm[3]));
+            // Lanes 0-31 hold entry i0 and lanes 32-63 entry i0 + 4 of tile column J.
+            auto write_masks = [&](int q, bool hit, bool rec) {
                 const uint64_t hb = ballot(hit), cb = ballot(rec);
                 if (lane == 0) {
                     const uint32_t i0 = 32 * I + 8 * (q >> 2) + (q & 3);
@@ -560,13 +560,90 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
                     s_C[i0 * 4 + J] = uint32_t(cb);
                     s_C[(i0 + 4) * 4 + J] = uint32_t(cb >> 32);
                 }
+            };
+            if (MFMA) {
+                f32x16 acc = {}, acc_hl = {}, acc_lh = {};
+                const uint32_t* sa = s_split + (32 * I + (lane & 31)) * vstride + 8 * (lane >> 5);
+                const uint32_t* sb = s_split + (32 * J + (lane & 31)) * vstride + 8 * (lane >> 5);
+                for (uint32_t k0 = 0; k0 < c.D; k0 += 16) {
+                    const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(sa + k0);
+                    const bf16x8 alo = *reinterpret_cast<const bf16x8*>(sa + k0 + 4);
+                    const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(sb + k0);
+                    const bf16x8 blo = *reinterpret_cast<const bf16x8*>(sb + k0 + 4);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+                    acc_hl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc_hl, 0, 0, 0);
+                    acc_lh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc_lh, 0, 0, 0);
+                }
+                const float nj = j < n ? s_norm[j] : 0.f, rnj = j < n ? s_rn[j] : 0.f;
+                // Per lane, bit q of each word: the decision for its pair q.
+                uint32_t hitb = 0, recb = 0, ambb = 0, nvalid = 0;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    const bool valid = (i < n) & (j < i) & (jnew | ((newI >> (i & 31)) & 1u));
+                    const float g = acc[q] + (acc_hl[q] + acc_lh[q]);
+                    const float ni = s_norm[i], rni = s_rn[i];
+                    const float4 th = s_thr[i];
+                    float key;
+                    if (metric == kL2Sqr || metric == kL2)
+                        key = (ni + nj) - 2.0f * g;
+                    else if (metric == kDot)
+                        key = -g;
+                    else
+                        key = 1.0f - g * (rni * rnj);
+                    const float e = key_radius(metric, kb, ni, nj, rni, rnj);
+                    const float lo = key - e, hi = key + e;
+                    const bool dp = hi < th.x, np_ = lo >= th.y, dd = hi < th.z, nd = lo >= th.w;
+                    const bool sure = (dp | np_) & (dd | nd);
+                    if (valid) s_P[tri_index(i, j)] = key;
+                    nvalid += valid ? 1u : 0u;
+                    hitb |= uint32_t(valid & sure & dp) << q;
+                    recb |= uint32_t(valid & sure & dd) << q;
+                    ambb |= uint32_t(valid & !sure) << q;
+                }
+                approx_pairs += nvalid;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) write_masks(q, (hitb >> q) & 1u, (recb >> q) & 1u);
+                // Ambiguous pairs of the tile: one queue reservation.
+                uint32_t namb = 0;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) namb += uint32_t(__popcll(ballot((ambb >> q) & 1u)));
+                if (namb) {
+                    uint32_t off = 0;
+                    if (lane == 0) off = atomicAdd(&s_nl, namb);
+                    off = readlane_u(off, 0);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const bool mine = (ambb >> q) & 1u;
+                        const uint64_t m = ballot(mine);
+                        if (mine) {
+                            const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                            const uint32_t slot = off + lanes_below(m);
+                            if (slot < c.list_cap) {
+                                s_list[slot] = uint16_t((i << 7) | j);
+                            } else {  // queue full: flagged in the overflow bitmap
+                                const uint32_t p = tri_index(i, j);
+                                atomicOr(s_pend + (p >> 5), 1u << (p & 31));
+                            }
+                        }
+                        off += uint32_t(__popcll(m));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                    const bool valid = i < n && j < i && (jnew || ((newI >> (i & 31)) & 1u));
+                    const float tv = valid ? s_P[tri_index(i, j)] : 0.f;
+                    const float di = i < n ? s_dist[i] : 0.f;
+                    write_masks(q, valid && tv < prune_of(di), valid && tv < di);
+                }
             }
         }
         __syncthreads();
         if (MFMA) {
-            // Ambiguous predicates: FP32 chain, bits set here.
-            auto settle = [&](uint32_t i, uint32_t j) {
-                const float te = exact_pair(i, j);
+            // Ambiguous predicates: FP32 chain (a quad of lanes per pair), bits set here.
+            auto settle = [&](uint32_t i, uint32_t j, float te) {
                 const uint32_t p = tri_index(i, j);
                 s_P[p] = te;
                 mark_exact(p);
@@ -575,17 +652,7 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
                 if (te < prune_of(di)) atomicOr(s_H + i * 4 + (j >> 5), 1u << (j & 31));
                 if (te < di) atomicOr(s_C + i * 4 + (j >> 5), 1u << (j & 31));
             };
-            const uint32_t nl = min(s_nl, c.list_cap);
-            for (uint32_t k = tid; k < nl; k += 256) settle(s_list[k] >> 7, s_list[k] & 127u);
-            if (s_nl > c.list_cap)
-                for (uint32_t w = tid; w < xwords; w += 256)
-                    for (uint32_t m = s_pend[w]; m; m &= m - 1) {
-                        const uint32_t p = 32 * w + __ffs(m) - 1;
-                        uint32_t i = uint32_t((1.0f + sqrtf(1.0f + 8.0f * float(p))) * 0.5f);
-                        while (i * (i - 1) / 2 > p) --i;
-                        while ((i + 1) * i / 2 <= p) ++i;
-                        settle(i, p - i * (i - 1) / 2);
-                    }
+            drain(settle);
             __syncthreads();
         }
 
@@ -594,8 +661,8 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
             uint32_t ha[4], hb[4];
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
-                ha[w] = s_H[lane * 4 + w];
-                hb[w] = s_H[(lane + 64) * 4 + w];
+                ha[w] = lane < n ? s_H[lane * 4 + w] : 0u;
+                hb[w] = lane + 64 < n ? s_H[(lane + 64) * 4 + w] : 0u;
             }
             uint32_t kr[4] = {0, 0, 0, 0}, kn[4] = {0, 0, 0, 0};
             bool da = lane >= n, db = lane + 64 >= n;
@@ -655,57 +722,79 @@ __global__ __launch_bounds__(256) void triangle_kernel(Ctx c, float inv_scale, f
         }
         const bool has_rec = (rec[0] | rec[1] | rec[2] | rec[3]) != 0;
         if (MFMA) {
-            // Candidates that may be the closest: interval low end at or below
-            // the smallest high end.  Those not yet exact go to the FP32 chain.
+            // One pass over the candidates, four at a time: the exact ones
+            // give the best target so far; every other candidate whose key
+            // interval starts at or below the smallest high end seen so far
+            // (a superset of those that can be the closest) is queued.
             if (has_rec) {
-                const float ni = s_norm[i];
+                const float ni = s_norm[i], rni = s_rn[i];
                 float m_hi = INFINITY;
-                for (int w = 0; w < 4; ++w)
-                    for (uint32_t m = rec[w]; m; m &= m - 1) {
-                        const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
-                        float lo = s_P[p], hi = lo;
-                        if (!is_exact(p)) approx_interval(c.metric, kb, s_P[p], ni, s_norm[r], lo, hi);
-                        m_hi = fminf(m_hi, hi);
-                    }
-                for (int w = 0; w < 4; ++w)
-                    for (uint32_t m = rec[w]; m; m &= m - 1) {
-                        const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
-                        if (is_exact(p)) continue;
-                        float lo, hi;
-                        approx_interval(c.metric, kb, s_P[p], ni, s_norm[r], lo, hi);
-                        if (!(lo <= m_hi) && m_hi == m_hi && lo == lo) continue;
-                        const uint32_t slot = atomicAdd(&s_nl, 1u);
-                        if (slot < c.list_cap) {
-                            s_list[slot] = uint16_t((i << 7) | r);
-                        } else {
-                            s_P[p] = exact_pair(i, r);
-                            mark_exact(p);
-                            exact_pairs++;
+                unsigned long long best = ~0ull;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    uint32_t m = rec[w];
+                    while (m) {
+                        uint32_t rr[4];
+                        bool ok[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            ok[u] = m != 0;
+                            rr[u] = ok[u] ? 32u * w + __ffs(m) - 1 : 0u;
+                            m &= m - 1;
+                        }
+                        float v[4], nr[4], rnr[4];
+                        bool ex[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const uint32_t p = tri_index(i, rr[u]);
+                            v[u] = ok[u] ? s_P[p] : 0.f;
+                            ex[u] = ok[u] && is_exact(p);
+                            nr[u] = s_norm[rr[u]];
+                            rnr[u] = s_rn[rr[u]];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            if (!ok[u]) continue;
+                            float lo, hi;
+                            if (ex[u]) {
+                                exact_key(metric, v[u], lo, hi);
+                                const unsigned long long k = target_key(v[u], rr[u]);
+                                best = k < best ? k : best;
+                            } else {
+                                const float e = key_radius(metric, kb, ni, nr[u], rni, rnr[u]);
+                                lo = v[u] - e;
+                                hi = v[u] + e;
+                            }
+                            m_hi = fminf(m_hi, hi);
+                            if (!ex[u] && !(lo > m_hi)) push_recheck(i, rr[u]);
                         }
                     }
+                }
+                s_best[i] = best;
             }
             __syncthreads();
-            const uint32_t nl = min(s_nl, c.list_cap);
-            for (uint32_t k = tid; k < nl; k += 256) {
-                const uint32_t a = s_list[k] >> 7, b = s_list[k] & 127u;
+            drain([&](uint32_t a, uint32_t b, float te) {
                 const uint32_t p = tri_index(a, b);
-                s_P[p] = exact_pair(a, b);
+                s_P[p] = te;
                 mark_exact(p);
                 exact_pairs++;
-            }
+                atomicMin(s_best + a, target_key(te, b));
+            });
             __syncthreads();
-        }
-        if (has_rec) {
-            // Closest candidate; equal distances go to the later position.  In
-            // the BF16 path only pairs settled by the FP32 chain can win: every
-            // other candidate's value is above the winner's.
+            if (has_rec) {
+                const uint32_t br = 0xFFu - uint32_t(s_best[i] & 0xFFu);
+                const uint32_t target = local_of(c, s_vid[br]);
+                if (target != 0xFFFFFFFFu)
+                    emit(c, row, atomicAdd(&s_ne, 1u), target, s_vid[i], s_P[tri_index(i, br)]);
+            }
+        } else if (has_rec) {
+            // Closest candidate; equal distances go to the later position.
             float best = 0.f;
             int br = -1;
             for (int w = 0; w < 4; ++w)
                 for (uint32_t m = rec[w]; m; m &= m - 1) {
-                    const uint32_t r = 32 * w + __ffs(m) - 1, p = tri_index(i, r);
-                    if (MFMA && !is_exact(p)) continue;
-                    const float v = s_P[p];
+                    const uint32_t r = 32 * w + __ffs(m) - 1;
+                    const float v = s_P[tri_index(i, r)];
                     if (br < 0 || v <= best) {
                         best = v;
                         br = int(r);
@@ -930,7 +1019,7 @@ JoinBounds join_bounds(uint32_t D) {
     JoinBounds b;
     b.l2 = float(2.0 * (kg + d * u + 6.0 * u + 0x1p-22 + 2.0 * (d / 16.0 + 6.0) * u));
     b.dot = float(2.0 * (kg + (d / 16.0 + 4.0) * u + d * u));
-    b.cos = float(2.0 * (kg + (d / 16.0 + 4.0) * u + 2.0 * d * u + 8.0 * u));
+    b.cos = float(2.0 * (kg + (d / 16.0 + 4.0) * u + 2.0 * d * u + 24.0 * u));
     return b;
 }
 
@@ -938,25 +1027,34 @@ int do_triangle(artea_refine* r, float scale, float shift) {
     Ctx& c = r->c;
     const float inv = 1.0f / scale;
     const float shift_term = (1.0f + inv) * shift;
-    const uint32_t vstride = c.D + 4;
     int dev = 0, max_lds = 65536;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    const bool staged = tri_layout(c.cap, c.D, true).total <= uint32_t(max_lds);
-    const size_t lds = tri_layout(c.cap, c.D, staged).total;
+    const bool mfma = r->join == ARTEA_JOIN_BF16 && tri_layout(c.cap, c.D, true, true).total <= uint32_t(max_lds);
+    const bool staged = mfma || tri_layout(c.cap, c.D, true, false).total <= uint32_t(max_lds);
+    const size_t lds = tri_layout(c.cap, c.D, staged, mfma).total;
     ARTEA_CHECK_ARG(lds <= size_t(max_lds), "triangle: row capacity too large for LDS");
-    const bool mfma = staged && r->join == ARTEA_JOIN_BF16;
-    const void* fn = mfma ? reinterpret_cast<const void*>(triangle_kernel<true>)
-                          : reinterpret_cast<const void*>(triangle_kernel<false>);
-    if (lds > 65536) ARTEA_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     const JoinBounds kb = join_bounds(c.D);
     const uint32_t per_cu = std::max<uint32_t>(1u, uint32_t(160u * 1024u / std::max<size_t>(lds, 1)));
     const dim3 grid(grid_for(r, 1, std::min<uint32_t>(per_cu, 4u)));
-    if (mfma)
-        hipLaunchKernelGGL(triangle_kernel<true>, grid, dim3(256), lds, r->s, c, inv, shift_term, 1, vstride, kb);
-    else
-        hipLaunchKernelGGL(triangle_kernel<false>, grid, dim3(256), lds, r->s, c, inv, shift_term, staged ? 1 : 0,
-                           vstride, kb);
+    auto launch = [&](auto kernel) -> int {
+        const void* fn = reinterpret_cast<const void*>(kernel);
+        if (lds > 65536) ARTEA_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        hipLaunchKernelGGL(kernel, grid, dim3(256), lds, r->s, c, inv, shift_term, staged ? 1 : 0, kb);
+        return ARTEA_OK;
+    };
+    int rc = ARTEA_OK;
+    switch (c.metric * 2 + (mfma ? 1 : 0)) {
+        case kL2Sqr * 2: rc = launch(triangle_kernel<false, kL2Sqr>); break;
+        case kL2Sqr * 2 + 1: rc = launch(triangle_kernel<true, kL2Sqr>); break;
+        case kDot * 2: rc = launch(triangle_kernel<false, kDot>); break;
+        case kDot * 2 + 1: rc = launch(triangle_kernel<true, kDot>); break;
+        case kCosine * 2: rc = launch(triangle_kernel<false, kCosine>); break;
+        case kCosine * 2 + 1: rc = launch(triangle_kernel<true, kCosine>); break;
+        case kL2 * 2: rc = launch(triangle_kernel<false, kL2>); break;
+        default: rc = launch(triangle_kernel<true, kL2>); break;
+    }
+    if (rc) return rc;
     ARTEA_HIP_TRY(hipGetLastError());
     return merge_logs(r);
 }
