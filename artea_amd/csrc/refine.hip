@@ -184,6 +184,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kStageBatch = 8;      // row-vector gathers in flight per thread
 constexpr uint32_t kListCap = 768;  // FP32 recheck queue per row; overflow goes to a bitmap
+constexpr uint32_t kMfmaMinCap = 64; // BF16 tiles only for rows of more entries
 
 // Half-widths of the BF16 key intervals (derived on the host from D).
 struct JoinBounds {
@@ -193,7 +194,7 @@ struct JoinBounds {
 };
 
 struct TriLayout {
-    uint32_t vid, dist, norm, rn, thr, best, H, Cw, X, pend, list, P, vec, vstride, total;
+    uint32_t vid, dist, norm, rn, pm, thr, best, H, Cw, X, pend, list, P, vec, vstride, total;
 };
 
 __host__ __device__ inline uint32_t align16u(uint32_t x) { return (x + 15u) & ~15u; }
@@ -223,6 +224,7 @@ __host__ __device__ inline TriLayout tri_layout(uint32_t cap, uint32_t D, bool s
     const uint32_t m = mfma ? 1u : 0u;
     take(L.norm, m * capr * 4);
     take(L.rn, m * capr * 4);
+    take(L.pm, m * capr * 4);
     take(L.thr, m * capr * 16);
     take(L.best, m * capr * 8);
     take(L.X, m * xw);
@@ -288,6 +290,32 @@ __device__ __forceinline__ float key_radius(int metric, const JoinBounds& kb, fl
     return kb.cos;
 }
 
+// Position of the k-th (0-based) set bit of a 128-bit mask held in 4 words.
+__device__ __forceinline__ uint32_t select_in_words(const uint32_t w[4], uint32_t k) {
+    uint32_t base = 0, x = w[3];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t pc = __popc(w[t]);
+        if (k < pc) {
+            x = w[t];
+            base = 32u * t;
+            break;
+        }
+        k -= pc;
+    }
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t sh = 16; sh >= 1; sh >>= 1) {
+        const uint32_t c = __popc(x & ((1u << sh) - 1u));
+        if (k >= c) {
+            k -= c;
+            x >>= sh;
+            pos += sh;
+        }
+    }
+    return base + pos;
+}
+
 // Recommendation target order: smaller distance first, then the later
 // position (-0 and +0 are equal, as the FP32 comparisons see them).
 __device__ __forceinline__ unsigned long long target_key(float d, uint32_t r) {
@@ -306,6 +334,7 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
     float* s_dist = reinterpret_cast<float*>(smem + L.dist);
     float* s_norm = reinterpret_cast<float*>(smem + L.norm);
     float* s_rn = reinterpret_cast<float*>(smem + L.rn);
+    float* s_pm = reinterpret_cast<float*>(smem + L.pm);  // prefix bound of the radius term over r < i
     float4* s_thr = reinterpret_cast<float4*>(smem + L.thr);
     unsigned long long* s_best = reinterpret_cast<unsigned long long*>(smem + L.best);
     uint32_t* s_H = reinterpret_cast<uint32_t*>(smem + L.H);
@@ -337,14 +366,19 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
     // lanes active, same pair).
     auto exact_pair_quad = [&](uint32_t i, uint32_t j) -> float {
         const uint32_t sub = lane & 3u;
-        const float qn2 = metric == kCosine ? quad_norm2(vec_of(i), int(c.C), sub) : 0.f;
-        return quad_distance<0>(vec_of(i), vec_of(j), int(c.C), metric, sub, qn2);
+        const float* a = vec_of(i);
+        const float* b = vec_of(j);
+        const float qn2 = metric == kCosine ? quad_norm2(a, int(c.C), sub) : 0.f;
+        // Fully unrolled chunk loops for the common widths (all loads in flight).
+        if (c.C == 8) return quad_distance<8>(a, b, 8, metric, sub, qn2);
+        if (c.C == 6) return quad_distance<6>(a, b, 6, metric, sub, qn2);
+        return quad_distance<0>(a, b, int(c.C), metric, sub, qn2);
     };
     auto mark_exact = [&](uint32_t p) { atomicOr(s_X + (p >> 5), 1u << (p & 31)); };
     auto is_exact = [&](uint32_t p) -> bool { return (s_X[p >> 5] >> (p & 31)) & 1u; };
-    // Queue of pairs for the FP32 chain; beyond list_cap entries, a bitmap.
-    auto push_recheck = [&](uint32_t i, uint32_t j) {
-        const uint32_t slot = atomicAdd(&s_nl, 1u);
+    // Queue of pairs for the FP32 chain (slot reserved from s_nl); beyond
+    // list_cap entries, a bitmap.
+    auto put_recheck = [&](uint32_t slot, uint32_t i, uint32_t j) {
         if (slot < c.list_cap) {
             s_list[slot] = uint16_t((i << 7) | j);
         } else {
@@ -500,6 +534,24 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
                 }
             }
             __syncthreads();
+            // Prefix bound of the radius term (norm, or root norm for dot):
+            // s_pm[i] = max over r < i.  Waves 0 and 1, read only after later
+            // barriers.
+            if (tid < 128) {
+                auto term = [&](uint32_t k) -> float {
+                    return k < n ? (metric == kDot ? s_rn[k] : s_norm[k]) : 0.f;
+                };
+                float t = term(tid);
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const float o = __shfl_up(t, d);
+                    if (int(lane) >= d) t = fmaxf(t, o);
+                }
+                float ex = __shfl_up(t, 1);
+                if (lane == 0) ex = 0.f;
+                if (wid == 1) ex = fmaxf(ex, -wave_min(-term(lane)));
+                if (tid < n) s_pm[tid] = ex;
+            }
         } else {
             if (staged) {
                 // Gathers issued kStageBatch at a time per thread before any LDS store.
@@ -616,16 +668,8 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
                     for (int q = 0; q < 16; ++q) {
                         const bool mine = (ambb >> q) & 1u;
                         const uint64_t m = ballot(mine);
-                        if (mine) {
-                            const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                            const uint32_t slot = off + lanes_below(m);
-                            if (slot < c.list_cap) {
-                                s_list[slot] = uint16_t((i << 7) | j);
-                            } else {  // queue full: flagged in the overflow bitmap
-                                const uint32_t p = tri_index(i, j);
-                                atomicOr(s_pend + (p >> 5), 1u << (p & 31));
-                            }
-                        }
+                        if (mine)
+                            put_recheck(off + lanes_below(m), 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3), j);
                         off += uint32_t(__popcll(m));
                     }
                 }
@@ -722,12 +766,24 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
         }
         const bool has_rec = (rec[0] | rec[1] | rec[2] | rec[3]) != 0;
         if (MFMA) {
-            // One pass over the candidates, four at a time: the exact ones
-            // give the best target so far; every other candidate whose key
-            // interval starts at or below the smallest high end seen so far
-            // (a superset of those that can be the closest) is queued.
+            // The target must be exact.  Candidates already settled by the
+            // FP32 chain give the best so far; of the others, those whose key
+            // interval starts at or below the smallest high end can be the
+            // closest and are queued.  Radii use the row's prefix bound
+            // (max over r < i of r's norm term), so each candidate costs one
+            // key load per pass.  The exact flags of row i's pairs are bits
+            // [i(i-1)/2, i(i-1)/2 + i) of the pair bitmap.
             if (has_rec) {
-                const float ni = s_norm[i], rni = s_rn[i];
+                const uint32_t b0 = tri_index(i, 0);
+                uint32_t xm[4];
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const uint32_t bit = b0 + 32u * w, wi = bit >> 5, sh = bit & 31u;
+                    const uint32_t lo = wi < xwords ? s_X[wi] : 0u;
+                    const uint32_t hi = (sh != 0 && wi + 1 < xwords) ? s_X[wi + 1] : 0u;
+                    xm[w] = sh ? (lo >> sh) | (hi << (32u - sh)) : lo;
+                }
+                const float e = key_radius(metric, kb, s_norm[i], s_pm[i], s_rn[i], s_pm[i]);
                 float m_hi = INFINITY;
                 unsigned long long best = ~0ull;
 #pragma unroll
@@ -735,42 +791,55 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
                     uint32_t m = rec[w];
                     while (m) {
                         uint32_t rr[4];
-                        bool ok[4];
+                        float v[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            ok[u] = m != 0;
-                            rr[u] = ok[u] ? 32u * w + __ffs(m) - 1 : 0u;
+                            rr[u] = m ? 32u * w + __ffs(m) - 1 : 0xFFFFFFFFu;
                             m &= m - 1;
-                        }
-                        float v[4], nr[4], rnr[4];
-                        bool ex[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            const uint32_t p = tri_index(i, rr[u]);
-                            v[u] = ok[u] ? s_P[p] : 0.f;
-                            ex[u] = ok[u] && is_exact(p);
-                            nr[u] = s_norm[rr[u]];
-                            rnr[u] = s_rn[rr[u]];
+                            v[u] = rr[u] != 0xFFFFFFFFu ? s_P[tri_index(i, rr[u])] : 0.f;
                         }
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            if (!ok[u]) continue;
+                            if (rr[u] == 0xFFFFFFFFu) continue;
                             float lo, hi;
-                            if (ex[u]) {
+                            if ((xm[w] >> (rr[u] & 31u)) & 1u) {
                                 exact_key(metric, v[u], lo, hi);
                                 const unsigned long long k = target_key(v[u], rr[u]);
                                 best = k < best ? k : best;
                             } else {
-                                const float e = key_radius(metric, kb, ni, nr[u], rni, rnr[u]);
-                                lo = v[u] - e;
                                 hi = v[u] + e;
                             }
                             m_hi = fminf(m_hi, hi);
-                            if (!ex[u] && !(lo > m_hi)) push_recheck(i, rr[u]);
                         }
                     }
                 }
                 s_best[i] = best;
+                uint32_t cm[4], cnt = 0;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    cm[w] = 0;
+                    uint32_t m = rec[w] & ~xm[w];
+                    while (m) {
+                        uint32_t rr[4];
+                        float v[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            rr[u] = m ? 32u * w + __ffs(m) - 1 : 0xFFFFFFFFu;
+                            m &= m - 1;
+                            v[u] = rr[u] != 0xFFFFFFFFu ? s_P[tri_index(i, rr[u])] : 0.f;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (rr[u] != 0xFFFFFFFFu && !(v[u] - e > m_hi)) cm[w] |= 1u << (rr[u] & 31u);
+                    }
+                    cnt += __popc(cm[w]);
+                }
+                if (cnt) {
+                    uint32_t off = atomicAdd(&s_nl, cnt);
+#pragma unroll
+                    for (int w = 0; w < 4; ++w)
+                        for (uint32_t m = cm[w]; m; m &= m - 1) put_recheck(off++, i, 32u * w + __ffs(m) - 1);
+                }
             }
             __syncthreads();
             drain([&](uint32_t a, uint32_t b, float te) {
@@ -1030,7 +1099,11 @@ int do_triangle(artea_refine* r, float scale, float shift) {
     int dev = 0, max_lds = 65536;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    const bool mfma = r->join == ARTEA_JOIN_BF16 && tri_layout(c.cap, c.D, true, true).total <= uint32_t(max_lds);
+    // Rows of at most kMfmaMinCap entries run the FP32 chain in either mode:
+    // their pair counts are small and the FP32 kernel's lower register count
+    // keeps twice the rows per CU in flight (measured faster at cap 48).
+    const bool mfma = r->join == ARTEA_JOIN_BF16 && c.cap > kMfmaMinCap &&
+                      tri_layout(c.cap, c.D, true, true).total <= uint32_t(max_lds);
     const bool staged = mfma || tri_layout(c.cap, c.D, true, false).total <= uint32_t(max_lds);
     const size_t lds = tri_layout(c.cap, c.D, staged, mfma).total;
     ARTEA_CHECK_ARG(lds <= size_t(max_lds), "triangle: row capacity too large for LDS");

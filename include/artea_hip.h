@@ -203,8 +203,8 @@ int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_i
  * accumulation) places every pair's distance in a proven interval; pairs whose
  * interval straddles a pruning or recommendation threshold, and candidates
  * that may be the recommendation target, are recomputed by the FP32 chain, so
- * rows and logs are identical to ARTEA_JOIN_FP32's.  Falls back to FP32 when
- * the row's vectors do not fit in LDS. */
+ * rows and logs are identical to ARTEA_JOIN_FP32's.  Layers of cap <= 64, and
+ * rows whose vectors do not fit in LDS, use the FP32 chain either way. */
 enum artea_join_precision { ARTEA_JOIN_FP32 = 0, ARTEA_JOIN_BF16 = 1 };
 int artea_refine_set_join(artea_refine* r, int precision);
 /* Entries of the BF16 join's per-row queue of pairs to recompute in FP32

@@ -76,7 +76,7 @@ def test_triangle_kats_on_device(orc, join):
 
 
 LAYER_CASES = [(3, 0, 128, 96, 1.5), (3, 0, 128, 48, 0.0), (2, 1, 100, 96, 0.0), (3, 2, 96, 96, 1.5),
-               (0, 2, 96, 128, 0.0), (1, 1, 100, 64, 0.0)]
+               (0, 2, 96, 128, 0.0), (1, 1, 100, 80, 0.0)]
 
 
 @pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
@@ -123,7 +123,7 @@ def test_bf16_join_settles_few_pairs_in_fp32():
     fp, bf = out[JOIN_FP32][1], out[JOIN_BF16][1]
     assert fp["approx_pair_distances"] == 0 and bf["approx_pair_distances"] == fp["pair_distances"]
     assert bf["logs"] == fp["logs"]
-    assert bf["pair_distances"] < 0.25 * fp["pair_distances"], (bf, fp)
+    assert bf["pair_distances"] < 0.15 * fp["pair_distances"], (bf, fp)
 
 
 @pytest.mark.parametrize("queue", [0, 7])
