@@ -21,7 +21,7 @@ from ._lib import ArteaError, check, lib
 
 __all__ = [
     "ArteaError", "DistanceMetrics", "DistFunc", "CompactHierarchicalGraph", "HierarchicalGraphFileManager",
-    "DeviceIndex", "HierarchicalGraphRouter", "BatchedRouter", "PropagateEngine", "generate", "pad_dim",
+    "DeviceIndex", "HierarchicalGraphRouter", "BatchedRouter", "LaunchGraph", "PropagateEngine", "generate", "pad_dim",
     "INVALID_VID", "EARLY_STOP", "L0_ONLY",
 ]
 
@@ -291,6 +291,55 @@ class BatchedRouter:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().artea_batched_router_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ndc": a.value, "expansions": b.value, "tail_overflows": c.value}
+
+    VARIANTS = ("narrow", "wide", "wide-3waves", "wide-4waves")
+
+    def autotune(self, d_queries: int, nq: int, d_topk: int, d_queue: int, d_out_vids: int, d_out_dists: int,
+                 flags: int = 0, thresholds=None):
+        """Times every kernel variant on this batch and keeps the fastest;
+        returns (chosen variant, per-variant milliseconds)."""
+        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
+        ms = np.zeros(len(self.VARIANTS), np.float32)
+        ch = C.c_int(-1)
+        check(lib().artea_batched_router_autotune(self.h, C.c_void_p(d_queries), nq, C.c_void_p(d_topk),
+                                                  C.c_void_p(d_queue), flags, _ptr(thr) if thr is not None else None,
+                                                  C.c_void_p(d_out_vids), C.c_void_p(d_out_dists), _ptr(ms),
+                                                  C.byref(ch)))
+        return ch.value, ms
+
+    def set_variant(self, variant: int):
+        check(lib().artea_batched_router_set_variant(self.h, variant))
+
+    def variant(self) -> int:
+        v = C.c_int(-1)
+        check(lib().artea_batched_router_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    def capture(self, d_queries: int, nq: int, d_topk: int, d_queue: int, d_out_vids: int, d_out_dists: int,
+                flags: int = 0, thresholds=None) -> "LaunchGraph":
+        """Records one query over these device buffers as a HIP graph."""
+        thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
+        h = C.c_void_p()
+        check(lib().artea_batched_router_capture(self.h, C.c_void_p(d_queries), nq, C.c_void_p(d_topk),
+                                                 C.c_void_p(d_queue), flags, _ptr(thr) if thr is not None else None,
+                                                 C.c_void_p(d_out_vids), C.c_void_p(d_out_dists), C.byref(h)))
+        return LaunchGraph(h, self)
+
+
+class LaunchGraph:
+    """An instantiated HIP graph of one batched-router query."""
+
+    def __init__(self, h, owner):
+        self.h = h
+        self.owner = owner  # keeps the router (and its buffers) alive
+
+    def launch(self, stream: int = 0):
+        check(lib().artea_launch_graph_launch(self.h, C.c_void_p(stream)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().artea_launch_graph_destroy(self.h)
+            self.h = None
 
 
 from .refine import PropagateEngine, build_full  # noqa: E402

@@ -47,6 +47,12 @@ SIGNATURES = {
     "artea_batched_router_destroy": (i32, [P]),
     "artea_batched_router_query_device": (i32, [P, P, u32, P, P, u32, P, P, P, P]),
     "artea_batched_router_last_stats": (i32, [P, P, P, P]),
+    "artea_batched_router_capture": (i32, [P, P, u32, P, P, u32, P, P, P, PP]),
+    "artea_batched_router_autotune": (i32, [P, P, u32, P, P, u32, P, P, P, P, C.POINTER(i32)]),
+    "artea_batched_router_set_variant": (i32, [P, i32]),
+    "artea_batched_router_get_variant": (i32, [P, C.POINTER(i32)]),
+    "artea_launch_graph_launch": (i32, [P, P]),
+    "artea_launch_graph_destroy": (i32, [P]),
     "artea_distance_matrix_device": (i32, [i32, P, u32, P, u32, u32, P, P]),
     "artea_generate": (i32, [i32, u64, u32, u64, u64, u32, u32, P, i32]),
     "artea_bruteforce_knn_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P]),

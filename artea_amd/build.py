@@ -24,6 +24,9 @@ ARCH = os.environ.get("ARTEA_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["index.hip", "router.hip", "refine.hip", "util.hip"]
 CXX_SOURCES = ["hgra.cpp", "datagen.cpp"]
+# router_inst.hip is compiled once per (metric family, candidate-list blocks)
+# pair; router_kernels.hpp lists the same pairs.
+ROUTER_INST = [(mf, k) for mf in (0, 1, 2) for k in (1, 2, 4, 8, 13, 16)]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-I", os.path.join(ROOT, "include")]
 HIP_FLAGS = COMMON + [
@@ -35,11 +38,12 @@ HIP_FLAGS = COMMON + [
 CXX_FLAGS = COMMON + ["-x", "c++", "-fopenmp", "-march=x86-64-v3"]
 
 
-def _compile(src: str) -> str:
+def _compile(job) -> str:
+    src, tag, defs = job if isinstance(job, tuple) else (job, "", [])
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src + ".o")
+    obj = os.path.join(OBJ, src + tag + ".o")
     flags = HIP_FLAGS if src.endswith(".hip") else CXX_FLAGS
-    cmd = [HIPCC] + flags + ["-c", path, "-o", obj]
+    cmd = [HIPCC] + flags + defs + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
@@ -48,9 +52,11 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    sources = [s for s in HIP_SOURCES + CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
-        objs = list(ex.map(_compile, sources))
+    jobs = [("router_inst.hip", "_mf%d_k%d" % (mf, k), ["-DARTEA_RK_MF=%d" % mf, "-DARTEA_RK_K=%d" % k])
+            for mf, k in ROUTER_INST]
+    jobs += [s for s in HIP_SOURCES + CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with ThreadPoolExecutor(max_workers=min(int(os.environ.get("MAX_JOBS", "8")), len(jobs))) as ex:
+        objs = list(ex.map(_compile, jobs))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -142,6 +142,118 @@ __device__ __forceinline__ float quad_distance(const float* __restrict__ q, cons
     return denom > 0.0f ? 1.0f - dot / denom : 1.0f;
 }
 
+// The metric value of one row whose quad slice (CT float4 per lane, chunk
+// order) is already in registers; same chains and rounding as quad_distance.
+template <int CT>
+__device__ __forceinline__ float quad_finish(const float* __restrict__ q, const float4 (&xv)[CT], int metric,
+                                             uint32_t sub, float qn2) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    if (metric == kL2Sqr || metric == kL2) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const float4 qv = *reinterpret_cast<const float4*>(q + c * 16 + sub * 4);
+            const float d0 = qv.x - xv[c].x, d1 = qv.y - xv[c].y, d2 = qv.z - xv[c].z, d3 = qv.w - xv[c].w;
+            a0 = __builtin_fmaf(d0, d0, a0);
+            a1 = __builtin_fmaf(d1, d1, a1);
+            a2 = __builtin_fmaf(d2, d2, a2);
+            a3 = __builtin_fmaf(d3, d3, a3);
+        }
+        const float s = reduce16_quad(a0, a1, a2, a3);
+        return metric == kL2 ? __builtin_sqrtf(s) : s;
+    }
+    if (metric == kDot) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const float4 qv = *reinterpret_cast<const float4*>(q + c * 16 + sub * 4);
+            a0 = __builtin_fmaf(qv.x, xv[c].x, a0);
+            a1 = __builtin_fmaf(qv.y, xv[c].y, a1);
+            a2 = __builtin_fmaf(qv.z, xv[c].z, a2);
+            a3 = __builtin_fmaf(qv.w, xv[c].w, a3);
+        }
+        return -reduce16_quad(a0, a1, a2, a3);
+    }
+    float n0 = 0.f, n1 = 0.f, n2 = 0.f, n3 = 0.f;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const float4 qv = *reinterpret_cast<const float4*>(q + c * 16 + sub * 4);
+        a0 = __builtin_fmaf(qv.x, xv[c].x, a0);
+        a1 = __builtin_fmaf(qv.y, xv[c].y, a1);
+        a2 = __builtin_fmaf(qv.z, xv[c].z, a2);
+        a3 = __builtin_fmaf(qv.w, xv[c].w, a3);
+        n0 = __builtin_fmaf(xv[c].x, xv[c].x, n0);
+        n1 = __builtin_fmaf(xv[c].y, xv[c].y, n1);
+        n2 = __builtin_fmaf(xv[c].z, xv[c].z, n2);
+        n3 = __builtin_fmaf(xv[c].w, xv[c].w, n3);
+    }
+    const float dot = reduce16_quad(a0, a1, a2, a3);
+    const float sq2 = reduce16_quad(n0, n1, n2, n3);
+    const float denom = __builtin_sqrtf(qn2 * sq2);
+    return denom > 0.0f ? 1.0f - dot / denom : 1.0f;
+}
+
+// Two rows per quad: both rows' slices are loaded before either is reduced,
+// so one memory round trip covers 32 rows per wavefront.  The chunk loop
+// feeds both rows from one read of the query chunk; every accumulator sees
+// the same operations in the same order as in quad_distance, so the results
+// equal two quad_distance calls bit for bit.
+template <int CT>
+__device__ __forceinline__ void quad_distance2(const float* __restrict__ q, const float* __restrict__ xa,
+                                               const float* __restrict__ xb, int C, int metric, uint32_t sub,
+                                               float qn2, float* da, float* db) {
+    if constexpr (CT == 0) {
+        *da = quad_distance<0>(q, xa, C, metric, sub, qn2);
+        *db = quad_distance<0>(q, xb, C, metric, sub, qn2);
+    } else {
+        float4 va[CT], vb[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) va[c] = *reinterpret_cast<const float4*>(xa + c * 16 + sub * 4);
+#pragma unroll
+        for (int c = 0; c < CT; ++c) vb[c] = *reinterpret_cast<const float4*>(xb + c * 16 + sub * 4);
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+        if (metric == kL2Sqr || metric == kL2) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const float4 qv = *reinterpret_cast<const float4*>(q + c * 16 + sub * 4);
+                const float x0 = qv.x - va[c].x, x1 = qv.y - va[c].y, x2 = qv.z - va[c].z, x3 = qv.w - va[c].w;
+                a0 = __builtin_fmaf(x0, x0, a0);
+                a1 = __builtin_fmaf(x1, x1, a1);
+                a2 = __builtin_fmaf(x2, x2, a2);
+                a3 = __builtin_fmaf(x3, x3, a3);
+                const float y0 = qv.x - vb[c].x, y1 = qv.y - vb[c].y, y2 = qv.z - vb[c].z, y3 = qv.w - vb[c].w;
+                b0 = __builtin_fmaf(y0, y0, b0);
+                b1 = __builtin_fmaf(y1, y1, b1);
+                b2 = __builtin_fmaf(y2, y2, b2);
+                b3 = __builtin_fmaf(y3, y3, b3);
+            }
+            const float sa = reduce16_quad(a0, a1, a2, a3), sb = reduce16_quad(b0, b1, b2, b3);
+            *da = metric == kL2 ? __builtin_sqrtf(sa) : sa;
+            *db = metric == kL2 ? __builtin_sqrtf(sb) : sb;
+            return;
+        }
+        if (metric == kDot) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const float4 qv = *reinterpret_cast<const float4*>(q + c * 16 + sub * 4);
+                a0 = __builtin_fmaf(qv.x, va[c].x, a0);
+                a1 = __builtin_fmaf(qv.y, va[c].y, a1);
+                a2 = __builtin_fmaf(qv.z, va[c].z, a2);
+                a3 = __builtin_fmaf(qv.w, va[c].w, a3);
+                b0 = __builtin_fmaf(qv.x, vb[c].x, b0);
+                b1 = __builtin_fmaf(qv.y, vb[c].y, b1);
+                b2 = __builtin_fmaf(qv.z, vb[c].z, b2);
+                b3 = __builtin_fmaf(qv.w, vb[c].w, b3);
+            }
+            *da = -reduce16_quad(a0, a1, a2, a3);
+            *db = -reduce16_quad(b0, b1, b2, b3);
+            return;
+        }
+        // Cosine keeps three chains per row; two rows' worth would outgrow the
+        // register budget of the search kernels, so the rows go one by one.
+        *da = quad_finish<CT>(q, va, kCosine, sub, qn2);
+        *db = quad_finish<CT>(q, vb, kCosine, sub, qn2);
+    }
+}
+
 // |q|^2 chain of the cosine kernel (same lane order), by a quad.
 __device__ __forceinline__ float quad_norm2(const float* __restrict__ q, int C, uint32_t sub) {
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;

@@ -25,8 +25,10 @@
 //    rule rejects it), and in a greedy walk a revisited vertex can never be
 //    strictly closer than the current best.  Results therefore equal the
 //    reference's for every cache size; only the distance count can grow.
-//  * Distances: quads of lanes per candidate row (16 rows per pass), see
-//    device.hpp quad_distance.
+//  * Distances: quads of lanes per candidate row, 32 rows per pass (every
+//    row's loads issued before any reduction), see device.hpp
+//    quad_distance2.  Insertions shift the register blocks with DPP
+//    wave_shr:1, and the tail is trimmed once per expansion chunk.
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
@@ -34,424 +36,12 @@
 #include <string>
 
 #include "common.hpp"
-#include "device.hpp"
+#include "router_kernels.hpp"
 
 using namespace artea;
-using namespace artea::dev;
+using namespace artea::rk;
 
 namespace {
-
-constexpr uint32_t kExplored = 0x80000000u;
-constexpr uint32_t kVidMask = 0x7FFFFFFFu;
-constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit tags
-
-struct RouterArgs {
-    const float* vecs;
-    uint32_t n, D, C;
-    int metric;
-    const uint32_t* ell0;
-    uint32_t stride0;
-    const uint2* vinfo;
-    const uint32_t* ellU;
-    uint32_t strideU;
-    uint32_t top, entry;
-    const float* queries;
-    uint32_t nq;
-    uint32_t topk, queue;                         // fixed per launch when arrays are null
-    const uint32_t* topk_arr;                     // per query (batched router)
-    const uint32_t* queue_arr;
-    uint32_t out_stride;
-    uint32_t flags;
-    const float* thresholds;                      // device, top+1 entries
-    uint32_t* out_vids;
-    float* out_dists;
-    unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
-    uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
-    uint32_t vis_wide;                            // 1: 4096 x 16-bit tags (B > 20), 0: 8192 x 8-bit
-    uint32_t* work_counter;                       // persistent mode when non-null
-};
-
-// Visited-cache key: h = v * odd mod 2^B is a bijection of [0, 2^B), so the
-// pair (slot = low bits of h, tag = 1 + the remaining bits) identifies v
-// exactly; tag 0 marks an empty slot.  With n <= 2^20 the 8 KiB hold 8192
-// slots of 8-bit tags, otherwise 4096 slots of 16-bit tags (n <= 2^28).
-struct VisKey {
-    uint32_t slot, tag;
-};
-__device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t wide) {
-    const uint32_t h = (v * 0x9E3779B1u) & hmask;
-    return wide ? VisKey{h & 4095u, (h >> 12) + 1} : VisKey{h & 8191u, (h >> 13) + 1};
-}
-__device__ __forceinline__ uint32_t vis_get(const unsigned char* vis, VisKey k, uint32_t wide) {
-    return wide ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[k.slot]) : uint32_t(vis[k.slot]);
-}
-__device__ __forceinline__ void vis_put(unsigned char* vis, VisKey k, uint32_t wide) {
-    if (wide) reinterpret_cast<uint16_t*>(vis)[k.slot] = uint16_t(k.tag);
-    else vis[k.slot] = uint8_t(k.tag);
-}
-
-struct WaveLds {
-    float* q;         // D floats
-    unsigned char* vis;  // kVisBytes
-    uint32_t* cand;   // 64
-    float* cdist;     // 64
-};
-
-// Collect the unvisited, valid ids of a 64-id chunk in row order into
-// cand[0..cnt), inserting them into the visited cache; returns cnt.
-__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask,
-                                            uint32_t wide) {
-    const bool valid = id < n;
-    const VisKey key = vis_key(id, hmask, wide);
-    const bool fresh = valid && vis_get(w.vis, key, wide) != key.tag;
-    wave_sync();
-    if (fresh) vis_put(w.vis, key, wide);
-    const uint64_t m = ballot(fresh);
-    if (fresh) w.cand[lanes_below(m)] = id;
-    wave_sync();
-    return uint32_t(__popcll(m));
-}
-
-// Distances of cand[0..cnt) into cdist[0..cnt).
-template <int CT>
-__device__ __forceinline__ void distances(const RouterArgs& a, const WaveLds& w, uint32_t cnt, float qn2) {
-    const uint32_t lane = lane_id();
-    const uint32_t sub = lane & 3u;
-    for (uint32_t g = 0; g < cnt; g += 16) {
-        const uint32_t r = g + (lane >> 2);
-        float d = 0.f;
-        if (r < cnt) {
-            const uint32_t v = w.cand[r];
-            d = quad_distance<CT>(w.q, a.vecs + size_t(v) * a.D, int(a.C), a.metric, sub, qn2);
-        }
-        if (r < cnt && sub == 0) w.cdist[r] = d;
-    }
-    wave_sync();
-}
-
-__device__ __forceinline__ const uint32_t* upper_row(const RouterArgs& a, uint32_t v, uint32_t level) {
-    const uint2 vi = a.vinfo[v];
-    if (vi.x == 0xFFFFFFFFu || level > vi.x || level == 0) return nullptr;
-    return a.ellU + size_t(vi.y + (vi.x - level)) * a.strideU;
-}
-
-template <int K>
-struct CandList {
-    float d[K];
-    uint32_t v[K];
-    uint32_t n;   // entries in W
-    uint32_t L;   // capacity of T
-    float lb;     // distance of W[L-1] when n >= L, else +inf
-    uint32_t overflow;
-
-    __device__ __forceinline__ void init(uint32_t cap) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            d[k] = __int_as_float(0x7F800000);
-            v[k] = 0xFFFFFFFFu;
-        }
-        n = 0;
-        L = cap;
-        lb = __int_as_float(0x7F800000);
-        overflow = 0;
-    }
-    __device__ __forceinline__ bool full() const { return n >= L; }
-
-    __device__ __forceinline__ float get_d(uint32_t idx) const {
-        float r = 0.f;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (idx / 64 == uint32_t(k)) r = readlane_f(d[k], int(idx % 64));
-        return r;
-    }
-
-    // try_push with the reference's rule already decided by the caller;
-    // inserts (dist, vid) unless the same key is present.
-    __device__ __forceinline__ void insert(float dist, uint32_t vid) {
-        const uint32_t lane = lane_id();
-        uint32_t p = 0;
-        bool dup = false;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t vk = v[k] & kVidMask;
-            p += uint32_t(__popcll(ballot(key_less(d[k], vk, dist, vid) && v[k] != 0xFFFFFFFFu)));
-            dup |= ballot(d[k] == dist && vk == vid && v[k] != 0xFFFFFFFFu) != 0;
-        }
-        if (dup) return;
-        if (n == uint32_t(K) * 64) overflow += 1;  // last entry falls off
-        // Shift [p, n) right by one, high blocks first.
-#pragma unroll
-        for (int k = K - 1; k >= 0; --k) {
-            float pd = __shfl_up(d[k], 1);
-            uint32_t pv = __shfl_up(v[k], 1);
-            if (k > 0) {
-                const float cd = readlane_f(d[k - 1], 63);
-                const uint32_t cv = readlane_u(v[k - 1], 63);
-                if (lane == 0) {
-                    pd = cd;
-                    pv = cv;
-                }
-            }
-            const uint32_t idx = uint32_t(k) * 64 + lane;
-            if (idx > p) {
-                d[k] = pd;
-                v[k] = pv;
-            } else if (idx == p) {
-                d[k] = dist;
-                v[k] = vid;
-            }
-        }
-        n = min(n + 1, uint32_t(K) * 64);
-        if (n > L) {
-            // W[L] was T's maximum: it stays only while tied with the new bound.
-            const float nlb = get_d(L - 1);
-            uint32_t keep = 0;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t idx = uint32_t(k) * 64 + lane;
-                keep += uint32_t(__popcll(ballot(idx >= L && idx < n && d[k] == nlb)));
-            }
-            const uint32_t nn = L + keep;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const uint32_t idx = uint32_t(k) * 64 + lane;
-                if (idx >= nn) {
-                    d[k] = __int_as_float(0x7F800000);
-                    v[k] = 0xFFFFFFFFu;
-                }
-            }
-            n = nn;
-        }
-        lb = n >= L ? get_d(L - 1) : __int_as_float(0x7F800000);
-    }
-
-    // pop_best_unexplored_entry: first unexplored entry; marks it explored.
-    // Every register block is indexed with compile-time constants only
-    // (a data-dependent block index would move the list to scratch memory).
-    __device__ __forceinline__ bool pop(uint32_t* vid) {
-        const uint32_t lane = lane_id();
-        int kk = -1, l = 0;
-        uint32_t found = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
-            if (kk < 0 && b) {
-                kk = k;
-                l = __ffsll((unsigned long long)b) - 1;
-                found = readlane_u(v[k], l);
-            }
-        }
-        if (kk < 0) return false;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (k == kk && int(lane) == l) v[k] |= kExplored;
-        *vid = found & kVidMask;
-        return true;
-    }
-
-    // The entry the next pop would return if no insertion precedes it
-    // (first unexplored entry), or 0xFFFFFFFF.
-    __device__ __forceinline__ uint32_t peek() const {
-        uint32_t found = 0xFFFFFFFFu;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
-            if (found == 0xFFFFFFFFu && b) found = readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
-        }
-        return found;
-    }
-};
-
-struct QueryStats {
-    uint32_t ndc, expansions, overflow;
-};
-
-template <int K, int CT>
-__device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi) {
-    const uint32_t lane = lane_id();
-    // Per-query work items are clamped to what this launch was sized for:
-    // topk <= row stride, topk <= L <= the router's maximum queue.
-    const uint32_t topk = min(a.topk_arr ? a.topk_arr[qi] : a.topk, a.out_stride);
-    const uint32_t L = max(min(a.queue_arr ? a.queue_arr[qi] : a.queue, a.queue), max(topk, 1u));
-    uint32_t* ov = a.out_vids + size_t(qi) * a.out_stride;
-    float* od = a.out_dists + size_t(qi) * a.out_stride;
-
-    // Query row into LDS.
-    const float* qg = a.queries + size_t(qi) * a.D;
-    for (uint32_t i = lane; i < a.D; i += 64) w.q[i] = qg[i];
-    for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
-    wave_sync();
-    const float qn2 = a.metric == kCosine ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
-
-    // Entry point distance (all lanes compute it; quad 0's value is used).
-    uint32_t cur = a.entry;
-    float cd = quad_distance<CT>(w.q, a.vecs + size_t(cur) * a.D, int(a.C), a.metric, lane & 3u, qn2);
-    cd = readlane_f(cd, 0);
-    uint32_t ndc = 1, expansions = 0;
-
-    if (!(a.flags & ARTEA_L0_ONLY)) {
-        for (uint32_t level = a.top; level >= 1; --level) {
-            // Greedy walk on `level` (greedy_loop_body); seed marked visited.
-            if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
-            wave_sync();
-            const bool early = (a.flags & ARTEA_EARLY_STOP) != 0;
-            const float thr = early ? a.thresholds[level] : 0.f;
-            while (true) {
-                if (early && cd <= thr) break;
-                const uint32_t* row = upper_row(a, cur, level);
-                if (!row) break;
-                uint32_t best_i = 0xFFFFFFFFu;
-                float best_d = cd;
-                uint32_t best_v = cur;
-                for (uint32_t base = 0; base < a.strideU; base += 64) {
-                    const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
-                    if (ballot(id != 0xFFFFFFFFu) == 0) break;
-                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
-                    if (cnt == 0) continue;
-                    distances<CT>(a, w, cnt, qn2);
-                    ndc += cnt;
-                    // First strict minimum in row order below the running best.
-                    float dd = lane < cnt ? w.cdist[lane] : __int_as_float(0x7F800000);
-                    uint32_t ii = lane;
-                    for (int off = 32; off >= 1; off >>= 1) {
-                        const float od2 = __shfl_xor(dd, off);
-                        const uint32_t oi = __shfl_xor(ii, off);
-                        if (od2 < dd || (od2 == dd && oi < ii)) {
-                            dd = od2;
-                            ii = oi;
-                        }
-                    }
-                    dd = readlane_f(dd, 0);
-                    ii = readlane_u(ii, 0);
-                    if (dd < best_d) {
-                        best_d = dd;
-                        best_i = ii;
-                        best_v = w.cand[ii];
-                    }
-                    wave_sync();
-                }
-                if (best_i == 0xFFFFFFFFu) break;  // local optimum
-                cur = best_v;
-                cd = best_d;
-            }
-            // visited.clear() after every level (:228-236).
-            for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
-            wave_sync();
-        }
-    }
-
-    // L0 beam search from the cursor (beam_search + beam_loop_body).
-    CandList<K> W;
-    W.init(L);
-    W.insert(cd, cur);
-    if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
-    wave_sync();
-    uint32_t pv;
-    // Speculative read of the first 64 ids of the row the next pop will most
-    // likely return, issued before this expansion's distance gathers so the
-    // next expansion usually starts without a dependent row load.
-    uint32_t pre_v = 0xFFFFFFFFu, pre_id = 0xFFFFFFFFu;
-    while (W.pop(&pv)) {
-        expansions++;
-        const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
-        const bool hit = pv == pre_v;
-        const uint32_t first = hit ? pre_id : (lane < a.stride0 ? row[lane] : 0xFFFFFFFFu);
-        pre_v = W.peek();
-        pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
-        for (uint32_t base = 0; base < a.stride0; base += 64) {
-            const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
-            if (ballot(id != 0xFFFFFFFFu) == 0) break;
-            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
-            if (cnt == 0) continue;
-            distances<CT>(a, w, cnt, qn2);
-            ndc += cnt;
-            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
-            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
-            wave_sync();
-            uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
-            while (pending) {
-                const bool ok = ((pending >> lane) & 1ull) && (!W.full() || myd < W.lb);
-                const uint64_t m = ballot(ok);
-                if (!m) break;
-                const int i = __ffsll((unsigned long long)m) - 1;
-                pending &= ~((2ull << i) - 1ull);
-                if (i == 63) pending = 0;
-                W.insert(readlane_f(myd, i), readlane_u(myv, i));
-            }
-        }
-    }
-
-    // extract_results(min(topk, |T|)), padded with the invalid entry.
-    const uint32_t nt = min(W.n, W.L);
-    const uint32_t kk = min(topk, nt);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const uint32_t idx = uint32_t(k) * 64 + lane;
-        if (idx < kk) {
-            ov[idx] = W.v[k] & kVidMask;
-            od[idx] = W.d[k];
-        }
-    }
-    for (uint32_t i = kk + lane; i < topk; i += 64) {
-        ov[i] = kInvalidVid;
-        od[i] = FLT_MAX;
-    }
-    return QueryStats{ndc, expansions, W.overflow};
-}
-
-__device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t wid = threadIdx.x / 64;
-    const size_t per_wave = size_t(a.D) * 4 + kVisBytes + 512;
-    unsigned char* base = smem + wid * per_wave;
-    WaveLds w;
-    w.q = reinterpret_cast<float*>(base);
-    w.vis = base + size_t(a.D) * 4;
-    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisBytes);
-    w.cdist = reinterpret_cast<float*>(w.cand + 64);
-    return w;
-}
-
-__device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long long ndc, unsigned long long exps,
-                                          unsigned long long ovf) {
-    if (lane_id() == 0 && a.stats) {
-        atomicAdd(a.stats + 0, ndc);
-        atomicAdd(a.stats + 1, exps);
-        if (ovf) atomicAdd(a.stats + 2, ovf);
-    }
-}
-
-// Plain router: one wavefront per query, the grid covers the batch.
-template <int K, int CT>
-__global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds(a);
-    const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
-    if (qi >= a.nq) return;
-    const QueryStats st = route_one<K, CT>(a, w, qi);
-    add_stats(a, st.ndc, st.expansions, st.overflow);
-}
-
-// Batched router: a persistent grid sized to the resident wave slots pulls
-// query indices from a device work counter until the batch is drained, so
-// queries of uneven work (mixed topk / queue, hop counts) balance across
-// waves and no wave idles at a launch tail.
-template <int K, int CT>
-__global__ __launch_bounds__(kWavesPerBlock * 64) void router_persistent_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds(a);
-    unsigned long long ndc = 0, exps = 0, ovf = 0;
-    while (true) {
-        uint32_t qi = 0;
-        if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
-        qi = readfirst_u(qi);
-        if (qi >= a.nq) break;
-        const QueryStats st = route_one<K, CT>(a, w, qi);
-        ndc += st.ndc;
-        exps += st.expansions;
-        ovf += st.overflow;
-    }
-    add_stats(a, ndc, exps, ovf);
-}
 
 // Clears the launch counters with a kernel rather than a memset node, so a
 // captured launch sequence replays correctly.
@@ -469,34 +59,41 @@ __global__ void fill_invalid_kernel(uint32_t* vids, float* dists, size_t count) 
     }
 }
 
-using KernelFn = void (*)(RouterArgs);
-
-template <int K>
-KernelFn pick_ct(uint32_t C, bool persistent) {
-    switch (C) {
-    case 6: return persistent ? router_persistent_kernel<K, 6> : router_kernel<K, 6>;
-    case 7: return persistent ? router_persistent_kernel<K, 7> : router_kernel<K, 7>;
-    case 8: return persistent ? router_persistent_kernel<K, 8> : router_kernel<K, 8>;
-    default: return persistent ? router_persistent_kernel<K, 0> : router_kernel<K, 0>;
-    }
-}
-
 constexpr uint32_t kMinTail = 16;
 
-int pick_kernel(uint32_t L, uint32_t C, KernelFn* fn, uint32_t* K, bool persistent = false) {
+int metric_family(int metric) { return metric == ARTEA_DOT ? 1 : (metric == ARTEA_COSINE ? 2 : 0); }
+
+// Kernel table for queue size L (candidate-list blocks K: L plus a tie tail
+// of at least kMinTail entries), row chunks C and the index metric.
+int pick_kernels(uint32_t L, uint32_t C, int metric, KernelSet* ks, uint32_t* K) {
     const uint32_t need = L + kMinTail;
-    if (need <= 64) { *K = 1; *fn = pick_ct<1>(C, persistent); }
-    else if (need <= 128) { *K = 2; *fn = pick_ct<2>(C, persistent); }
-    else if (need <= 256) { *K = 4; *fn = pick_ct<4>(C, persistent); }
-    else if (need <= 512) { *K = 8; *fn = pick_ct<8>(C, persistent); }
-    else if (need <= 832) { *K = 13; *fn = pick_ct<13>(C, persistent); }
-    else if (need <= 1024) { *K = 16; *fn = pick_ct<16>(C, persistent); }
+    uint32_t k = 0;
+    if (need <= 64) k = 1;
+    else if (need <= 128) k = 2;
+    else if (need <= 256) k = 4;
+    else if (need <= 512) k = 8;
+    else if (need <= 832) k = 13;
+    else if (need <= 1024) k = 16;
     else {
         set_error("candidate_queue_size " + std::to_string(L) + " exceeds the device maximum of 1008");
         return ARTEA_EINVAL;
     }
+    if (K) *K = k;
+    if (!ks) return ARTEA_OK;
+    switch (metric_family(metric) * 100 + int(k)) {
+#define ARTEA_RK_CASE(MF, KK) \
+    case MF * 100 + KK: *ks = kernels_mf##MF##_k##KK(C); break;
+        ARTEA_RK_LIST(ARTEA_RK_CASE)
+#undef ARTEA_RK_CASE
+    default: set_error("router: no kernel for this queue size"); return ARTEA_EINVAL;
+    }
     return ARTEA_OK;
 }
+
+// Persistent variant used until an autotune picks one: measured at
+// 1,000,000 x 128 L2 on MI355X, the unbounded 32-row kernel is fastest up
+// to 4 blocks and the 3-waves-per-SIMD budget above.
+int default_variant(uint32_t K) { return K <= 4 ? kVarWide : kVarWide3; }
 
 size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisBytes + 512); }
 
@@ -513,11 +110,17 @@ struct artea_router {
     const void* occ_fn = nullptr;  // cached persistent grid size for (kernel, LDS bytes)
     size_t occ_lds = 0;
     uint32_t occ_blocks = 0;
+    int variant = -1;  // persistent kernel variant (-1: default_variant)
 };
 
 struct artea_batched_router {
     artea_router base;
     uint32_t max_topk = 0, max_queue = 0;
+};
+
+struct artea_launch_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -543,7 +146,7 @@ void router_free(artea_router* r) {
 // Common launch path of both routers.  The graph must be attached.
 int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue, const uint32_t* d_topk,
            const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds, uint32_t* d_ov,
-           float* d_od, hipStream_t s, bool persistent) {
+           float* d_od, hipStream_t s, bool persistent, int thr_mode = 0) {
     artea_index* idx = r->idx;
     ARTEA_CHECK_ARG(idx->dg.attached, "router: no graph attached to the index");
     ARTEA_CHECK_ARG((flags & ~uint32_t(ARTEA_EARLY_STOP | ARTEA_L0_ONLY)) == 0, "router: unknown query flags");
@@ -568,7 +171,13 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
                 return ARTEA_EOVERFLOW;
             }
         }
-        ARTEA_HIP_TRY(hipMemcpyAsync(r->d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice, s));
+        // thr_mode: 0 = enqueue the upload on s, 1 = upload synchronously (before a
+        // capture), 2 = already on the device (inside a capture).
+        if (thr_mode == 1) {
+            ARTEA_HIP_TRY(hipMemcpy(r->d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice));
+        } else if (thr_mode == 0) {
+            ARTEA_HIP_TRY(hipMemcpyAsync(r->d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice, s));
+        }
     }
     RouterArgs a{};
     a.vecs = idx->d_vecs;
@@ -603,10 +212,12 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
         a.hmask = uint32_t((uint64_t(1) << B) - 1);
         a.vis_wide = B > 20 ? 1u : 0u;
     }
-    KernelFn fn;
+    KernelSet ks;
     uint32_t K;
-    int rc = pick_kernel(queue, a.C, &fn, &K, persistent);
+    int rc = pick_kernels(queue, a.C, idx->metric, &ks, &K);
     if (rc) return rc;
+    const int variant = r->variant >= 0 ? r->variant : default_variant(K);
+    const KernelFn fn = persistent ? ks.persistent[variant] : ks.plain;
     const size_t lds = lds_bytes(a.D);
     if (persistent) {
         a.work_counter = r->d_counter;
@@ -650,9 +261,7 @@ int artea_router_create(artea_index* idx, uint32_t topk, uint32_t queue, artea_r
         set_error("candidate_queue_size (" + std::to_string(queue) + ") must be >= topk (" + std::to_string(topk) + ")");
         return ARTEA_EINVAL;
     }
-    KernelFn fn;
-    uint32_t K;
-    int rc = pick_kernel(queue, idx->dim_pad / 16, &fn, &K);
+    int rc = pick_kernels(queue, idx->dim_pad / 16, idx->metric, nullptr, nullptr);
     if (rc) return rc;
     auto r = std::make_unique<artea_router>();
     rc = router_init(r.get(), idx, topk, queue);
@@ -736,9 +345,7 @@ int artea_batched_router_create(artea_index* idx, uint32_t max_topk, uint32_t ma
                   std::to_string(max_topk) + ")");
         return ARTEA_EINVAL;
     }
-    KernelFn fn;
-    uint32_t K;
-    int rc = pick_kernel(max_queue, idx->dim_pad / 16, &fn, &K);
+    int rc = pick_kernels(max_queue, idx->dim_pad / 16, idx->metric, nullptr, nullptr);
     if (rc) return rc;
     auto r = std::make_unique<artea_batched_router>();
     rc = router_init(&r->base, idx, max_topk, max_queue);
@@ -769,6 +376,130 @@ int artea_batched_router_query_device(artea_batched_router* r, const float* d_q,
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : r->base.stream;
     return launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
                   d_od, s, true);
+}
+
+int artea_batched_router_set_variant(artea_batched_router* r, int variant) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r, "artea_batched_router_set_variant: null router");
+    ARTEA_CHECK_ARG(variant >= -1 && variant < kNumVariants, "artea_batched_router_set_variant: unknown variant");
+    r->base.variant = variant;
+    return ARTEA_OK;
+}
+
+int artea_batched_router_get_variant(artea_batched_router* r, int* variant) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r && variant, "artea_batched_router_get_variant: null argument");
+    uint32_t K = 0;
+    int rc = pick_kernels(r->max_queue, r->base.idx->dim_pad / 16, r->base.idx->metric, nullptr, &K);
+    if (rc) return rc;
+    *variant = r->base.variant >= 0 ? r->base.variant : default_variant(K);
+    return ARTEA_OK;
+}
+
+int artea_batched_router_autotune(artea_batched_router* r, const float* d_q, uint32_t nq, const uint32_t* d_topk,
+                                  const uint32_t* d_queue, uint32_t flags, const float* thresholds, uint32_t* d_ov,
+                                  float* d_od, float* ms, int* chosen) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r, "artea_batched_router_autotune: null router");
+    ARTEA_CHECK_ARG(nq > 0 && d_q && d_ov && d_od && d_topk && d_queue, "artea_batched_router_autotune: empty or null batch");
+    hipStream_t s = r->base.stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ARTEA_HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error("artea_batched_router_autotune: hipEventCreate failed");
+        return ARTEA_EHIP;
+    }
+    const int saved = r->base.variant;
+    int best = -1, rc = ARTEA_OK;
+    float best_ms = 0.f;
+    for (int v = 0; v < kNumVariants && rc == ARTEA_OK; ++v) {
+        r->base.variant = v;
+        // One untimed launch (occupancy query, caches), then one timed launch.
+        rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
+                    d_od, s, true);
+        if (rc) break;
+        float t = 0.f;
+        if (hipEventRecord(e0, s) != hipSuccess ||
+            (rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags,
+                         thresholds, d_ov, d_od, s, true)) != ARTEA_OK ||
+            hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            if (rc == ARTEA_OK) {
+                set_error("artea_batched_router_autotune: timing failed");
+                rc = ARTEA_EHIP;
+            }
+            break;
+        }
+        if (ms) ms[v] = t;
+        if (best < 0 || t < best_ms) {
+            best = v;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        r->base.variant = saved;
+        return rc;
+    }
+    r->base.variant = best;
+    if (chosen) *chosen = best;
+    return ARTEA_OK;
+}
+
+int artea_batched_router_capture(artea_batched_router* r, const float* d_q, uint32_t nq, const uint32_t* d_topk,
+                                 const uint32_t* d_queue, uint32_t flags, const float* thresholds, uint32_t* d_ov,
+                                 float* d_od, artea_launch_graph** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r && out, "artea_batched_router_capture: null argument");
+    ARTEA_CHECK_ARG(nq == 0 || (d_q && d_ov && d_od && d_topk && d_queue), "artea_batched_router_capture: null buffer");
+    auto g = std::make_unique<artea_launch_graph>();
+    hipStream_t s = r->base.stream;
+    // One eager launch sizes the persistent grid (occupancy query) and puts
+    // the early-stop thresholds on the device, so the captured sequence holds
+    // kernel nodes only: the counter reset and the router.
+    int rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
+                    d_od, s, true, 1);
+    if (rc) return rc;
+    ARTEA_HIP_TRY(hipStreamSynchronize(s));
+    ARTEA_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
+                d_od, s, true, 2);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture failed: ") + hipGetErrorString(ec));
+        return ARTEA_EHIP;
+    }
+    g->graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
+        return ARTEA_EHIP;
+    }
+    *out = g.release();
+    return ARTEA_OK;
+}
+
+int artea_launch_graph_launch(artea_launch_graph* g, void* stream) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(g && g->exec, "artea_launch_graph_launch: null graph");
+    ARTEA_HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return ARTEA_OK;
+}
+
+int artea_launch_graph_destroy(artea_launch_graph* g) {
+    if (!g) return ARTEA_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return ARTEA_OK;
 }
 
 int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint64_t* exps, uint64_t* ovf) {

@@ -1,0 +1,461 @@
+// Device code of the hierarchical graph router: one wavefront per query.
+// Included by router_inst.hip, which instantiates the kernels of one
+// (metric family, candidate-list blocks) pair per compilation.
+//
+// Semantics follow HierarchicalGraphRouter::_query
+// (router/hierarchical_graph_router.hpp:195-248): greedy best-improvement
+// walks on levels top..1 (router/detail/beam_loop.hpp:129-166), then the L0
+// beam search with StdCandidateQueue (router/data_structures/
+// std_candidate_queue.hpp:358-513) and results by (distance, vid).
+//
+// Device design:
+//  * The candidate list W is held in registers: K blocks of 64 (distance,
+//    vid|explored) pairs, entry i in block i/64, lane i%64, sorted by
+//    (distance, vid).  Positions [0, L) are the reference's top-L max-heap T;
+//    positions >= L hold evicted, still-unexplored entries whose distance
+//    equals the current lower bound (the only stale members of the
+//    reference's unexplored heap U that can still be popped: U's top is
+//    popped only while its distance is <= lb, and lb never grows once T is
+//    full).  Popping = first unexplored entry of W.
+//  * try_push's distance-only rejection (`|T| >= L && d >= lb`) is applied to
+//    the candidates of an expansion in CSR order; only candidates below the
+//    current bound are visited serially (ballot + find-first-set).
+//  * The visited table is a lossy direct-mapped cache of vids in LDS.  It
+//    never reports a vertex that was not inserted, and a vertex it forgets
+//    is recomputed: in the beam the recomputed candidate is either still in
+//    W (dropped as a duplicate key) or has distance >= lb with T full (the
+//    rule rejects it), and in a greedy walk a revisited vertex can never be
+//    strictly closer than the current best.  Results therefore equal the
+//    reference's for every cache size; only the distance count can grow.
+//  * Distances: quads of lanes per candidate row, 32 rows per pass (every
+//    row's loads issued before any reduction), see device.hpp
+//    quad_distance2.  Insertions shift the register blocks with DPP
+//    wave_shr:1, and the tail is trimmed once per expansion chunk.
+#pragma once
+
+#include <cfloat>
+
+#include "common.hpp"
+#include "device.hpp"
+#include "router_kernels.hpp"
+
+using namespace artea;
+using namespace artea::dev;
+using namespace artea::rk;
+
+namespace {
+
+constexpr uint32_t kExplored = 0x80000000u;
+constexpr uint32_t kVidMask = 0x7FFFFFFFu;
+
+
+
+// Visited-cache key: h = v * odd mod 2^B is a bijection of [0, 2^B), so the
+// pair (slot = low bits of h, tag = 1 + the remaining bits) identifies v
+// exactly; tag 0 marks an empty slot.  With n <= 2^20 the 8 KiB hold 8192
+// slots of 8-bit tags, otherwise 4096 slots of 16-bit tags (n <= 2^28).
+struct VisKey {
+    uint32_t slot, tag;
+};
+__device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t wide) {
+    const uint32_t h = (v * 0x9E3779B1u) & hmask;
+    return wide ? VisKey{h & 4095u, (h >> 12) + 1} : VisKey{h & 8191u, (h >> 13) + 1};
+}
+__device__ __forceinline__ uint32_t vis_get(const unsigned char* vis, VisKey k, uint32_t wide) {
+    return wide ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[k.slot]) : uint32_t(vis[k.slot]);
+}
+__device__ __forceinline__ void vis_put(unsigned char* vis, VisKey k, uint32_t wide) {
+    if (wide) reinterpret_cast<uint16_t*>(vis)[k.slot] = uint16_t(k.tag);
+    else vis[k.slot] = uint8_t(k.tag);
+}
+
+struct WaveLds {
+    float* q;         // D floats
+    unsigned char* vis;  // kVisBytes
+    uint32_t* cand;   // 64
+    float* cdist;     // 64
+};
+
+// Collect the unvisited, valid ids of a 64-id chunk in row order into
+// cand[0..cnt), inserting them into the visited cache; returns cnt.
+__device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask,
+                                            uint32_t wide) {
+    const bool valid = id < n;
+    const VisKey key = vis_key(id, hmask, wide);
+    const bool fresh = valid && vis_get(w.vis, key, wide) != key.tag;
+    wave_sync();
+    if (fresh) vis_put(w.vis, key, wide);
+    const uint64_t m = ballot(fresh);
+    if (fresh) w.cand[lanes_below(m)] = id;
+    wave_sync();
+    return uint32_t(__popcll(m));
+}
+
+// Distances of cand[0..cnt) into cdist[0..cnt).
+// The metric a kernel of metric family MF evaluates: 0 = L2SQR / L2 (sqrt
+// chosen at run time), 1 = DOT, 2 = COSINE.  Fixing the family at compile
+// time keeps the other families' register demand out of the kernel.
+template <int MF>
+__device__ __forceinline__ int family_metric(int metric) {
+    return MF == 0 ? (metric == kL2 ? int(kL2) : int(kL2Sqr)) : (MF == 1 ? int(kDot) : int(kCosine));
+}
+
+template <int CT, bool WIDE, int MF>
+__device__ __forceinline__ void distances(const RouterArgs& a, const WaveLds& w, uint32_t cnt, float qn2) {
+    const uint32_t lane = lane_id();
+    const int metric = family_metric<MF>(a.metric);
+    const uint32_t sub = lane & 3u;
+    uint32_t g = 0;
+    // WIDE: 32 rows per pass while more than 16 remain: rows g + lane/4 and
+    // g + 16 + lane/4; a quad past the end re-reads its first row (a cache
+    // hit) instead of branching around the loads.  Otherwise 16 rows per pass.
+    for (; WIDE && g + 16 < cnt; g += 32) {
+        const uint32_t ra = g + (lane >> 2), rb = ra + 16;
+        const uint32_t va = w.cand[ra];
+        const uint32_t vb = rb < cnt ? w.cand[rb] : va;
+        float da, db;
+        quad_distance2<CT>(w.q, a.vecs + size_t(va) * a.D, a.vecs + size_t(vb) * a.D, int(a.C), metric, sub, qn2,
+                           &da, &db);
+        if (sub == 0) {
+            w.cdist[ra] = da;
+            if (rb < cnt) w.cdist[rb] = db;
+        }
+    }
+    for (; g < cnt; g += 16) {
+        const uint32_t r = g + (lane >> 2);
+        if (r < cnt) {
+            const uint32_t v = w.cand[r];
+            const float d = quad_distance<CT>(w.q, a.vecs + size_t(v) * a.D, int(a.C), metric, sub, qn2);
+            if (sub == 0) w.cdist[r] = d;
+        }
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ const uint32_t* upper_row(const RouterArgs& a, uint32_t v, uint32_t level) {
+    const uint2 vi = a.vinfo[v];
+    if (vi.x == 0xFFFFFFFFu || level > vi.x || level == 0) return nullptr;
+    return a.ellU + size_t(vi.y + (vi.x - level)) * a.strideU;
+}
+
+template <int K>
+struct CandList {
+    float d[K];
+    uint32_t v[K];
+    uint32_t n;   // entries in W
+    uint32_t L;   // capacity of T
+    float lb;     // distance of W[L-1] when n >= L, else +inf
+    uint32_t overflow;
+
+    __device__ __forceinline__ void init(uint32_t cap) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            d[k] = __int_as_float(0x7F800000);
+            v[k] = 0xFFFFFFFFu;
+        }
+        n = 0;
+        L = cap;
+        lb = __int_as_float(0x7F800000);
+        overflow = 0;
+    }
+    __device__ __forceinline__ bool full() const { return n >= L; }
+
+    __device__ __forceinline__ float get_d(uint32_t idx) const {
+        float r = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (idx / 64 == uint32_t(k)) r = readlane_f(d[k], int(idx % 64));
+        return r;
+    }
+
+    // Tail trimming: W[L..n) keeps only the entries tied with the current
+    // bound.  Trimming after every insertion or once after a run of
+    // insertions leaves the same list (the bound never grows, and a tail
+    // entry tied with the final bound was tied with every bound in between),
+    // so the beam trims once per expansion chunk, and before an insertion
+    // that would otherwise push an entry off the end.
+    __device__ __forceinline__ void trim() {
+        if (n <= L) return;
+        const uint32_t lane = lane_id();
+        uint32_t keep = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = uint32_t(k) * 64 + lane;
+            keep += uint32_t(__popcll(ballot(idx >= L && idx < n && d[k] == lb)));
+        }
+        const uint32_t nn = L + keep;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = uint32_t(k) * 64 + lane;
+            if (idx >= nn) {
+                d[k] = __int_as_float(0x7F800000);
+                v[k] = 0xFFFFFFFFu;
+            }
+        }
+        n = nn;
+    }
+
+    // try_push with the reference's rule already decided by the caller;
+    // inserts (dist, vid) unless the same key is present.  Empty slots hold
+    // (+inf, 0xFFFFFFFF): never below a candidate key, never equal to one.
+    __device__ __forceinline__ void insert(float dist, uint32_t vid) {
+        const uint32_t lane = lane_id();
+        uint32_t p = 0;
+        uint64_t dup = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t vk = v[k] & kVidMask;
+            p += uint32_t(__popcll(ballot(key_less(d[k], vk, dist, vid))));
+            dup |= ballot(d[k] == dist && vk == vid);
+        }
+        if (dup) return;
+        if (n == uint32_t(K) * 64) {
+            trim();
+            if (n == uint32_t(K) * 64) overflow += 1;  // last entry falls off
+        }
+        // Shift [p, n] right by one (DPP wave shift; lane 0 takes the previous
+        // block's lane 63), high blocks first; blocks wholly below p or past
+        // n are unchanged.
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            if (p >= uint32_t(k + 1) * 64 || uint32_t(k) * 64 > n) continue;
+            const int cd = k > 0 ? __builtin_amdgcn_readlane(__float_as_int(d[k - 1]), 63) : 0;
+            const int cv = k > 0 ? __builtin_amdgcn_readlane(int(v[k - 1]), 63) : 0;
+            const float pd = __int_as_float(__builtin_amdgcn_update_dpp(cd, __float_as_int(d[k]), 0x138, 0xF, 0xF, false));
+            const uint32_t pv = uint32_t(__builtin_amdgcn_update_dpp(cv, int(v[k]), 0x138, 0xF, 0xF, false));
+            const uint32_t idx = uint32_t(k) * 64 + lane;
+            d[k] = idx > p ? pd : (idx == p ? dist : d[k]);
+            v[k] = idx > p ? pv : (idx == p ? vid : v[k]);
+        }
+        n = min(n + 1, uint32_t(K) * 64);
+        if (n >= L) lb = get_d(L - 1);
+    }
+
+    // pop_best_unexplored_entry: first unexplored entry; marks it explored.
+    // Every register block is indexed with compile-time constants only
+    // (a data-dependent block index would move the list to scratch memory).
+    __device__ __forceinline__ bool pop(uint32_t* vid) {
+        const uint32_t lane = lane_id();
+        int kk = -1, l = 0;
+        uint32_t found = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
+            if (kk < 0 && b) {
+                kk = k;
+                l = __ffsll((unsigned long long)b) - 1;
+                found = readlane_u(v[k], l);
+            }
+        }
+        if (kk < 0) return false;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k == kk && int(lane) == l) v[k] |= kExplored;
+        *vid = found & kVidMask;
+        return true;
+    }
+
+    // The entry the next pop would return if no insertion precedes it
+    // (first unexplored entry), or 0xFFFFFFFF.
+    __device__ __forceinline__ uint32_t peek() const {
+        uint32_t found = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
+            if (found == 0xFFFFFFFFu && b) found = readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
+        }
+        return found;
+    }
+};
+
+struct QueryStats {
+    uint32_t ndc, expansions, overflow;
+};
+
+template <int K, int CT, int MF, bool WIDE>
+__device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi) {
+    const uint32_t lane = lane_id();
+    const int metric = family_metric<MF>(a.metric);
+    // Per-query work items are clamped to what this launch was sized for:
+    // topk <= row stride, topk <= L <= the router's maximum queue.
+    const uint32_t topk = min(a.topk_arr ? a.topk_arr[qi] : a.topk, a.out_stride);
+    const uint32_t L = max(min(a.queue_arr ? a.queue_arr[qi] : a.queue, a.queue), max(topk, 1u));
+    uint32_t* ov = a.out_vids + size_t(qi) * a.out_stride;
+    float* od = a.out_dists + size_t(qi) * a.out_stride;
+
+    // Query row into LDS.
+    const float* qg = a.queries + size_t(qi) * a.D;
+    for (uint32_t i = lane; i < a.D; i += 64) w.q[i] = qg[i];
+    for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+    wave_sync();
+    const float qn2 = MF == 2 ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
+
+    // Entry point distance (all lanes compute it; quad 0's value is used).
+    uint32_t cur = a.entry;
+    float cd = quad_distance<CT>(w.q, a.vecs + size_t(cur) * a.D, int(a.C), metric, lane & 3u, qn2);
+    cd = readlane_f(cd, 0);
+    uint32_t ndc = 1, expansions = 0;
+
+    if (!(a.flags & ARTEA_L0_ONLY)) {
+        for (uint32_t level = a.top; level >= 1; --level) {
+            // Greedy walk on `level` (greedy_loop_body); seed marked visited.
+            if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
+            wave_sync();
+            const bool early = (a.flags & ARTEA_EARLY_STOP) != 0;
+            const float thr = early ? a.thresholds[level] : 0.f;
+            while (true) {
+                if (early && cd <= thr) break;
+                const uint32_t* row = upper_row(a, cur, level);
+                if (!row) break;
+                uint32_t best_i = 0xFFFFFFFFu;
+                float best_d = cd;
+                uint32_t best_v = cur;
+                for (uint32_t base = 0; base < a.strideU; base += 64) {
+                    const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
+                    if (ballot(id != 0xFFFFFFFFu) == 0) break;
+                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
+                    if (cnt == 0) continue;
+                    distances<CT, false, MF>(a, w, cnt, qn2);
+                    ndc += cnt;
+                    // First strict minimum in row order below the running best.
+                    float dd = lane < cnt ? w.cdist[lane] : __int_as_float(0x7F800000);
+                    uint32_t ii = lane;
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        const float od2 = __shfl_xor(dd, off);
+                        const uint32_t oi = __shfl_xor(ii, off);
+                        if (od2 < dd || (od2 == dd && oi < ii)) {
+                            dd = od2;
+                            ii = oi;
+                        }
+                    }
+                    dd = readlane_f(dd, 0);
+                    ii = readlane_u(ii, 0);
+                    if (dd < best_d) {
+                        best_d = dd;
+                        best_i = ii;
+                        best_v = w.cand[ii];
+                    }
+                    wave_sync();
+                }
+                if (best_i == 0xFFFFFFFFu) break;  // local optimum
+                cur = best_v;
+                cd = best_d;
+            }
+            // visited.clear() after every level (:228-236).
+            for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+            wave_sync();
+        }
+    }
+
+    // L0 beam search from the cursor (beam_search + beam_loop_body).
+    CandList<K> W;
+    W.init(L);
+    W.insert(cd, cur);
+    if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
+    wave_sync();
+    uint32_t pv;
+    // Speculative read of the first 64 ids of the row the next pop will most
+    // likely return, issued before this expansion's distance gathers so the
+    // next expansion usually starts without a dependent row load.
+    uint32_t pre_v = 0xFFFFFFFFu, pre_id = 0xFFFFFFFFu;
+    while (W.pop(&pv)) {
+        expansions++;
+        const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
+        const bool hit = pv == pre_v;
+        const uint32_t first = hit ? pre_id : (lane < a.stride0 ? row[lane] : 0xFFFFFFFFu);
+        pre_v = W.peek();
+        pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
+        for (uint32_t base = 0; base < a.stride0; base += 64) {
+            const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
+            if (ballot(id != 0xFFFFFFFFu) == 0) break;
+            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
+            if (cnt == 0) continue;
+            distances<CT, WIDE, MF>(a, w, cnt, qn2);
+            ndc += cnt;
+            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
+            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
+            wave_sync();
+            uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
+            while (pending) {
+                const bool ok = ((pending >> lane) & 1ull) && (!W.full() || myd < W.lb);
+                const uint64_t m = ballot(ok);
+                if (!m) break;
+                const int i = __ffsll((unsigned long long)m) - 1;
+                pending &= ~((2ull << i) - 1ull);
+                if (i == 63) pending = 0;
+                W.insert(readlane_f(myd, i), readlane_u(myv, i));
+            }
+            W.trim();
+        }
+    }
+
+    // extract_results(min(topk, |T|)), padded with the invalid entry.
+    const uint32_t nt = min(W.n, W.L);
+    const uint32_t kk = min(topk, nt);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t idx = uint32_t(k) * 64 + lane;
+        if (idx < kk) {
+            ov[idx] = W.v[k] & kVidMask;
+            od[idx] = W.d[k];
+        }
+    }
+    for (uint32_t i = kk + lane; i < topk; i += 64) {
+        ov[i] = kInvalidVid;
+        od[i] = FLT_MAX;
+    }
+    return QueryStats{ndc, expansions, W.overflow};
+}
+
+__device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t wid = threadIdx.x / 64;
+    const size_t per_wave = size_t(a.D) * 4 + kVisBytes + 512;
+    unsigned char* base = smem + wid * per_wave;
+    WaveLds w;
+    w.q = reinterpret_cast<float*>(base);
+    w.vis = base + size_t(a.D) * 4;
+    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisBytes);
+    w.cdist = reinterpret_cast<float*>(w.cand + 64);
+    return w;
+}
+
+__device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long long ndc, unsigned long long exps,
+                                          unsigned long long ovf) {
+    if (lane_id() == 0 && a.stats) {
+        atomicAdd(a.stats + 0, ndc);
+        atomicAdd(a.stats + 1, exps);
+        if (ovf) atomicAdd(a.stats + 2, ovf);
+    }
+}
+
+// Plain router: one wavefront per query, the grid covers the batch.
+template <int K, int CT, int MF>
+__global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
+    const WaveLds w = wave_lds(a);
+    const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
+    if (qi >= a.nq) return;
+    const QueryStats st = route_one<K, CT, MF, true>(a, w, qi);
+    add_stats(a, st.ndc, st.expansions, st.overflow);
+}
+
+// Batched router: a persistent grid sized to the resident wave slots pulls
+// query indices from a device work counter until the batch is drained, so
+// queries of uneven work (mixed topk / queue, hop counts) balance across
+// waves and no wave idles at a launch tail.  WAVES = the minimum resident
+// waves per SIMD the register allocation must allow (1 = no constraint);
+// WIDE = 32 rows per distance pass instead of 16.
+template <int K, int CT, int MF, int WAVES, bool WIDE>
+__global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_kernel(RouterArgs a) {
+    const WaveLds w = wave_lds(a);
+    while (true) {
+        uint32_t qi = 0;
+        if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
+        qi = readfirst_u(qi);
+        if (qi >= a.nq) break;
+        const QueryStats st = route_one<K, CT, MF, WIDE>(a, w, qi);
+        add_stats(a, st.ndc, st.expansions, st.overflow);
+    }
+}
+
+}  // namespace

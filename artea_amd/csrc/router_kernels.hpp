@@ -1,0 +1,70 @@
+// Router kernel sets: the launch arguments and the per-(metric family,
+// candidate-list blocks) kernel tables shared by the host code (router.hip)
+// and the instantiation unit (router_inst.hip, compiled once per pair).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace artea {
+namespace rk {
+
+constexpr int kWavesPerBlock = 4;
+constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit tags
+
+struct RouterArgs {
+    const float* vecs;
+    uint32_t n, D, C;
+    int metric;
+    const uint32_t* ell0;
+    uint32_t stride0;
+    const uint2* vinfo;
+    const uint32_t* ellU;
+    uint32_t strideU;
+    uint32_t top, entry;
+    const float* queries;
+    uint32_t nq;
+    uint32_t topk, queue;                         // fixed per launch when arrays are null
+    const uint32_t* topk_arr;                     // per query (batched router)
+    const uint32_t* queue_arr;
+    uint32_t out_stride;
+    uint32_t flags;
+    const float* thresholds;                      // device, top+1 entries
+    uint32_t* out_vids;
+    float* out_dists;
+    unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
+    uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
+    uint32_t vis_wide;                            // 1: 4096 x 16-bit tags (B > 20), 0: 8192 x 8-bit
+    uint32_t* work_counter;                       // persistent mode when non-null
+};
+
+using KernelFn = void (*)(RouterArgs);
+
+// Persistent-kernel variants (the batched router's autotuner picks one):
+// distance rows per pass and the register budget of the compiled kernel.
+enum PersistentVariant : int {
+    kVarNarrow = 0,  // 16 rows per pass, no register bound
+    kVarWide = 1,    // 32 rows per pass, no register bound
+    kVarWide3 = 2,   // 32 rows per pass, registers for 3 waves per SIMD
+    kVarWide4 = 3,   // 32 rows per pass, registers for 4 waves per SIMD
+    kNumVariants = 4
+};
+
+struct KernelSet {
+    KernelFn plain;
+    KernelFn persistent[kNumVariants];
+};
+
+// Metric family 0 = L2SQR / L2, 1 = DOT, 2 = COSINE; K = 64-entry register
+// blocks of the candidate list.
+#define ARTEA_RK_LIST(X) \
+    X(0, 1) X(0, 2) X(0, 4) X(0, 8) X(0, 13) X(0, 16) \
+    X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 13) X(1, 16) \
+    X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(2, 13) X(2, 16)
+#define ARTEA_RK_DECLARE(MF, K) KernelSet kernels_mf##MF##_k##K(uint32_t chunks);
+ARTEA_RK_LIST(ARTEA_RK_DECLARE)
+#undef ARTEA_RK_DECLARE
+
+}  // namespace rk
+}  // namespace artea

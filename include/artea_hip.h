@@ -158,6 +158,40 @@ int artea_batched_router_query_device(artea_batched_router* r, const float* d_qu
 int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint64_t* expansions,
                                     uint64_t* tail_overflows);
 
+/* Kernel variants of the batched router (no reference counterpart): the
+ * distance gather width (16 or 32 candidate rows per pass) and the register
+ * budget the kernel is compiled for (no bound, 3 or 4 resident waves per
+ * SIMD).  Every variant returns the same results; only the speed differs. */
+#define ARTEA_ROUTER_VARIANTS 4
+/* Autotuner: times each variant on this batch (one untimed and one timed
+ * launch each, on the router's stream, synchronously) and keeps the fastest
+ * for the router's later queries.  ms (optional) receives
+ * ARTEA_ROUTER_VARIANTS launch times in milliseconds, *chosen (optional)
+ * the variant kept.  The output buffers hold the batch's results after. */
+int artea_batched_router_autotune(artea_batched_router* r, const float* d_queries, uint32_t nq,
+                                  const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
+                                  const float* thresholds, uint32_t* d_out_vids, float* d_out_dists, float* ms,
+                                  int* chosen);
+/* Pins a variant (0 .. ARTEA_ROUTER_VARIANTS-1), or -1 for the built-in
+ * default for the router's queue size; get returns the variant in use. */
+int artea_batched_router_set_variant(artea_batched_router* r, int variant);
+int artea_batched_router_get_variant(artea_batched_router* r, int* variant);
+
+/* HIP-graph form of artea_batched_router_query_device over fixed device
+ * buffers (no reference counterpart: the reference has no launch graphs).
+ * Capture runs the query once eagerly, then records the launch sequence
+ * (counter-reset kernel + persistent router kernel, kernel nodes only) on
+ * the router's stream; every replay re-runs the query on the buffers'
+ * current contents and refreshes the router's last_stats.  Thresholds are
+ * copied at capture time. */
+typedef struct artea_launch_graph artea_launch_graph;
+int artea_batched_router_capture(artea_batched_router* r, const float* d_queries, uint32_t nq,
+                                 const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
+                                 const float* thresholds, uint32_t* d_out_vids, float* d_out_dists,
+                                 artea_launch_graph** out);
+int artea_launch_graph_launch(artea_launch_graph* g, void* stream);
+int artea_launch_graph_destroy(artea_launch_graph* g);
+
 /* ----------------------------------------------------------------- refine */
 /* PropagateEngine over one dynamic::SingleLayerGraph
  * (refiner/propagate_engine.hpp:38-325, index/dynamic_structure/

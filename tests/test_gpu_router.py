@@ -229,3 +229,89 @@ def test_device_entry_point_matches_host_buffers(orc, small_l2):
     r.batch_query_device(tq.ptr, qs.shape[0], ov.ptr, od.ptr, stream=s.h)
     s.synchronize()
     assert_same(ov.numpy(), od.numpy(), hv, hd, "device api")
+
+
+def test_batched_router_graph_replays_match_eager(orc, small_l2):
+    """Captured launch sequence: every replay, with eager launches and host
+    synchronisations in between, returns the eager result; changing the
+    query buffer's contents between replays changes the result accordingly."""
+    from artea_amd.device import DeviceBuffer, Stream
+
+    g, vecs, qs = small_l2
+    idx = device_index(orc, g, vecs, 0)
+    nq = qs.shape[0]
+    rng = np.random.default_rng(11)
+    topk = rng.choice([10, 100], nq).astype(np.uint32)
+    queue = np.maximum(topk, rng.choice([100, 200], nq)).astype(np.uint32)
+    br = aa.BatchedRouter(idx, 100, 200)
+    tq, tk, tl = DeviceBuffer.from_numpy(qs), DeviceBuffer.from_numpy(topk), DeviceBuffer.from_numpy(queue)
+    ov = DeviceBuffer.empty((nq, 100), np.uint32)
+    od = DeviceBuffer.empty((nq, 100), np.float32)
+    ev = DeviceBuffer.empty((nq, 100), np.uint32)
+    ed = DeviceBuffer.empty((nq, 100), np.float32)
+    s = Stream()
+    br.query_device(tq.ptr, nq, tk.ptr, tl.ptr, ev.ptr, ed.ptr, stream=s.h)
+    s.synchronize()
+    eager_v, eager_d = ev.numpy(), ed.numpy()
+    eager_stats = br.last_stats()
+    graph = br.capture(tq.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr)
+    for rep in range(4):
+        ov.zero()
+        graph.launch(s.h)
+        s.synchronize()
+        mask = np.arange(100)[None, :] < topk[:, None]
+        assert np.array_equal(np.where(mask, ov.numpy(), 0), np.where(mask, eager_v, 0)), "replay %d" % rep
+        assert np.array_equal(np.where(mask, od.numpy(), 0), np.where(mask, eager_d, 0)), "replay %d" % rep
+        assert br.last_stats() == eager_stats
+        # eager work between replays
+        br.query_device(tq.ptr, nq, tk.ptr, tl.ptr, ev.ptr, ed.ptr, stream=s.h)
+        s.synchronize()
+    # new query contents, same buffers: the replay follows them
+    qs2 = qs[::-1].copy()
+    tq.upload(qs2)
+    graph.launch(s.h)
+    s.synchronize()
+    for k in (10, 100):
+        sel = np.where(topk == k)[0]
+        for L in (100, 200):
+            sub = sel[queue[sel] == L]
+            if sub.size == 0:
+                continue
+            pv, pd = aa.HierarchicalGraphRouter(idx, None, k, L).batch_query(qs2[sub])
+            assert_same(ov.numpy()[sub, :k], od.numpy()[sub, :k], pv, pd, "replay after upload k=%d L=%d" % (k, L))
+
+
+@pytest.mark.parametrize("metric,kind,dim", [(0, 0, 128), (2, 1, 100), (1, 2, 96)])
+def test_batched_router_variants_and_autotune_agree(orc, metric, kind, dim):
+    """Every kernel variant (gather width, register budget) and the
+    autotuner's pick return the plain router's results, for each metric
+    family and for list sizes of 2, 4 and 8 register blocks."""
+    from artea_amd.device import DeviceBuffer, Stream
+
+    vecs = aa.generate(kind, 5, 0, 3000, dim)
+    qs = aa.generate(kind, 5, 3, 48, dim)
+    idx = aa.DeviceIndex(vecs, metric)
+    aa.build_full(idx, max_level=2, build_loops=3, triu_iters=2, seed=7)
+    nq = qs.shape[0]
+    s = Stream()
+    tq = DeviceBuffer.from_numpy(qs)
+    for L in (100, 200, 400):
+        topk = np.full(nq, 10, np.uint32)
+        queue = np.full(nq, L, np.uint32)
+        tk, tl = DeviceBuffer.from_numpy(topk), DeviceBuffer.from_numpy(queue)
+        ov = DeviceBuffer.empty((nq, 10), np.uint32)
+        od = DeviceBuffer.empty((nq, 10), np.float32)
+        pv, pd = aa.HierarchicalGraphRouter(idx, None, 10, L).batch_query(qs)
+        br = aa.BatchedRouter(idx, 10, L)
+        for v in range(len(aa.BatchedRouter.VARIANTS)):
+            br.set_variant(v)
+            assert br.variant() == v
+            br.query_device(tq.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr, stream=s.h)
+            s.synchronize()
+            assert_same(ov.numpy(), od.numpy(), pv, pd, "metric %d L %d variant %d" % (metric, L, v))
+        chosen, ms = br.autotune(tq.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr)
+        assert 0 <= chosen < len(ms) and np.all(ms > 0) and br.variant() == chosen
+        assert_same(ov.numpy(), od.numpy(), pv, pd, "metric %d L %d autotuned" % (metric, L))
+        br.set_variant(-1)
+        with pytest.raises(aa.ArteaError):
+            br.set_variant(len(ms))
