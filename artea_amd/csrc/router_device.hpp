@@ -144,6 +144,7 @@ struct CandList {
     uint32_t v[K];
     uint32_t n;   // entries in W
     uint32_t L;   // capacity of T
+    uint32_t lb_block, lb_lane;  // position L-1
     float lb;     // distance of W[L-1] when n >= L, else +inf
     uint32_t overflow;
 
@@ -155,10 +156,21 @@ struct CandList {
         }
         n = 0;
         L = cap;
+        lb_block = (cap - 1) / 64;
+        lb_lane = (cap - 1) % 64;
         lb = __int_as_float(0x7F800000);
         overflow = 0;
     }
     __device__ __forceinline__ bool full() const { return n >= L; }
+
+    // Distance of W[L-1] (block and lane of L-1 fixed per query).
+    __device__ __forceinline__ float bound_d() const {
+        float r = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (lb_block == uint32_t(k)) r = readlane_f(d[k], int(lb_lane));
+        return r;
+    }
 
     __device__ __forceinline__ float get_d(uint32_t idx) const {
         float r = 0.f;
@@ -202,11 +214,14 @@ struct CandList {
         const uint32_t lane = lane_id();
         uint32_t p = 0;
         uint64_t dup = 0;
+        // One ballot per comparison, combined as lane masks: position
+        // p = #{keys below (dist, vid)}, dup = the key is present.
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t vk = v[k] & kVidMask;
-            p += uint32_t(__popcll(ballot(key_less(d[k], vk, dist, vid))));
-            dup |= ballot(d[k] == dist && vk == vid);
+            const uint64_t eq = ballot(d[k] == dist);
+            p += uint32_t(__popcll(ballot(d[k] < dist) | (eq & ballot(vk < vid))));
+            dup |= eq & ballot(vk == vid);
         }
         if (dup) return;
         if (n == uint32_t(K) * 64) {
@@ -228,7 +243,7 @@ struct CandList {
             v[k] = idx > p ? pv : (idx == p ? vid : v[k]);
         }
         n = min(n + 1, uint32_t(K) * 64);
-        if (n >= L) lb = get_d(L - 1);
+        if (n >= L) lb = bound_d();
     }
 
     // pop_best_unexplored_entry: first unexplored entry; marks it explored.
@@ -278,8 +293,8 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     const int metric = family_metric<MF>(a.metric);
     // Per-query work items are clamped to what this launch was sized for:
     // topk <= row stride, topk <= L <= the router's maximum queue.
-    const uint32_t topk = min(a.topk_arr ? a.topk_arr[qi] : a.topk, a.out_stride);
-    const uint32_t L = max(min(a.queue_arr ? a.queue_arr[qi] : a.queue, a.queue), max(topk, 1u));
+    const uint32_t topk = readfirst_u(min(a.topk_arr ? a.topk_arr[qi] : a.topk, a.out_stride));
+    const uint32_t L = readfirst_u(max(min(a.queue_arr ? a.queue_arr[qi] : a.queue, a.queue), max(topk, 1u)));
     uint32_t* ov = a.out_vids + size_t(qi) * a.out_stride;
     float* od = a.out_dists + size_t(qi) * a.out_stride;
 
@@ -377,8 +392,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
             wave_sync();
             uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
             while (pending) {
-                const bool ok = ((pending >> lane) & 1ull) && (!W.full() || myd < W.lb);
-                const uint64_t m = ballot(ok);
+                const uint64_t m = pending & (W.full() ? ballot(myd < W.lb) : ~0ull);
                 if (!m) break;
                 const int i = __ffsll((unsigned long long)m) - 1;
                 pending &= ~((2ull << i) - 1ull);
