@@ -208,29 +208,41 @@ struct CandList {
     }
 
     // try_push with the reference's rule already decided by the caller;
-    // inserts (dist, vid) unless the same key is present.  Empty slots hold
-    // (+inf, 0xFFFFFFFF): never below a candidate key, never equal to one.
-    __device__ __forceinline__ void insert(float dist, uint32_t vid) {
+    // inserts (dist, vid) unless the same key is present and returns whether
+    // it did.  Empty slots hold (+inf, 0xFFFFFFFF): never below a candidate
+    // key, never equal to one.  Position p = #{keys below (dist, vid)}: the
+    // distance ballots decide it unless some entry has the same distance;
+    // only then are vids compared (ties order by vid, and an equal vid means
+    // the key is already present).
+    __device__ __forceinline__ bool insert(float dist, uint32_t vid) {
         const uint32_t lane = lane_id();
         uint32_t p = 0;
-        uint64_t dup = 0;
-        // One ballot per comparison, combined as lane masks: position
-        // p = #{keys below (dist, vid)}, dup = the key is present.
+        uint64_t eq[K], any_eq = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint32_t vk = v[k] & kVidMask;
-            const uint64_t eq = ballot(d[k] == dist);
-            p += uint32_t(__popcll(ballot(d[k] < dist) | (eq & ballot(vk < vid))));
-            dup |= eq & ballot(vk == vid);
+            eq[k] = ballot(d[k] == dist);
+            any_eq |= eq[k];
+            p += uint32_t(__popcll(ballot(d[k] < dist)));
         }
-        if (dup) return;
+        if (any_eq) {
+            uint64_t dup = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (!eq[k]) continue;
+                const uint32_t vk = v[k] & kVidMask;
+                p += uint32_t(__popcll(eq[k] & ballot(vk < vid)));
+                dup |= eq[k] & ballot(vk == vid);
+            }
+            if (dup) return false;
+        }
         if (n == uint32_t(K) * 64) {
             trim();
             if (n == uint32_t(K) * 64) overflow += 1;  // last entry falls off
         }
         // Shift [p, n] right by one (DPP wave shift; lane 0 takes the previous
         // block's lane 63), high blocks first; blocks wholly below p or past
-        // n are unchanged.
+        // n are unchanged.  The new entry is written into lane p % 64 of
+        // block p / 64.
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
             if (p >= uint32_t(k + 1) * 64 || uint32_t(k) * 64 > n) continue;
@@ -238,12 +250,18 @@ struct CandList {
             const int cv = k > 0 ? __builtin_amdgcn_readlane(int(v[k - 1]), 63) : 0;
             const float pd = __int_as_float(__builtin_amdgcn_update_dpp(cd, __float_as_int(d[k]), 0x138, 0xF, 0xF, false));
             const uint32_t pv = uint32_t(__builtin_amdgcn_update_dpp(cv, int(v[k]), 0x138, 0xF, 0xF, false));
-            const uint32_t idx = uint32_t(k) * 64 + lane;
-            d[k] = idx > p ? pd : (idx == p ? dist : d[k]);
-            v[k] = idx > p ? pv : (idx == p ? vid : v[k]);
+            const bool shifted = uint32_t(k) * 64 + lane > p;
+            d[k] = shifted ? pd : d[k];
+            v[k] = shifted ? pv : v[k];
+            if (p >= uint32_t(k) * 64) {
+                const bool at = lane == p - uint32_t(k) * 64;
+                d[k] = at ? dist : d[k];
+                v[k] = at ? vid : v[k];
+            }
         }
         n = min(n + 1, uint32_t(K) * 64);
         if (n >= L) lb = bound_d();
+        return true;
     }
 
     // pop_best_unexplored_entry: first unexplored entry; marks it explored.
@@ -272,13 +290,19 @@ struct CandList {
 
     // The entry the next pop would return if no insertion precedes it
     // (first unexplored entry), or 0xFFFFFFFF.
-    __device__ __forceinline__ uint32_t peek() const {
+    __device__ __forceinline__ uint32_t peek(float* dist) const {
         uint32_t found = 0xFFFFFFFFu;
+        float fd = __int_as_float(0x7F800000);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
-            if (found == 0xFFFFFFFFu && b) found = readlane_u(v[k], __ffsll((unsigned long long)b) - 1) & kVidMask;
+            if (found == 0xFFFFFFFFu && b) {
+                const int l = __ffsll((unsigned long long)b) - 1;
+                found = readlane_u(v[k], l) & kVidMask;
+                fd = readlane_f(d[k], l);
+            }
         }
+        *dist = fd;
         return found;
     }
 };
@@ -372,13 +396,17 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // Speculative read of the first 64 ids of the row the next pop will most
     // likely return, issued before this expansion's distance gathers so the
     // next expansion usually starts without a dependent row load.
+    // An insertion whose key precedes that entry becomes the next pop (unless
+    // a later one of the same expansion precedes it too), so its row read is
+    // issued at once and overlaps the rest of the expansion.
     uint32_t pre_v = 0xFFFFFFFFu, pre_id = 0xFFFFFFFFu;
+    float pre_d = 0.f;
     while (W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
         const bool hit = pv == pre_v;
         const uint32_t first = hit ? pre_id : (lane < a.stride0 ? row[lane] : 0xFFFFFFFFu);
-        pre_v = W.peek();
+        pre_v = W.peek(&pre_d);
         pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
         for (uint32_t base = 0; base < a.stride0; base += 64) {
             const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
@@ -397,7 +425,13 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 const int i = __ffsll((unsigned long long)m) - 1;
                 pending &= ~((2ull << i) - 1ull);
                 if (i == 63) pending = 0;
-                W.insert(readlane_f(myd, i), readlane_u(myv, i));
+                const float dd = readlane_f(myd, i);
+                const uint32_t vv = readlane_u(myv, i);
+                if (W.insert(dd, vv) && (pre_v == 0xFFFFFFFFu || key_less(dd, vv, pre_d, pre_v))) {
+                    pre_v = vv;
+                    pre_d = dd;
+                    pre_id = lane < a.stride0 ? a.ell0[size_t(vv) * a.stride0 + lane] : 0xFFFFFFFFu;
+                }
             }
             W.trim();
         }
