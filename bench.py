@@ -7,7 +7,8 @@ Gaussian mixture (seed 20261204), full-hierarchy index built on this GPU
 l0_min_distance 1), queries of mixed topk {10, 100} (seeded 50/50 draw).  The
 queue size sweep {100, 200, 400, 800} picks the smallest queue with
 recall@100 >= 0.95 against an FP64 brute-force ground truth; the timed steps
-run the batched persistent router at that queue.  The plain router (one
+replay a HIP graph of the batched persistent router at that queue, with the
+kernel variant its autotuner picked on the same batch.  The plain router (one
 launch per topk) is timed beside it, and the oracle (C/OpenMP restatement of
 the reference router) is timed on the host cores for the CPU baseline.
 """
@@ -133,12 +134,17 @@ def main():
         p10 = aa.HierarchicalGraphRouter(idx, None, 10, L)
         p100 = aa.HierarchicalGraphRouter(idx, None, 100, L)
 
-        def run_batched(br=br, ql_d=ql_d):
-            br.query_device(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr, stream=sh)
-
         def run_plain(p10=p10, p100=p100):
             p10.batch_query_device(q10.ptr, len(sel10), o10v.ptr, o10d.ptr, stream=sh)
             p100.batch_query_device(q100.ptr, len(sel100), o100v.ptr, o100d.ptr, stream=sh)
+
+        # Autotune the persistent kernel variant on this batch, then capture
+        # the batched query (counter reset + router) as a HIP graph.
+        variant, var_ms = br.autotune(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr)
+        graph = br.capture(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr)
+
+        def run_batched(graph=graph):
+            graph.launch(sh)
 
         run_batched()
         stream.synchronize()
@@ -153,13 +159,15 @@ def main():
         same = bool(np.array_equal(pv, pred[sel100]) and np.array_equal(o10v.numpy(stream), pred[sel10, :10]))
         sweep.append({"queue": L, "recall@100": round(r100, 4), "recall@10": round(r10, 4),
                       "batched_qps": round(nq / tb), "plain_qps": round(nq / tp), "ndc_per_query":
-                      round(st["ndc"] / nq, 1), "batched_equals_plain": same, "tail_overflows": st["tail_overflows"]})
-        log("queue %d: recall@100 %.4f recall@10 %.4f batched %.0f q/s plain %.0f q/s ndc %.0f same %s" % (
-            L, r100, r10, nq / tb, nq / tp, st["ndc"] / nq, same))
-        routers[L] = (br, p10, p100, run_batched, run_plain, st)
+                      round(st["ndc"] / nq, 1), "batched_equals_plain": same, "tail_overflows": st["tail_overflows"],
+                      "variant": aa.BatchedRouter.VARIANTS[variant],
+                      "variant_ms": {aa.BatchedRouter.VARIANTS[i]: round(float(x), 3) for i, x in enumerate(var_ms)}})
+        log("queue %d: recall@100 %.4f recall@10 %.4f batched %.0f q/s plain %.0f q/s ndc %.0f same %s variant %s" % (
+            L, r100, r10, nq / tb, nq / tp, st["ndc"] / nq, same, aa.BatchedRouter.VARIANTS[variant]))
+        routers[L] = (br, p10, p100, run_batched, run_plain, st, graph)
     ok = [s for s in sweep if s["recall@100"] >= args.target_recall]
     chosen = ok[0]["queue"] if ok else sweep[-1]["queue"]
-    br, p10, p100, run_batched, run_plain, st = routers[chosen]
+    br, p10, p100, run_batched, run_plain, st, graph = routers[chosen]
 
     for _ in range(args.warmup):
         run_batched()
@@ -172,6 +180,15 @@ def main():
     qps_plain = nq / t_plain
     log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
     pred = ov.numpy(stream)
+    pdist = od.numpy(stream)
+    # FP32 router distances against FP64 recomputation (same ids), on the
+    # top-100 queries: the integer-valued data makes every FP32 partial sum
+    # exact, so the expected error is zero.
+    smp = sel100[:200]
+    x = base[pred[smp, :100].astype(np.int64)].astype(np.float64)
+    d64 = ((x - queries[smp][:, None, :].astype(np.float64)) ** 2).sum(-1)
+    dist_err = float(np.max(np.abs(pdist[smp, :100].astype(np.float64) - d64) / np.maximum(d64, 1e-300)))
+    log("router FP32 distance max relative error vs FP64: %.3g" % dist_err)
     if args.calibrate_copy:
         # One plain copy of the base vectors (B bytes in, B bytes out): the
         # counter calibration dispatch for the HBM traffic figure.
@@ -194,6 +211,8 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "router_c1_traffic.json")
     if os.path.exists(tpath) and args.n == 1_000_000 and nq == 10_000 and chosen == 200:
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
+    log("roofline: %.1f GB algorithmic per launch, %.1f GB/s (%.3f of peak)" % (alg_bytes / 1e9, achieved,
+                                                                          achieved / HBM_PEAK_GBS))
 
     cpu = None
     if not args.no_cpu:
@@ -267,6 +286,9 @@ def main():
         "recall@100": next(s["recall@100"] for s in sweep if s["queue"] == chosen),
         "build_seconds": round(build_s, 2),
         "sweep": sweep,
+        "router_variant": aa.BatchedRouter.VARIANTS[br.variant()],
+        "launch": "HIP graph replay (counter-reset kernel + persistent router kernel)",
+        "distance_max_rel_err_vs_fp64": dist_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "router_persistent_kernel (batched router), one launch per step",
