@@ -177,6 +177,28 @@ class DeviceIndex:
     def device_vectors(self) -> int:
         return lib().artea_index_device_vectors(self.h)
 
+    def export_graph(self) -> CompactHierarchicalGraph:
+        """Host copy of the attached graph in the grouped CSR."""
+        g = C.c_void_p()
+        check(lib().artea_index_get_graph(self.h, C.byref(g)))
+        try:
+            vals = [C.c_uint32() for _ in range(5)]
+            nn = C.c_uint64()
+            check(lib().artea_graph_info(g, *[C.byref(x) for x in vals], C.byref(nn)))
+            n, top, ul, bl, entry = [x.value for x in vals]
+            vi = np.ctypeslib.as_array(lib().artea_graph_vertex_info(g), (max(2 * n, 1),))[: 2 * n].copy()
+            offs = []
+            for h in range(top + 1 if n else 0):
+                cnt = C.c_uint64()  # vertices in group h: count * (h + 1) + 1 offsets
+                ptr = lib().artea_graph_group_offsets(g, h, C.byref(cnt))
+                m = cnt.value * (h + 1) + 1
+                offs.append(np.ctypeslib.as_array(ptr, (m,)).copy() if ptr else np.zeros(1, np.uint64))
+            nb = (np.ctypeslib.as_array(lib().artea_graph_nbrs(g), (nn.value,)).copy() if nn.value
+                  else np.zeros(0, np.uint32))
+        finally:
+            lib().artea_graph_destroy(g)
+        return CompactHierarchicalGraph(n, top, ul, bl, entry, vi.reshape(-1, 2), offs, nb)
+
 
 class HierarchicalGraphFileManager:
     """snapshot / restore of HGRA v1 through the device index

@@ -56,6 +56,7 @@ SIGNATURES = {
     "artea_distance_matrix_device": (i32, [i32, P, u32, P, u32, u32, P, P]),
     "artea_generate": (i32, [i32, u64, u32, u64, u64, u32, u32, P, i32]),
     "artea_bruteforce_knn_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P]),
+    "artea_bruteforce_knn_mfma_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P, P]),
     "artea_device_alloc": (i32, [C.c_size_t, PP]),
     "artea_device_free": (i32, [P]),
     "artea_memcpy_htod": (i32, [P, P, C.c_size_t, P]),
@@ -85,7 +86,7 @@ SIGNATURES = {
     "artea_refine_set_join": (i32, [P, i32]),
     "artea_refine_set_recheck_queue": (i32, [P, u32]),
     "artea_refine_last_stats": (i32, [P, P, P, P]),
-    "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, P]),
+    "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, P]),
 }
 
 STATUS = {0: "ARTEA_OK", 1: "ARTEA_EINVAL", 2: "ARTEA_EIO", 3: "ARTEA_EHIP", 4: "ARTEA_ENOMEM", 5: "ARTEA_EOVERFLOW"}

@@ -1228,15 +1228,38 @@ int refine_make(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int me
 
 // FP64 centroid partial sums in a fixed order (deterministic), then argmin
 // of dist(centroid, v) with ties to the smaller vid.
-__global__ void centroid_partial_kernel(const float* vecs, uint32_t n, uint32_t D, uint32_t per_block, double* part) {
+__global__ void centroid_partial_kernel(const float* vecs, const uint32_t* ids, uint32_t n, uint32_t D,
+                                        uint32_t per_block, double* part) {
     const uint32_t b = blockIdx.x;
     const uint32_t lo = b * per_block, hi = min(n, lo + per_block);
     for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
         double s = 0;
-        for (uint32_t v = lo; v < hi; ++v) s += double(vecs[size_t(v) * D + d]);
+        for (uint32_t i = lo; i < hi; ++i) s += double(vecs[size_t(ids[i]) * D + d]);
         part[size_t(b) * D + d] = s;
     }
 }
+// Source level of vertex v: with level_fraction f = 1 every vertex is on
+// every level 0..max_level; otherwise the highest h <= max_level with
+// u(v) < f^h, u(v) a seeded uniform draw in [0, 1) (a geometric level
+// assignment standing in for the r-net insertion, which is not built here).
+uint32_t build_level_of(uint64_t seed, uint32_t v, uint32_t max_level, float f) {
+    if (f >= 1.0f) return max_level;
+    uint64_t z = seed ^ 0x6C6576656C73ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z += uint64_t(v) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = double(z >> 11) * 0x1.0p-53;
+    uint32_t h = 0;
+    double p = double(f);
+    while (h < max_level && u < p) {
+        ++h;
+        p *= double(f);
+    }
+    return h;
+}
+
 __global__ void centroid_final_kernel(const double* part, uint32_t blocks, uint32_t n, uint32_t D, float* out) {
     for (uint32_t d = threadIdx.x; d < D; d += blockDim.x) {
         double s = 0;
@@ -1370,90 +1393,141 @@ int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pai
 
 int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
-                     float l0_min_distance, uint64_t seed, int join_precision, double* seconds_per_level) {
+                     float l0_min_distance, uint64_t seed, int join_precision, float level_fraction,
+                     double* seconds_per_level) {
     clear_error();
     ARTEA_CHECK_ARG(idx, "artea_build_full: null index");
     ARTEA_CHECK_ARG(join_precision == ARTEA_JOIN_FP32 || join_precision == ARTEA_JOIN_BF16,
                     "artea_build_full: join_precision must be ARTEA_JOIN_FP32 or ARTEA_JOIN_BF16");
     ARTEA_CHECK_ARG(max_level < 64, "artea_build_full: max_level must be < 64");
+    ARTEA_CHECK_ARG(level_fraction > 0.0f && level_fraction <= 1.0f, "artea_build_full: level_fraction must be in (0, 1]");
     ARTEA_CHECK_ARG(uint32_t(float(bl_max) * 1.5f) <= kMaxCap && uint32_t(float(ul_max) * 1.5f) <= kMaxCap,
                     "artea_build_full: refine caps (1.5 x max) must be <= 128");
     const uint32_t n = idx->n;
     // Build metric: L2 for L2SQR search (build_infra_dispatch,
     // framework/type_context/infra_dispatcher.hpp:224-230).
     const int bmetric = idx->metric == ARTEA_L2SQR ? ARTEA_L2 : idx->metric;
-    HostCsr g;
-    g.n = n;
-    g.top = max_level;
-    g.ul_max = ul_max;
-    g.bl_max = bl_max;
-    g.highest.assign(n, max_level);
-    g.local.resize(n);
-    for (uint32_t v = 0; v < n; ++v) g.local[v] = v;
-    std::vector<std::vector<uint32_t>> cnts(max_level + 1), ids(max_level + 1);
-    for (uint32_t l = 0; l <= max_level; ++l) {
+    if (seconds_per_level)
+        for (uint32_t l = 0; l <= max_level; ++l) seconds_per_level[l] = 0.0;
+    // Source levels, then the compactor's trim: the first bucket from the top
+    // holding >= 32 vertices becomes the top and higher vertices are demoted
+    // to it (hierarchical_graph_compactor.hpp:118-252).  Levels above the
+    // new top are never refined: the compactor would drop their rows.
+    std::vector<uint32_t> src_h(n);
+    for (uint32_t v = 0; v < n; ++v) src_h[v] = build_level_of(seed, v, max_level, level_fraction);
+    uint32_t src_top = 0;
+    for (uint32_t v = 0; v < n; ++v) src_top = std::max(src_top, src_h[v]);
+    std::vector<uint64_t> bucket_n(src_top + 1, 0);
+    for (uint32_t v = 0; v < n; ++v) bucket_n[src_h[v]]++;
+    uint32_t top = 0;
+    for (uint32_t h = src_top;; --h) {
+        if (bucket_n[h] >= 32) {
+            top = h;
+            break;
+        }
+        if (h == 0) break;
+    }
+    // Rows of every refined level, indexed by vid (members only).
+    std::vector<std::vector<uint64_t>> roff(top + 1);
+    std::vector<std::vector<uint32_t>> rids(top + 1);
+    for (uint32_t l = 0; l <= top; ++l) {
         const auto t0 = std::chrono::steady_clock::now();
+        std::vector<uint32_t> members;
+        for (uint32_t v = 0; v < n; ++v)
+            if (src_h[v] >= l) members.push_back(v);
+        const uint32_t nl = uint32_t(members.size());
+        const bool dense = nl == n;
         const uint32_t slot_cap = l == 0 ? bl_max : ul_max;
         const uint32_t cap = uint32_t(float(slot_cap) * 1.5f);
         artea_refine* r = nullptr;
-        int rc = refine_make(idx->d_vecs, n, idx->dim_pad, bmetric, cap, nullptr, n, seed, &r);
+        int rc = refine_make(idx->d_vecs, n, idx->dim_pad, bmetric, cap, dense ? nullptr : members.data(), nl, seed,
+                             &r);
         if (rc) return rc;
         std::unique_ptr<artea_refine, int (*)(artea_refine*)> guard(r, artea_refine_destroy);
         r->join = join_precision;
         const uint32_t gen = uint32_t(float(cap) * prefill_ratio);
-        if (gen > 0) {
-            rc = do_prefill(r, gen, 0, n, 0, n, l);
+        if (gen > 0 && nl > 0) {
+            rc = do_prefill(r, gen, 0, nl, 0, nl, l);
             if (rc) return rc;
         }
         const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
         rc = do_schedule(r, build_loops, triu_iters, scale_coeffs, shift);
         if (rc) return rc;
         // writeback_layer_from_single_layer_graph: first slot_cap entries.
-        std::vector<uint2> dense(std::max<size_t>(size_t(n) * cap, 1));
-        cnts[l].resize(n);
-        ARTEA_HIP_TRY(hipMemcpyAsync(dense.data(), r->c.rows, size_t(n) * cap * sizeof(uint2), hipMemcpyDeviceToHost,
+        std::vector<uint2> rows(std::max<size_t>(size_t(nl) * cap, 1));
+        std::vector<uint32_t> cnt(std::max<uint32_t>(nl, 1));
+        ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), r->c.rows, size_t(nl) * cap * sizeof(uint2), hipMemcpyDeviceToHost,
                                      r->s));
-        ARTEA_HIP_TRY(hipMemcpyAsync(cnts[l].data(), r->c.cnt, size_t(n) * 4, hipMemcpyDeviceToHost, r->s));
+        ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), r->c.cnt, size_t(nl) * 4, hipMemcpyDeviceToHost, r->s));
         ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
-        ids[l].clear();
-        ids[l].reserve(size_t(n) * slot_cap);
-        for (uint32_t v = 0; v < n; ++v) {
-            cnts[l][v] = std::min(cnts[l][v], slot_cap);
-            for (uint32_t k = 0; k < cnts[l][v]; ++k) ids[l].push_back(dense[size_t(v) * cap + k].x & kMask);
+        roff[l].assign(size_t(n) + 1, 0);
+        for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[i], slot_cap);
+        for (uint32_t v = 0; v < n; ++v) roff[l][v + 1] += roff[l][v];
+        rids[l].resize(roff[l][n]);
+        for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t v = members[i];
+            for (uint64_t k = 0; k < roff[l][v + 1] - roff[l][v]; ++k)
+                rids[l][roff[l][v] + k] = rows[size_t(i) * cap + k].x & kMask;
         }
         if (seconds_per_level)
             seconds_per_level[l] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    // Grouped CSR: one group (h = max_level), rows ordered max_level..0.
-    g.offsets.assign(max_level + 1, std::vector<uint64_t>(1, 0));
-    auto& o = g.offsets[max_level];
-    o.assign(size_t(n) * (max_level + 1) + 1, 0);
-    std::vector<size_t> pos(max_level + 1, 0);
-    uint64_t total = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-        for (uint32_t l = max_level;; --l) {
-            o[size_t(v) * (max_level + 1) + (max_level - l)] = total;
-            const uint32_t c = cnts[l][v];
-            g.nbrs.insert(g.nbrs.end(), ids[l].begin() + pos[l], ids[l].begin() + pos[l] + c);
-            pos[l] += c;
-            total += c;
-            if (l == 0) break;
-        }
+    // Grouped CSR (compactor step 2): buckets below the top keep ascending
+    // vid order; the top bucket holds its own vertices, then the demoted
+    // ones by ascending source level.  Groups are laid out h = 0..top in the
+    // neighbour array, rows of a vertex from its highest level down.
+    HostCsr g;
+    g.n = n;
+    g.top = top;
+    g.ul_max = ul_max;
+    g.bl_max = bl_max;
+    g.highest.assign(n, 0);
+    g.local.assign(n, 0);
+    std::vector<std::vector<uint32_t>> bucket(top + 1);
+    for (uint32_t sh = 0; sh <= src_top; ++sh) {
+        const uint32_t grp = std::min(sh, top);
+        for (uint32_t v = 0; v < n; ++v)
+            if (src_h[v] == sh) {
+                g.highest[v] = grp;
+                g.local[v] = uint32_t(bucket[grp].size());
+                bucket[grp].push_back(v);
+            }
     }
-    o.back() = total;
-    // Entry point: vertex closest to the FP64 centroid (build metric).
-    {
-        const uint32_t per = 4096, blocks = std::max(1u, (n + per - 1) / per);
+    g.offsets.resize(top + 1);
+    uint64_t total = 0;
+    for (uint32_t h = 0; h <= top; ++h) {
+        auto& o = g.offsets[h];
+        o.assign(bucket[h].size() * (h + 1) + 1, 0);
+        for (size_t loc = 0; loc < bucket[h].size(); ++loc) {
+            const uint32_t v = bucket[h][loc];
+            for (uint32_t l = h;; --l) {
+                o[loc * (h + 1) + (h - l)] = total;
+                const uint64_t a = roff[l][v], b = roff[l][v + 1];
+                g.nbrs.insert(g.nbrs.end(), rids[l].begin() + a, rids[l].begin() + b);
+                total += b - a;
+                if (l == 0) break;
+            }
+        }
+        o.back() = total;
+    }
+    // Entry point: the top-bucket vertex closest to the FP64 centroid of the
+    // top bucket (build metric); ties keep the earliest bucket position.
+    if (n > 0) {
+        const std::vector<uint32_t>& tb = bucket[top];
+        const uint32_t per = 4096, blocks = std::max(1u, uint32_t((tb.size() + per - 1) / per));
         double* part = nullptr;
         float* cen = nullptr;
         float* dists = nullptr;
+        uint32_t* d_ids = nullptr;
         ARTEA_HIP_TRY(hipMalloc(&part, size_t(blocks) * idx->dim_pad * 8));
         ARTEA_HIP_TRY(hipMalloc(&cen, size_t(idx->dim_pad) * 4));
         ARTEA_HIP_TRY(hipMalloc(&dists, std::max<size_t>(n, 1) * 4));
-        hipLaunchKernelGGL(centroid_partial_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->d_vecs, n,
-                           idx->dim_pad, per, part);
-        hipLaunchKernelGGL(centroid_final_kernel, dim3(1), dim3(256), 0, idx->stream, part, blocks, n, idx->dim_pad,
-                           cen);
+        ARTEA_HIP_TRY(hipMalloc(&d_ids, std::max<size_t>(tb.size(), 1) * 4));
+        ARTEA_HIP_TRY(hipMemcpyAsync(d_ids, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, idx->stream));
+        hipLaunchKernelGGL(centroid_partial_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->d_vecs, d_ids,
+                           uint32_t(tb.size()), idx->dim_pad, per, part);
+        hipLaunchKernelGGL(centroid_final_kernel, dim3(1), dim3(256), 0, idx->stream, part, blocks,
+                           uint32_t(tb.size()), idx->dim_pad, cen);
         ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
         int rc = artea_distance_matrix_device(bmetric, cen, 1, idx->d_vecs, n, idx->dim_pad, dists, idx->stream);
         if (rc) return rc;
@@ -1462,7 +1536,7 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
         ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
         float best = FLT_MAX;
         uint32_t bv = kNoVertex;
-        for (uint32_t v = 0; v < n; ++v)
+        for (uint32_t v : tb)
             if (h[v] < best) {
                 best = h[v];
                 bv = v;
@@ -1471,6 +1545,7 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
         (void)hipFree(part);
         (void)hipFree(cen);
         (void)hipFree(dists);
+        (void)hipFree(d_ids);
     }
     std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2));
     for (uint32_t v = 0; v < n; ++v) {

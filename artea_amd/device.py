@@ -116,13 +116,25 @@ def device_info():
 
 
 def bruteforce_knn(metric: int, base: DeviceBuffer, n: int, queries: DeviceBuffer, nq: int, dim_pad: int, k: int,
-                   stream: Stream = None):
-    """FP64 exact top-k by (distance, vid); returns host (ids, distances)."""
+                   stream: Stream = None, method: str = "fp64", stats: dict = None):
+    """Exact top-k by (FP64 distance, vid); returns host (ids, distances).
+    method "fp64": the FP64 kernel; "mfma": the BF16 MFMA filter with FP64
+    rechecks (same result; ``stats`` receives its counters)."""
     ov = DeviceBuffer.empty((nq, k), np.uint32)
     od = DeviceBuffer.empty((nq, k), np.float64)
-    check(lib().artea_bruteforce_knn_device(metric, C.c_void_p(base.ptr), n, C.c_void_p(queries.ptr), nq, dim_pad,
-                                            k, C.c_void_p(ov.ptr), C.c_void_p(od.ptr),
-                                            C.c_void_p(stream.h if stream else None)))
+    sh = C.c_void_p(stream.h if stream else None)
+    if method == "mfma":
+        st = np.zeros(3, np.uint64)
+        check(lib().artea_bruteforce_knn_mfma_device(metric, C.c_void_p(base.ptr), n, C.c_void_p(queries.ptr), nq,
+                                                     dim_pad, k, C.c_void_p(ov.ptr), C.c_void_p(od.ptr), sh,
+                                                     st.ctypes.data_as(C.c_void_p)))
+        if stats is not None:
+            stats.update(kept_pairs=int(st[0]), fp64_fallback_queries=int(st[1]), sample_rows=int(st[2]))
+    elif method == "fp64":
+        check(lib().artea_bruteforce_knn_device(metric, C.c_void_p(base.ptr), n, C.c_void_p(queries.ptr), nq, dim_pad,
+                                                k, C.c_void_p(ov.ptr), C.c_void_p(od.ptr), sh))
+    else:
+        raise ValueError("method must be 'fp64' or 'mfma'")
     if stream:
         stream.synchronize()
     return ov.numpy(), od.numpy()

@@ -100,12 +100,14 @@ def max_level_for(n: int) -> int:
 
 
 def build_full(index, ul_max=32, bl_max=64, max_level=None, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-               shift=1.5, l0_min_distance=1.0, seed=0, join=JOIN_FP32):
-    """Full-hierarchy build on the GPU; attaches the graph to ``index``.
+               shift=1.5, l0_min_distance=1.0, seed=0, join=JOIN_FP32, level_fraction=1.0):
+    """Hierarchy build on the GPU; attaches the graph to ``index``.
     ``join`` picks the triangle join's pair-distance path (the graph is the
-    same either way).  Returns seconds per level."""
+    same either way); ``level_fraction`` < 1 draws each vertex's level
+    geometrically (sparse upper layers) instead of putting every vertex on
+    every level.  Returns seconds per level."""
     ml = max_level_for(index.n) if max_level is None else max_level
     secs = np.zeros(ml + 1, np.float64)
     check(lib().artea_build_full(index.h, ul_max, bl_max, ml, build_loops, triu_iters, prefill, scale, shift,
-                                 l0_min_distance, seed, join, _ptr(secs)))
+                                 l0_min_distance, seed, join, level_fraction, _ptr(secs)))
     return secs

@@ -41,9 +41,167 @@ def recall(pred, gt, k):
     return hits / (len(pred) * k)
 
 
+def sweep_queues(aa, idx, q_d, nq, gt_i, topk, queues, stream, target):
+    """Batched router at each queue size (autotuned, HIP-graph replay);
+    returns per-queue rows and the smallest queue reaching the target."""
+    from artea_amd.device import DeviceBuffer, Event
+
+    tk = DeviceBuffer.from_numpy(np.full(nq, topk, np.uint32))
+    ov = DeviceBuffer.empty((nq, topk), np.uint32)
+    od = DeviceBuffer.empty((nq, topk), np.float32)
+    rows = []
+    for L in queues:
+        ql = DeviceBuffer.from_numpy(np.full(nq, L, np.uint32))
+        br = aa.BatchedRouter(idx, topk, L)
+        variant, _ = br.autotune(q_d.ptr, nq, tk.ptr, ql.ptr, ov.ptr, od.ptr)
+        graph = br.capture(q_d.ptr, nq, tk.ptr, ql.ptr, ov.ptr, od.ptr)
+        graph.launch(stream.h)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(3):
+            graph.launch(stream.h)
+        e1.record(stream)
+        stream.synchronize()
+        t = e0.elapsed_ms(e1) / 3 / 1e3
+        pred = ov.numpy(stream)
+        r = recall(pred, gt_i, topk)
+        st = br.last_stats()
+        rows.append({"queue": L, "recall@%d" % topk: round(r, 4), "qps": round(nq / t), "ms": round(t * 1e3, 3),
+                     "ndc_per_query": round(st["ndc"] / nq, 1), "variant": aa.BatchedRouter.VARIANTS[variant]})
+        log("queue %d: recall@%d %.4f, %.0f q/s (%s)" % (L, topk, r, nq / t, aa.BatchedRouter.VARIANTS[variant]))
+    ok = [x for x in rows if x["recall@%d" % topk] >= target]
+    return rows, (ok[0] if ok else rows[-1])
+
+
+def workload_cosine(args):
+    """1,000,000 x 100 (padded 112) unit-norm power-law Gaussian, cosine
+    metric, topk 10 and 100, queue sweep 64..800, FP64 ground truth."""
+    import artea_amd as aa
+    from artea_amd.device import DeviceBuffer, Stream, bruteforce_knn
+    from artea_amd.refine import build_full
+
+    n, nq, dim, seed = args.n, args.nq, 100, 20261203 + 2
+    base = aa.generate(1, seed, 0, n, dim)
+    queries = aa.generate(1, seed, 1, nq, dim)
+    D = base.shape[1]
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.COSINE)
+    t0 = time.time()
+    secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
+                      l0_min_distance=1.0, seed=seed)
+    build_s = time.time() - t0
+    log("cosine build %.1fs (levels %s)" % (build_s, np.round(secs, 2).tolist()))
+    stream = Stream()
+    q_d = DeviceBuffer.from_numpy(queries)
+    base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
+    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.COSINE, base_view, n, q_d, nq, D, 100, stream)
+    out = {"metric": "QPS @ recall>=0.95 (1M x 100 cosine)", "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "f32", "data": "synthetic (power-law-spectrum Gaussian, unit-norm)",
+           "config": {"workload": "cosine search, 1,000,000 x 100 (pad 112), %d queries" % nq, "n": n, "nq": nq},
+           "build_seconds": round(build_s, 2)}
+    for topk in (10, 100):
+        queues = [L for L in (64, 100, 200, 400, 800) if L >= topk]
+        rows, best = sweep_queues(aa, idx, q_d, nq, gt_i[:, :topk], topk, queues, stream, args.target_recall)
+        out["topk%d" % topk] = {"sweep": rows, "chosen": best}
+    # FP32 cosine distances of returned ids against FP64 (norms in FP64).
+    r = aa.HierarchicalGraphRouter(idx, None, 10, 200)
+    pv, pd = r.batch_query(queries[:200])
+    xb = base[pv.astype(np.int64)].astype(np.float64)
+    qq = queries[:200].astype(np.float64)
+    cos64 = 1.0 - (xb * qq[:, None, :]).sum(-1) / np.sqrt((xb * xb).sum(-1) * (qq * qq).sum(-1)[:, None])
+    out["distance_max_abs_err_vs_fp64"] = float(np.max(np.abs(pd.astype(np.float64) - cos64)))
+    out["value"] = out["topk100"]["chosen"]["qps"]
+    print(json.dumps(out))
+
+
+def workload_build(args):
+    """10,000,000 x 96 unit-norm 1024-component mixture, L2: the propagate
+    build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) of layer 0
+    and one upper layer over every vertex, with the FP32 join and with the
+    BF16 MFMA join; edge-set equality, recall@100 of the resulting index,
+    then the serving leg on the same resident index (open-loop Poisson
+    arrivals, topk {10,50,100}, queue {64,128,256,512,800}).  Layer 0 alone
+    does not route on this data: the shift (1.5) exceeds every unit-norm
+    distance, so layer 0 is a plain k-NN graph whose clusters do not link
+    up; the upper layer (shift 0, pruned) connects them."""
+    import hashlib
+
+    import artea_amd as aa
+    from artea_amd.device import DeviceBuffer, Stream, bruteforce_knn
+    from artea_amd.refine import JOIN_BF16, JOIN_FP32, build_full
+
+    n, nq, dim, seed = args.n, args.nq, 96, 20261203 + 3
+    t0 = time.time()
+    base = aa.generate(2, seed, 0, n, dim)
+    queries = aa.generate(2, seed, 1, nq, dim)
+    log("generated %d x %d in %.1fs" % (n, dim, time.time() - t0))
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    builds = {}
+    graphs = {}
+    for name, join in (("fp32", JOIN_FP32), ("bf16", JOIN_BF16)):
+        t0 = time.time()
+        lv = build_full(idx, ul_max=32, bl_max=64, max_level=1, build_loops=15, triu_iters=4, prefill=0.4,
+                        scale=1.10, shift=1.5, l0_min_distance=1.0, seed=seed, join=join)
+        secs = time.time() - t0
+        g = idx.export_graph()
+        digest = hashlib.sha256(g.nbrs.tobytes() + b"".join(o.tobytes() for o in g.group_offsets)).hexdigest()[:16]
+        builds[name] = {"seconds": round(secs, 2), "layer_seconds": np.round(lv, 2).tolist(),
+                        "edges": int(g.nbrs.size), "digest": digest}
+        graphs[name] = g
+        log("%s build: %.1fs (layers %s), %d edges, digest %s" % (name, secs, np.round(lv, 2).tolist(), g.nbrs.size,
+                                                                digest))
+    same = bool(np.array_equal(graphs["fp32"].nbrs, graphs["bf16"].nbrs) and
+                all(np.array_equal(a, b) for a, b in zip(graphs["fp32"].group_offsets, graphs["bf16"].group_offsets)))
+    del graphs
+    stream = Stream()
+    q_d = DeviceBuffer.from_numpy(queries)
+    base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
+    t0 = time.time()
+    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, n, q_d, nq, dim, 100, stream)
+    log("FP64 ground truth in %.1fs" % (time.time() - t0))
+    rows, best = sweep_queues(aa, idx, q_d, nq, gt_i, 100, [100, 200, 400, 800], stream, args.target_recall)
+    out = {"metric": "10M x 96 propagate build seconds (layers 0-1), BF16 vs FP32 join; serving p50/p99 ms",
+           "value": builds["bf16"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "bf16 MFMA join tiles (FP32 accumulate, FP32 rechecks) / f32",
+           "data": "synthetic (1024-component mixture, unit-norm)",
+           "config": {"workload": "layers 0-1 build + serving, %d x 96 L2" % n, "n": n, "nq": nq},
+           "builds": builds, "bf16_speedup": round(builds["fp32"]["seconds"] / builds["bf16"]["seconds"], 3),
+           "edge_sets_identical": same, "search_sweep": rows, "search_chosen": best}
+    if args.serve_requests > 0:
+        from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
+
+        rng = np.random.default_rng(seed)
+        ns = args.serve_requests
+        pick = rng.integers(0, nq, ns)
+        stopk = rng.choice(np.array([10, 50, 100], np.uint32), ns)
+        squeue = rng.choice(np.array([64, 128, 256, 512, 800], np.uint32), ns)
+        squeue = np.maximum(squeue, stopk)
+        srv = Server(idx, max_topk=100, max_queue=800, lanes=4, budget=2048)
+        # saturation: one closed batch of the same mix
+        v, d, lat, wall = srv.run(queries[pick], stopk, squeue, np.zeros(ns))
+        sat = ns / wall
+        hits = sum(len(set(v[i, :stopk[i]].tolist()) & set(gt_i[pick[i], :stopk[i]].tolist())) for i in range(ns))
+        rec = hits / float(stopk.sum())
+        loads = []
+        for i, frac in enumerate((0.25, 0.5, 0.75, 0.9)):
+            rate = frac * sat
+            v, d, lat, wall = srv.run(queries[pick], stopk, squeue, poisson_arrivals(ns, rate, seed=seed + i))
+            loads.append({"offered_qps": round(rate), "achieved_qps": round(ns / wall),
+                          "p50_ms": round(percentile_nearest_rank(lat, 50) * 1e3, 3),
+                          "p99_ms": round(percentile_nearest_rank(lat, 99) * 1e3, 3)})
+            log("serving at %.0f q/s offered: %.0f achieved, p50 %.2f ms p99 %.2f ms" % (
+                rate, ns / wall, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
+        out["serving"] = {"requests_per_load": ns, "saturation_qps": round(sat), "recall_at_topk": round(rec, 4),
+                          "loads": loads}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("search", "cosine", "build"), default="search",
+                    help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
+                         "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -63,6 +221,12 @@ def main():
         print("bench.py measures one GPU: every box here has one MI355X and this path uses no collectives; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cosine":
+        return workload_cosine(args)
+    if args.workload == "build":
+        if args.n == 1_000_000:
+            args.n = 10_000_000
+        return workload_build(args)
 
     import artea_amd as aa
     from artea_amd.device import DeviceBuffer, Event, Stream, bruteforce_knn, device_info

@@ -63,7 +63,8 @@ int artea_graph_snapshot(const artea_graph* g, const char* path);
  * entry_point_vid (:83-132); top is ARTEA_INVALID_LEVEL for an empty graph. */
 int artea_graph_info(const artea_graph* g, uint32_t* n, uint32_t* top, uint32_t* ul_max, uint32_t* bl_max,
                      uint32_t* entry_vid, uint64_t* n_nbrs);
-/* Views valid until the graph is destroyed. */
+/* Views valid until the graph is destroyed.  group_offsets: *count = the
+ * group's vertices; the array holds count*(h+1)+1 offsets. */
 const uint32_t* artea_graph_vertex_info(const artea_graph* g);
 const uint64_t* artea_graph_group_offsets(const artea_graph* g, uint32_t h, uint64_t* count);
 const uint32_t* artea_graph_nbrs(const artea_graph* g);
@@ -249,19 +250,25 @@ int artea_refine_set_recheck_queue(artea_refine* r, uint32_t entries);
  * FP32 chain, pair distances evaluated on the BF16 tiles. */
 int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pairs, uint64_t* approx_pairs);
 
-/* Full-hierarchy build on the GPU, attached to idx: every vertex on levels
- * 0..max_level (the r-net hierarchy when no vertex is absorbed,
- * index/stacked_rgraph/index_factory.hpp:364-381); each level refined by
- * refine_layer with caps floor(1.5 x max), prefill floor(cap x prefill_ratio)
- * from the level's random stream, shift = shifted_coeffs x l0_min_distance on
- * L0 and 0 above; rows written back truncated to bl_max / ul_max and
- * compacted with the FP64-centroid entry point
- * (compactor/hierarchical_graph_compactor.hpp:118-327).  Build metric is L2
- * for an L2SQR index; join_precision is an artea_join_precision.
+/* Hierarchy build on the GPU, attached to idx.  level_fraction = 1: every
+ * vertex on levels 0..max_level (the r-net hierarchy when no vertex is
+ * absorbed, index/stacked_rgraph/index_factory.hpp:364-381).  Below 1:
+ * vertex v's source level is the highest h <= max_level with u(v) <
+ * level_fraction^h (u a seeded uniform draw; a geometric stand-in for the
+ * r-net insertion) and level l is refined as a sparse layer of the vertices
+ * at level >= l.  Each level is refined by refine_layer with caps
+ * floor(1.5 x max), prefill floor(cap x prefill_ratio) from the level's random
+ * stream, shift = shifted_coeffs x l0_min_distance on L0 and 0 above; rows
+ * are written back truncated to bl_max / ul_max and compacted
+ * (compactor/hierarchical_graph_compactor.hpp:118-327: the first bucket from
+ * the top with >= 32 vertices is the top, higher vertices are demoted, entry
+ * = the top-bucket vertex nearest its FP64 centroid).  Build metric is L2 for
+ * an L2SQR index; join_precision is an artea_join_precision.
  * seconds_per_level (optional) gets max_level+1 values. */
 int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
-                     float l0_min_distance, uint64_t seed, int join_precision, double* seconds_per_level);
+                     float l0_min_distance, uint64_t seed, int join_precision, float level_fraction,
+                     double* seconds_per_level);
 
 /* -------------------------------------------------------------- utilities */
 /* dist_func_t on the device (utils/simd_distance.hpp:64-79): dists[i*nb+j]
@@ -277,6 +284,20 @@ int artea_distance_matrix_device(int metric, const float* d_a, uint32_t na, cons
 int artea_bruteforce_knn_device(int metric, const float* d_base, uint32_t n, const float* d_queries, uint32_t nq,
                                 uint32_t dim_pad, uint32_t k, uint32_t* d_out_vids, double* d_out_dists,
                                 void* stream);
+
+/* The same result (ids and FP64 distances identical to
+ * artea_bruteforce_knn_device) through a BF16 MFMA filter: the FP64 kernel
+ * on a prefix sample of the base bounds each query's k-th distance, a BF16
+ * MFMA GEMM (two-term split operands, FP32 accumulation, L2 rows centred on
+ * the base mean) keeps every pair whose proven error interval reaches that
+ * bound, and the kept pairs are recomputed in FP64 and sorted by (distance,
+ * vid); a query with more than 4096 kept pairs is recomputed by the FP64
+ * kernel.  Synchronous on `stream`.  dim_pad > 128 or small bases go to the
+ * FP64 kernel directly.  stats (optional, 3 values): pairs kept, queries
+ * recomputed by the FP64 kernel, sample rows. */
+int artea_bruteforce_knn_mfma_device(int metric, const float* d_base, uint32_t n, const float* d_queries,
+                                     uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* d_out_vids,
+                                     double* d_out_dists, void* stream, uint64_t* stats);
 
 /* Device plumbing for callers without a GPU framework of their own: plain
  * allocations, copies, streams (default priority) and timing events of the

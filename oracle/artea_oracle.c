@@ -1243,37 +1243,78 @@ uint32_t orc_max_level(uint32_t n) {
     return c > 1 ? c : 1;
 }
 
+/* Source level of vertex v: level_fraction f = 1 puts every vertex on every
+ * level 0..max_level; otherwise the highest h <= max_level with u(v) < f^h,
+ * u(v) a seeded uniform draw in [0, 1).  This geometric assignment stands in
+ * for the r-net insertion (index/stacked_rgraph/index_factory.hpp:281-480),
+ * which is not restated here. */
+uint32_t orc_build_level_of(uint64_t seed, uint32_t v, uint32_t max_level, float f) {
+    if (f >= 1.0f) return max_level;
+    uint64_t z = seed ^ 0x6C6576656C73ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z += (uint64_t)v * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * 0x1.0p-53;
+    uint32_t h = 0;
+    double p = (double)f;
+    while (h < max_level && u < p) {
+        ++h;
+        p *= (double)f;
+    }
+    return h;
+}
+
+/* refine_layer for every level of the source hierarchy (level l holds the
+ * vertices with source level >= l, as a sparse layer when not all), rows
+ * written back per vid, then compact_graph (trim, demotion, entry point). */
 orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t D, int build_metric, uint32_t ul_max,
                           uint32_t bl_max, uint32_t max_level, uint32_t build_loops, uint32_t triu_iters,
                           float prefill_ratio, float scale_coeffs, float shifted_coeffs, float l0_min_distance,
-                          uint64_t seed, int nthreads, double* secs) {
+                          uint64_t seed, float level_fraction, int nthreads, double* secs) {
+    uint32_t* highest = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t src_top = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        highest[v] = orc_build_level_of(seed, v, max_level, level_fraction);
+        if (highest[v] > src_top) src_top = highest[v];
+    }
+    uint32_t* members = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
     uint64_t** offs = (uint64_t**)calloc(max_level + 1, sizeof(uint64_t*));
     uint32_t** ids = (uint32_t**)calloc(max_level + 1, sizeof(uint32_t*));
     for (uint32_t l = 0; l <= max_level; ++l) {
+        if (secs) secs[l] = 0.0;
+        offs[l] = (uint64_t*)calloc((uint64_t)n + 1, sizeof(uint64_t));
+        if (l > src_top) {
+            ids[l] = (uint32_t*)malloc(sizeof(uint32_t));
+            continue;
+        }
         const double t0 = omp_get_wtime();
+        uint32_t nl = 0;
+        for (uint32_t v = 0; v < n; ++v)
+            if (highest[v] >= l) members[nl++] = v;
         const uint32_t slot_cap = l == 0 ? bl_max : ul_max;
         /* IndexStructure refine caps: floor(1.5 * max) (artea_graph/index_structure.hpp:52-57) */
         const uint32_t cap = (uint32_t)((float)slot_cap * 1.5f);
-        orc_layer* L = orc_layer_create(vecs, n, D, build_metric, cap, NULL, n);
+        orc_layer* L = orc_layer_create(vecs, n, D, build_metric, cap, nl == n ? NULL : members, nl);
         const uint32_t g = (uint32_t)((float)cap * prefill_ratio);
-        if (g > 0) orc_next_prefill(L, g, 0, n, 0, n, seed, l, nthreads);
+        if (g > 0 && nl > 0) orc_next_prefill(L, g, 0, nl, 0, nl, seed, l, nthreads);
         const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
         orc_refine_schedule(L, build_loops, triu_iters, scale_coeffs, shift, nthreads);
-        offs[l] = (uint64_t*)malloc(sizeof(uint64_t) * ((uint64_t)n + 1));
-        offs[l][0] = 0;
-        for (uint32_t v = 0; v < n; ++v) {
-            const uint32_t c = L->rows[v].n < slot_cap ? L->rows[v].n : slot_cap;
-            offs[l][v + 1] = offs[l][v] + c;
+        for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t c = L->rows[i].n < slot_cap ? L->rows[i].n : slot_cap;
+            offs[l][members[i] + 1] = c;
         }
+        for (uint32_t v = 0; v < n; ++v) offs[l][v + 1] += offs[l][v];
         ids[l] = (uint32_t*)malloc(sizeof(uint32_t) * (offs[l][n] ? offs[l][n] : 1));
-        for (uint32_t v = 0; v < n; ++v)
-            for (uint64_t i = 0; i < offs[l][v + 1] - offs[l][v]; ++i)
-                ids[l][offs[l][v] + i] = L->rows[v].a[i].vid_status & ORC_INVALID_VID;
+        for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t v = members[i];
+            for (uint64_t k = 0; k < offs[l][v + 1] - offs[l][v]; ++k)
+                ids[l][offs[l][v] + k] = L->rows[i].a[k].vid_status & ORC_INVALID_VID;
+        }
         orc_layer_free(L);
         if (secs) secs[l] = omp_get_wtime() - t0;
     }
-    uint32_t* highest = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
-    for (uint32_t v = 0; v < n; ++v) highest[v] = max_level;
     orc_graph* gr = orc_compact_graph(n, max_level, ul_max, bl_max, highest, (const uint64_t* const*)offs,
                                       (const uint32_t* const*)ids, vecs, D, build_metric, NULL, 0);
     for (uint32_t l = 0; l <= max_level; ++l) {
@@ -1283,6 +1324,7 @@ orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t D, int build_m
     free(offs);
     free(ids);
     free(highest);
+    free(members);
     return gr;
 }
 

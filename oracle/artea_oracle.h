@@ -184,12 +184,17 @@ void orc_refine_schedule(orc_layer* L, uint32_t build_loops, uint32_t triu_iters
  * (the reference fills L0 this way; its upper levels start from r-net
  * insertion edges, which this builder replaces by the same prefill).
  * Level l draws stream l.  Returns the compact graph; build metric is L2 or
- * COSINE/DOT as given (the search metric for L2 builds is L2SQR). */
+ * COSINE/DOT as given (the search metric for L2 builds is L2SQR).
+ * level_fraction = 1: every vertex on every level; below 1 each vertex gets
+ * the seeded geometric level of orc_build_level_of, level l is refined as a
+ * sparse layer of the vertices at level >= l, and the compactor trims and
+ * demotes as usual. */
+uint32_t orc_build_level_of(uint64_t seed, uint32_t v, uint32_t max_level, float level_fraction);
 orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t dim_pad, int build_metric,
                           uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                           uint32_t triu_iters, float prefill_ratio, float scale_coeffs,
-                          float shifted_coeffs, float l0_min_distance, uint64_t seed, int nthreads,
-                          double* seconds_per_level);
+                          float shifted_coeffs, float l0_min_distance, uint64_t seed, float level_fraction,
+                          int nthreads, double* seconds_per_level);
 uint32_t orc_max_level(uint32_t n);
 
 /* Counters of the last call (logs merged, pair distances evaluated). */

@@ -98,7 +98,8 @@ def _declare(l):
     d("orc_layer_pair_distances", u64, P)
     d("orc_rand_draw", u32, u64, u64, u32, u32, u32, u32)
     d("orc_bruteforce_knn", None, i32, P, u32, P, u32, u32, u32, P, P, i32)
-    d("orc_build_full", P, P, u32, u32, i32, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, P)
+    d("orc_build_full", P, P, u32, u32, i32, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, f32, i32, P)
+    d("orc_build_level_of", u32, u64, u32, u32, f32)
     d("orc_max_level", u32, u32)
 
 
@@ -418,12 +419,17 @@ def max_level(n):
 
 
 def build_full(vecs, metric=L2, ul_max=32, bl_max=64, max_level_=None, build_loops=15, triu_iters=4,
-               prefill=0.4, scale=1.10, shift=1.5, l0_min_distance=1.0, seed=0, nthreads=0):
-    """Full-hierarchy build (all vertices on every level); returns (Graph, seconds per level)."""
+               prefill=0.4, scale=1.10, shift=1.5, l0_min_distance=1.0, seed=0, level_fraction=1.0, nthreads=0):
+    """Hierarchy build: all vertices on every level (level_fraction 1) or the
+    seeded geometric level assignment; returns (Graph, seconds per level)."""
     v = _f32(vecs)
     n = v.shape[0]
     ml = max_level(n) if max_level_ is None else max_level_
     secs = np.zeros(ml + 1, np.float64)
     h = lib().orc_build_full(_ptr(v), n, v.shape[1], metric, ul_max, bl_max, ml, build_loops, triu_iters, prefill,
-                             scale, shift, l0_min_distance, seed, nthreads, _ptr(secs))
+                             scale, shift, l0_min_distance, seed, level_fraction, nthreads, _ptr(secs))
     return Graph(h), secs
+
+
+def build_level_of(seed: int, v: int, max_level_: int, level_fraction: float) -> int:
+    return int(lib().orc_build_level_of(seed, v, max_level_, level_fraction))

@@ -144,19 +144,23 @@ def test_bf16_join_recheck_queue_overflow(orc, queue):
         assert rows_of(eng) == orc_rows(L)
 
 
-@pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
-def test_full_build_parity(orc, join):
+@pytest.mark.parametrize("join,fraction", [(JOIN_FP32, 1.0), (JOIN_BF16, 1.0), (JOIN_FP32, 0.25), (JOIN_BF16, 0.25)])
+def test_full_build_parity(orc, join, fraction):
     n = 3000
     vecs = aa.generate(0, 13, 0, n, 128)
     idx = aa.DeviceIndex(vecs, 0)
-    build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=17, join=join)
-    g, _ = orc.build_full(vecs, metric=orc.L2, max_level_=2, build_loops=4, triu_iters=2, seed=17, nthreads=8)
+    build_full(idx, max_level=3, build_loops=4, triu_iters=2, seed=17, join=join, level_fraction=fraction)
+    g, _ = orc.build_full(vecs, metric=orc.L2, max_level_=3, build_loops=4, triu_iters=2, seed=17,
+                          level_fraction=fraction, nthreads=8)
     info = idx.info()
     assert info["top"] == g.top and info["entry"] == g.entry
-    hl, lc, groups, nbrs = g.csr()
+    if fraction < 1.0:
+        # sparse upper layers: level 3 (~47 vertices) holds the trimmed top
+        assert g.top == 3 and sum(g.highest(v) == 3 for v in range(n)) >= 32
+    ex = idx.export_graph()
     for v in range(0, n, 7):
-        for lvl in range(3):
-            assert np.array_equal(orc.Graph.level_nbrs(g, v, lvl), g.level_nbrs(v, lvl))
+        for lvl in range(4):
+            assert np.array_equal(ex.fetch_level_nbrs(v, lvl), g.level_nbrs(v, lvl))
     # byte-identical HGRA snapshots
     import tempfile, os
 

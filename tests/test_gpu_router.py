@@ -315,3 +315,42 @@ def test_batched_router_variants_and_autotune_agree(orc, metric, kind, dim):
         br.set_variant(-1)
         with pytest.raises(aa.ArteaError):
             br.set_variant(len(ms))
+
+
+@pytest.mark.parametrize("metric,kind,dim,k", [(0, 0, 128, 100), (0, 0, 128, 1), (3, 2, 96, 10), (1, 1, 100, 50),
+                                               (2, 1, 100, 100), (0, 2, 96, 256)])
+def test_bruteforce_mfma_equals_fp64(metric, kind, dim, k):
+    """The BF16 MFMA filter with FP64 rechecks returns the FP64 kernel's ids
+    and distances exactly (every metric, row widths 96 / 112 / 128)."""
+    from artea_amd.device import DeviceBuffer, Stream, bruteforce_knn
+
+    n, nq = 40000, 300
+    base = aa.generate(kind, 21, 0, n, dim)
+    qs = aa.generate(kind, 21, 1, nq, dim)
+    D = base.shape[1]
+    s = Stream()
+    tb, tq = DeviceBuffer.from_numpy(base), DeviceBuffer.from_numpy(qs)
+    fi, fd = bruteforce_knn(metric, tb, n, tq, nq, D, k, s)
+    st = {}
+    mi, md = bruteforce_knn(metric, tb, n, tq, nq, D, k, s, method="mfma", stats=st)
+    assert st["sample_rows"] > 0, "the MFMA path was not taken"
+    assert st["kept_pairs"] >= nq * k
+    assert np.array_equal(mi, fi) and np.array_equal(md, fd)
+
+
+def test_bruteforce_mfma_overflow_falls_back_to_fp64():
+    """Every pair tied: each query keeps all 12000 rows, more than the 4096
+    candidate slots, so every query is recomputed by the FP64 kernel."""
+    from artea_amd.device import DeviceBuffer, Stream, bruteforce_knn
+
+    row = aa.generate(0, 3, 0, 1, 128)
+    base = np.repeat(row, 12000, axis=0)
+    qs = aa.generate(0, 3, 1, 40, 128)
+    s = Stream()
+    tb, tq = DeviceBuffer.from_numpy(base), DeviceBuffer.from_numpy(qs)
+    st = {}
+    mi, md = bruteforce_knn(0, tb, 12000, tq, 40, 128, 10, s, method="mfma", stats=st)
+    fi, fd = bruteforce_knn(0, tb, 12000, tq, 40, 128, 10, s)
+    assert st["fp64_fallback_queries"] == 40
+    assert np.array_equal(mi, fi) and np.array_equal(md, fd)
+    assert np.array_equal(mi, np.tile(np.arange(10, dtype=np.uint32), (40, 1)))

@@ -498,3 +498,43 @@ def test_prefill_draws_are_uniform_and_in_range(orc):
         assert u not in vids and len(r) <= 38 and len(set(vids)) == len(vids)
         keys = [(e["dist"], e["vid_status"] & 0x7FFFFFFF) for e in r]
         assert keys == sorted(keys)
+
+
+def orc_rows(n, dim=32, seed=3):
+    """Small integer-valued clustered rows (8 centres), float32."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, 128, (8, dim))
+    x = centres[rng.integers(0, 8, n)] + rng.normal(0, 12, (n, dim))
+    return np.clip(np.rint(x), 0, 255).astype(np.float32)
+
+
+def test_geometric_levels_and_compaction_trim(orc):
+    """Seeded geometric level draw: fraction f^h of vertices at level >= h;
+    the compactor keeps the first bucket from the top with >= 32 vertices
+    as the top (index_traits.hpp:79), demotes the rest to it, and picks the
+    entry from the top bucket."""
+    n, f, ml = 20000, 0.25, 6
+    lv = np.array([orc.build_level_of(5, v, ml, f) for v in range(n)])
+    for h in range(1, 4):
+        frac = np.mean(lv >= h)
+        assert abs(frac - f ** h) < 4 * np.sqrt(f ** h / n) + 1e-3, (h, frac)
+    assert all(orc.build_level_of(5, v, ml, 1.0) == ml for v in range(100))
+    vecs = orc_rows(3000)
+    g, _ = orc.build_full(vecs, metric=orc.L2, max_level_=5, build_loops=2, triu_iters=1, seed=9,
+                          level_fraction=0.25, nthreads=4)
+    src = np.array([orc.build_level_of(9, v, 5, 0.25) for v in range(3000)])
+    counts = np.bincount(src, minlength=6)
+    top = max(h for h in range(6) if counts[h] >= 32)
+    assert g.top == top
+    hl = np.array([g.highest(v) for v in range(3000)])
+    assert np.array_equal(hl, np.minimum(src, top))
+    assert hl[g.entry] == top
+    # rows exist exactly on levels 0..highest, every id in range and no self loops
+    for v in range(0, 3000, 37):
+        for lvl in range(top + 1):
+            ids = g.level_nbrs(v, lvl)
+            if lvl > hl[v]:
+                assert ids.size == 0
+            else:
+                assert ids.size > 0 and np.all(ids < 3000) and v not in ids
+                assert np.all(hl[ids] >= lvl)  # neighbours live on the same level
