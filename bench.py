@@ -88,17 +88,19 @@ def workload_cosine(args):
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.COSINE)
     t0 = time.time()
     secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                      l0_min_distance=1.0, seed=seed)
+                      l0_min_distance=1.0, seed=seed, level_fraction=args.level_fraction)
     build_s = time.time() - t0
-    log("cosine build %.1fs (levels %s)" % (build_s, np.round(secs, 2).tolist()))
+    log("cosine build %.1fs (levels %s, top %d)" % (build_s, np.round(secs, 2).tolist(), idx.info()["top"]))
     stream = Stream()
     q_d = DeviceBuffer.from_numpy(queries)
     base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
-    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.COSINE, base_view, n, q_d, nq, D, 100, stream)
+    t0 = time.time()
+    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.COSINE, base_view, n, q_d, nq, D, 100, stream, method="mfma")
+    log("ground truth (BF16 MFMA filter + FP64 recheck) in %.2fs" % (time.time() - t0))
     out = {"metric": "QPS @ recall>=0.95 (1M x 100 cosine)", "unit": "queries/s", "n_gpus": 1,
            "higher_is_better": True, "dtype": "f32", "data": "synthetic (power-law-spectrum Gaussian, unit-norm)",
            "config": {"workload": "cosine search, 1,000,000 x 100 (pad 112), %d queries" % nq, "n": n, "nq": nq},
-           "build_seconds": round(build_s, 2)}
+           "build_seconds": round(build_s, 2), "level_fraction": args.level_fraction}
     for topk in (10, 100):
         queues = [L for L in (64, 100, 200, 400, 800) if L >= topk]
         rows, best = sweep_queues(aa, idx, q_d, nq, gt_i[:, :topk], topk, queues, stream, args.target_recall)
@@ -115,15 +117,16 @@ def workload_cosine(args):
 
 
 def workload_build(args):
-    """10,000,000 x 96 unit-norm 1024-component mixture, L2: the propagate
-    build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) of layer 0
-    and one upper layer over every vertex, with the FP32 join and with the
-    BF16 MFMA join; edge-set equality, recall@100 of the resulting index,
-    then the serving leg on the same resident index (open-loop Poisson
-    arrivals, topk {10,50,100}, queue {64,128,256,512,800}).  Layer 0 alone
-    does not route on this data: the shift (1.5) exceeds every unit-norm
-    distance, so layer 0 is a plain k-NN graph whose clusters do not link
-    up; the upper layer (shift 0, pruned) connects them."""
+    """10,000,000 x 96 unit-norm 1024-component mixture, L2: the hierarchy
+    build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) with sparse
+    upper layers (seeded geometric levels, --level-fraction), with the FP32
+    join and with the BF16 MFMA join; edge-set equality, recall@100 of the
+    resulting index, then the serving leg on the same resident index
+    (open-loop Poisson arrivals, topk {10,50,100}, queue {64,128,256,512,800}).
+    On unit-norm data the r-net's covering radii (1, 2, ...) absorb almost
+    every vertex, so the reference's upper levels are sparse; layer 0 alone
+    does not route here (the 1.5 shift exceeds every distance, leaving a
+    plain k-NN graph whose clusters do not link up)."""
     import hashlib
 
     import artea_amd as aa
@@ -140,8 +143,8 @@ def workload_build(args):
     graphs = {}
     for name, join in (("fp32", JOIN_FP32), ("bf16", JOIN_BF16)):
         t0 = time.time()
-        lv = build_full(idx, ul_max=32, bl_max=64, max_level=1, build_loops=15, triu_iters=4, prefill=0.4,
-                        scale=1.10, shift=1.5, l0_min_distance=1.0, seed=seed, join=join)
+        lv = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
+                        shift=1.5, l0_min_distance=1.0, seed=seed, join=join, level_fraction=args.level_fraction)
         secs = time.time() - t0
         g = idx.export_graph()
         digest = hashlib.sha256(g.nbrs.tobytes() + b"".join(o.tobytes() for o in g.group_offsets)).hexdigest()[:16]
@@ -157,8 +160,10 @@ def workload_build(args):
     q_d = DeviceBuffer.from_numpy(queries)
     base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
     t0 = time.time()
-    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, n, q_d, nq, dim, 100, stream)
-    log("FP64 ground truth in %.1fs" % (time.time() - t0))
+    gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, n, q_d, nq, dim, 100, stream,
+                                method="mfma")
+    gt_s = time.time() - t0
+    log("ground truth (BF16 MFMA filter + FP64 recheck) in %.2fs" % gt_s)
     rows, best = sweep_queues(aa, idx, q_d, nq, gt_i, 100, [100, 200, 400, 800], stream, args.target_recall)
     out = {"metric": "10M x 96 propagate build seconds (layers 0-1), BF16 vs FP32 join; serving p50/p99 ms",
            "value": builds["bf16"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
@@ -166,7 +171,8 @@ def workload_build(args):
            "data": "synthetic (1024-component mixture, unit-norm)",
            "config": {"workload": "layers 0-1 build + serving, %d x 96 L2" % n, "n": n, "nq": nq},
            "builds": builds, "bf16_speedup": round(builds["fp32"]["seconds"] / builds["bf16"]["seconds"], 3),
-           "edge_sets_identical": same, "search_sweep": rows, "search_chosen": best}
+           "edge_sets_identical": same, "top": idx.info()["top"], "level_fraction": args.level_fraction,
+           "ground_truth_s": round(gt_s, 2), "search_sweep": rows, "search_chosen": best}
     if args.serve_requests > 0:
         from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
 
@@ -199,6 +205,10 @@ def workload_build(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--join", choices=("fp32", "bf16"), default="fp32",
+                    help="triangle-join pair distances of the search workload's build (the graph is the same)")
+    ap.add_argument("--level-fraction", type=float, default=None,
+                    help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
     ap.add_argument("--workload", choices=("search", "cosine", "build"), default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index")
@@ -221,6 +231,8 @@ def main():
         print("bench.py measures one GPU: every box here has one MI355X and this path uses no collectives; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.level_fraction is None:
+        args.level_fraction = 1.0 if args.workload == "search" else 1.0 / 32
     if args.workload == "cosine":
         return workload_cosine(args)
     if args.workload == "build":
@@ -250,8 +262,11 @@ def main():
         secs = np.zeros(0)
         log("loaded the index graph from %s (build not timed in this run)" % args.graph_cache)
     else:
+        from artea_amd.refine import JOIN_BF16, JOIN_FP32
+
         secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-                          shift=1.5, l0_min_distance=1.0, seed=seed)
+                          shift=1.5, l0_min_distance=1.0, seed=seed, level_fraction=args.level_fraction,
+                          join=JOIN_BF16 if args.join == "bf16" else JOIN_FP32)
         if args.graph_cache:
             idx.snapshot_hgra(args.graph_cache)
     build_s = time.time() - t0
@@ -265,7 +280,15 @@ def main():
     base_view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
     t0 = time.time()
     gt_i, gt_d = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, args.n, q_d, nq, D, 100, stream)
-    log("FP64 ground truth in %.1fs" % (time.time() - t0))
+    gt_fp64_s = time.time() - t0
+    t0 = time.time()
+    gst = {}
+    mi, md = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, base_view, args.n, q_d, nq, D, 100, stream,
+                            method="mfma", stats=gst)
+    gt_mfma_s = time.time() - t0
+    gt_same = bool(np.array_equal(mi, gt_i) and np.array_equal(md, gt_d))
+    log("ground truth: FP64 kernel %.2fs, BF16 MFMA filter + FP64 recheck %.2fs, identical %s, kept %.0f pairs/query" % (
+        gt_fp64_s, gt_mfma_s, gt_same, gst.get("kept_pairs", 0) / nq))
 
     tk_d = DeviceBuffer.from_numpy(topk_q)
     ov = DeviceBuffer.empty((nq, 100), np.uint32)
@@ -450,6 +473,8 @@ def main():
         "recall@100": next(s["recall@100"] for s in sweep if s["queue"] == chosen),
         "build_seconds": round(build_s, 2),
         "sweep": sweep,
+        "ground_truth": {"fp64_kernel_s": round(gt_fp64_s, 3), "bf16_mfma_s": round(gt_mfma_s, 3),
+                         "identical": gt_same, "kept_pairs_per_query": round(gst.get("kept_pairs", 0) / nq, 1)},
         "router_variant": aa.BatchedRouter.VARIANTS[br.variant()],
         "launch": "HIP graph replay (counter-reset kernel + persistent router kernel)",
         "distance_max_rel_err_vs_fp64": dist_err,
