@@ -95,7 +95,7 @@ int pick_kernels(uint32_t L, uint32_t C, int metric, KernelSet* ks, uint32_t* K)
 // to 4 blocks and the 3-waves-per-SIMD budget above.
 int default_variant(uint32_t K) { return K <= 4 ? kVarWide : kVarWide3; }
 
-size_t lds_bytes(uint32_t D) { return kWavesPerBlock * (size_t(D) * 4 + kVisBytes + 512); }
+size_t lds_bytes(uint32_t D, uint32_t K) { return kWavesPerBlock * wave_lds_bytes(D, K); }
 
 }  // namespace
 
@@ -218,7 +218,12 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     if (rc) return rc;
     const int variant = r->variant >= 0 ? r->variant : default_variant(K);
     const KernelFn fn = persistent ? ks.persistent[variant] : ks.plain;
-    const size_t lds = lds_bytes(a.D);
+    const size_t lds = lds_bytes(a.D, K);
+    if (lds > 65536) {
+        // above the default dynamic-LDS limit (large queues, wide rows)
+        ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          int(lds)));
+    }
     if (persistent) {
         a.work_counter = r->d_counter;
         if (r->occ_fn != reinterpret_cast<const void*>(fn) || r->occ_lds != lds) {

@@ -74,6 +74,8 @@ struct WaveLds {
     unsigned char* vis;  // kVisBytes
     uint32_t* cand;   // 64
     float* cdist;     // 64
+    float* wd;        // 64 K: candidate-list distances by position (merge_chunk)
+    uint32_t* wv;     // 64 K: candidate-list vids by position
 };
 
 // Collect the unvisited, valid ids of a 64-id chunk in row order into
@@ -264,6 +266,135 @@ struct CandList {
         return true;
     }
 
+    // One expansion chunk at once: lanes in P hold the candidates that pass
+    // the distance prefilter, in CSR order.  try_push's rule for the p-th
+    // candidate, |T| < L or d < lb, is the count test
+    //   min(#{list entries with distance <= d_p}, L)
+    //     + #{earlier accepted candidates with distance <= d_p} < L
+    // (T is the L smallest keys of the list plus the accepted ones, and the
+    // bound exceeds d_p exactly when fewer than L of them are <= d_p); a
+    // candidate whose key is already in the list, or equals an earlier
+    // accepted one, is dropped as insert() would drop it.  The accepted
+    // candidates are then merged in one gather: a new entry lands at its list
+    // position plus its rank among the accepted, an old entry at its position
+    // plus the accepted keys below it.  Returns 0 with the list unchanged when
+    // the merge could overflow the register blocks (the caller then inserts
+    // one by one), 1 when nothing was accepted, 2 after a merge; before the
+    // merge on_first(d, vid) receives the smallest accepted key.
+    template <class OnFirst>
+    __device__ int merge_chunk(float myd, uint32_t myv, uint64_t P, const WaveLds& w, OnFirst&& on_first) {
+        constexpr uint32_t CAP = uint32_t(K) * 64;
+        constexpr uint32_t POW2 = K <= 1 ? 64 : (K <= 2 ? 128 : (K <= 4 ? 256 : (K <= 8 ? 512 : 1024)));
+        const uint32_t lane = lane_id();
+        // Every candidate's place in the list at once, from an LDS mirror of
+        // (distance, vid): a binary search for the entries below its
+        // distance, then a scan of the entries tied with it (usually none).
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            w.wd[k * 64 + lane] = d[k];
+            w.wv[k * 64 + lane] = v[k] & kVidMask;
+        }
+        wave_sync();
+        uint32_t lt = 0;
+#pragma unroll
+        for (uint32_t st = POW2 / 2; st >= 1; st >>= 1) {
+            const uint32_t q = lt + st;
+            if (q <= CAP && w.wd[q - 1] < myd) lt = q;
+        }
+        uint32_t le = lt, mypos = lt;  // entries <= d; list keys below (d, vid)
+        bool dupl = false;
+        if ((P >> lane) & 1ull) {
+            while (le < CAP && w.wd[le] == myd) {
+                const uint32_t vj = w.wv[le];
+                mypos += vj < myv ? 1u : 0u;
+                dupl |= vj == myv;
+                ++le;
+            }
+        }
+        const uint64_t dupm = ballot(dupl);
+        uint64_t A = 0;
+        for (uint64_t it = P; it; it &= it - 1) {
+            const int p = __ffsll((unsigned long long)it) - 1;
+            const float dp = readlane_f(myd, p);
+            const uint32_t vp = readlane_u(myv, p);
+            const uint32_t c = min(readlane_u(le, p), L) + uint32_t(__popcll(A & ballot(myd <= dp)));
+            const bool dup = ((dupm >> p) & 1ull) || (A & ballot(myv == vp)) != 0;
+            if (!dup && c < L) A |= 1ull << p;
+        }
+        wave_sync();
+        if (!A) return 1;
+        const uint32_t m = uint32_t(__popcll(A));
+        if (n + m > CAP) return 0;
+        // Smallest accepted key first: the caller's next-pop row read can
+        // then overlap the merge.
+        {
+            const bool in = (A >> lane) & 1ull;
+            const float dmin = wave_min(in ? myd : __int_as_float(0x7F800000));
+            uint64_t c = A & ballot(myd == dmin);
+            uint32_t vmin = readlane_u(myv, __ffsll((unsigned long long)c) - 1);
+            for (c &= c - 1; c; c &= c - 1) vmin = min(vmin, readlane_u(myv, __ffsll((unsigned long long)c) - 1));
+            on_first(dmin, vmin);
+        }
+        // rank among the accepted; accepted entries to LDS in rank order
+        uint32_t rank = 0;
+        for (uint64_t it = A; it; it &= it - 1) {
+            const int a = __ffsll((unsigned long long)it) - 1;
+            rank += key_less(readlane_f(myd, a), readlane_u(myv, a), myd, myv) ? 1u : 0u;
+        }
+        const bool inA = (A >> lane) & 1ull;
+        if (inA) {
+            w.cdist[rank] = myd;
+            w.cand[rank] = myv;
+        }
+        const uint32_t f = mypos + rank;  // final position of an accepted lane
+        uint64_t nm[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) nm[k] = 0;
+        for (uint64_t it = A; it; it &= it - 1) {
+            const uint32_t fa = readlane_u(f, __ffsll((unsigned long long)it) - 1);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if ((fa >> 6) == uint32_t(k)) nm[k] |= 1ull << (fa & 63u);
+        }
+        wave_sync();
+        uint32_t below[K];  // new positions in blocks before k
+        uint32_t acc = 0, fmin = CAP;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            below[k] = acc;
+            acc += uint32_t(__popcll(nm[k]));
+            if (fmin == CAP && nm[k]) fmin = uint32_t(k) * 64 + uint32_t(__ffsll((unsigned long long)nm[k]) - 1);
+        }
+        // Gather, high blocks first (block k reads the old blocks k and k-1).
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            if (uint32_t(k) * 64 + 63 < fmin) continue;
+            const uint32_t c = below[k] + lanes_below(nm[k]);
+            const bool isnew = (nm[k] >> lane) & 1ull;
+            const uint32_t t = uint32_t(k) * 64 + lane - c;
+            const int addr = int((t & 63u) << 2);
+            float od = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(d[k])));
+            uint32_t ov = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(v[k])));
+            if (k > 0) {
+                const float od1 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(d[k - 1])));
+                const uint32_t ov1 = uint32_t(__builtin_amdgcn_ds_bpermute(addr, int(v[k - 1])));
+                const bool here = t >= uint32_t(k) * 64;
+                od = here ? od : od1;
+                ov = here ? ov : ov1;
+            }
+            const uint32_t ri = min(c, 63u);
+            const float nd = w.cdist[ri];
+            const uint32_t nv = w.cand[ri];
+            d[k] = isnew ? nd : od;
+            v[k] = isnew ? nv : ov;
+        }
+        n += m;
+        if (n >= L) lb = bound_d();
+        trim();
+        wave_sync();
+        return 2;
+    }
+
     // pop_best_unexplored_entry: first unexplored entry; marks it explored.
     // Every register block is indexed with compile-time constants only
     // (a data-dependent block index would move the list to scratch memory).
@@ -401,17 +532,25 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // issued at once and overlaps the rest of the expansion.
     uint32_t pre_v = 0xFFFFFFFFu, pre_id = 0xFFFFFFFFu;
     float pre_d = 0.f;
+    // The read is issued after this row's ids are consumed and before its
+    // distance gathers, whose in-order waits then cover it; issued earlier,
+    // the wait on this row's ids would also wait for it.
     while (W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
         const bool hit = pv == pre_v;
         const uint32_t first = hit ? pre_id : (lane < a.stride0 ? row[lane] : 0xFFFFFFFFu);
-        pre_v = W.peek(&pre_d);
-        pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
+        bool predicted = false;
         for (uint32_t base = 0; base < a.stride0; base += 64) {
             const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
             if (ballot(id != 0xFFFFFFFFu) == 0) break;
             const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
+            if (!predicted) {
+                predicted = true;
+                pre_v = W.peek(&pre_d);
+                pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane]
+                                                                     : 0xFFFFFFFFu;
+            }
             if (cnt == 0) continue;
             distances<CT, WIDE, MF>(a, w, cnt, qn2);
             ndc += cnt;
@@ -419,6 +558,17 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
             const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
             wave_sync();
             uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
+            const uint64_t P = pending & (W.full() ? ballot(myd < W.lb) : ~0ull);
+            if (!P) continue;
+            const int merged = W.merge_chunk(myd, myv, P, w, [&](float d0, uint32_t v0) {
+                if (pre_v == 0xFFFFFFFFu || key_less(d0, v0, pre_d, pre_v)) {
+                    pre_v = v0;
+                    pre_d = d0;
+                    pre_id = lane < a.stride0 ? a.ell0[size_t(v0) * a.stride0 + lane] : 0xFFFFFFFFu;
+                }
+            });
+            if (merged) continue;
+            // One by one (distance ties with the list, or a possible overflow).
             while (pending) {
                 const uint64_t m = pending & (W.full() ? ballot(myd < W.lb) : ~0ull);
                 if (!m) break;
@@ -435,6 +585,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
             }
             W.trim();
         }
+        if (!predicted) pre_v = 0xFFFFFFFFu;  // empty row: no read in flight
     }
 
     // extract_results(min(topk, |T|)), padded with the invalid entry.
@@ -455,16 +606,19 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     return QueryStats{ndc, expansions, W.overflow};
 }
 
+template <int K>
 __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wid = threadIdx.x / 64;
-    const size_t per_wave = size_t(a.D) * 4 + kVisBytes + 512;
+    const size_t per_wave = wave_lds_bytes(a.D, K);
     unsigned char* base = smem + wid * per_wave;
     WaveLds w;
     w.q = reinterpret_cast<float*>(base);
     w.vis = base + size_t(a.D) * 4;
     w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisBytes);
     w.cdist = reinterpret_cast<float*>(w.cand + 64);
+    w.wd = w.cdist + 64;
+    w.wv = reinterpret_cast<uint32_t*>(w.wd + 64 * K);
     return w;
 }
 
@@ -480,7 +634,7 @@ __device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long lon
 // Plain router: one wavefront per query, the grid covers the batch.
 template <int K, int CT, int MF>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds(a);
+    const WaveLds w = wave_lds<K>(a);
     const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
     if (qi >= a.nq) return;
     const QueryStats st = route_one<K, CT, MF, true>(a, w, qi);
@@ -495,7 +649,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
 // WIDE = 32 rows per distance pass instead of 16.
 template <int K, int CT, int MF, int WAVES, bool WIDE>
 __global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds(a);
+    const WaveLds w = wave_lds<K>(a);
     while (true) {
         uint32_t qi = 0;
         if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);

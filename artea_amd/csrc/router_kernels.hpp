@@ -13,6 +13,13 @@ namespace rk {
 constexpr int kWavesPerBlock = 4;
 constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit tags
 
+// LDS of one wavefront: query row, visited cache, 64-entry candidate
+// staging (ids and distances), and the (distance, vid) mirror of a K-block
+// candidate list.
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t D, uint32_t K) {
+    return size_t(D) * 4 + kVisBytes + 512 + size_t(K) * 512;
+}
+
 struct RouterArgs {
     const float* vecs;
     uint32_t n, D, C;
