@@ -17,9 +17,10 @@
 //    reference's unexplored heap U that can still be popped: U's top is
 //    popped only while its distance is <= lb, and lb never grows once T is
 //    full).  Popping = first unexplored entry of W.
-//  * try_push's distance-only rejection (`|T| >= L && d >= lb`) is applied to
-//    the candidates of an expansion in CSR order; only candidates below the
-//    current bound are visited serially (ballot + find-first-set).
+//  * try_push's distance-only rejection (`|T| >= L && d >= lb`) prefilters
+//    the 64 candidates of an expansion chunk against the current bound; the
+//    survivors are decided and merged into W together (merge_chunk), with
+//    the same outcome as pushing them one by one in CSR order.
 //  * The visited table is a lossy direct-mapped cache of vids in LDS.  It
 //    never reports a vertex that was not inserted, and a vertex it forgets
 //    is recomputed: in the beam the recomputed candidate is either still in
@@ -29,8 +30,8 @@
 //    reference's for every cache size; only the distance count can grow.
 //  * Distances: quads of lanes per candidate row, 32 rows per pass (every
 //    row's loads issued before any reduction), see device.hpp
-//    quad_distance2.  Insertions shift the register blocks with DPP
-//    wave_shr:1, and the tail is trimmed once per expansion chunk.
+//    quad_distance2.  A merge moves every register block in one
+//    ds_bpermute gather, and the tail is trimmed once per expansion chunk.
 #pragma once
 
 #include <cfloat>
@@ -269,20 +270,27 @@ struct CandList {
     // One expansion chunk at once: lanes in P hold the candidates that pass
     // the distance prefilter, in CSR order.  try_push's rule for the p-th
     // candidate, |T| < L or d < lb, is the count test
-    //   min(#{list entries with distance <= d_p}, L)
-    //     + #{earlier accepted candidates with distance <= d_p} < L
+    //   c_p = min(#{list entries with distance <= d_p}, L)
+    //         + #{earlier accepted candidates with distance <= d_p} < L
     // (T is the L smallest keys of the list plus the accepted ones, and the
     // bound exceeds d_p exactly when fewer than L of them are <= d_p); a
     // candidate whose key is already in the list, or equals an earlier
-    // accepted one, is dropped as insert() would drop it.  The accepted
-    // candidates are then merged in one gather: a new entry lands at its list
-    // position plus its rank among the accepted, an old entry at its position
-    // plus the accepted keys below it.  Returns 0 with the list unchanged when
-    // the merge could overflow the register blocks (the caller then inserts
-    // one by one), 1 when nothing was accepted, 2 after a merge; before the
-    // merge on_first(d, vid) receives the smallest accepted key.
+    // accepted one, is dropped as insert() would drop it.
+    //
+    // The test needs no serial pass: a candidate i before p with
+    // d_i <= d_p that the test rejects has c_p >= c_i >= L, so p is accepted
+    // exactly when the count over ALL earlier candidates not in the list
+    // keeps c_p below L.  Only a vid repeated inside the chunk (a row that
+    // lists a neighbour twice) takes the serial route.  The accepted
+    // candidates are merged in one gather: the accepted key of rank r lands
+    // at its list position plus r, an old entry moves up by the accepted
+    // keys below it.  Entries pushed past the register capacity are dropped:
+    // they lie after the new bound and trim() would drop them as well,
+    // unless one ties with that bound, which is counted as an overflow.
+    // Returns whether anything was accepted; before the merge on_first(d,
+    // vid) receives the smallest accepted key.
     template <class OnFirst>
-    __device__ int merge_chunk(float myd, uint32_t myv, uint64_t P, const WaveLds& w, OnFirst&& on_first) {
+    __device__ bool merge_chunk(float myd, uint32_t myv, uint64_t P, const WaveLds& w, OnFirst&& on_first) {
         constexpr uint32_t CAP = uint32_t(K) * 64;
         constexpr uint32_t POW2 = K <= 1 ? 64 : (K <= 2 ? 128 : (K <= 4 ? 256 : (K <= 8 ? 512 : 1024)));
         const uint32_t lane = lane_id();
@@ -311,66 +319,68 @@ struct CandList {
                 ++le;
             }
         }
-        const uint64_t dupm = ballot(dupl);
-        uint64_t A = 0;
-        for (uint64_t it = P; it; it &= it - 1) {
-            const int p = __ffsll((unsigned long long)it) - 1;
-            const float dp = readlane_f(myd, p);
-            const uint32_t vp = readlane_u(myv, p);
-            const uint32_t c = min(readlane_u(le, p), L) + uint32_t(__popcll(A & ballot(myd <= dp)));
-            const bool dup = ((dupm >> p) & 1ull) || (A & ballot(myv == vp)) != 0;
-            if (!dup && c < L) A |= 1ull << p;
+        const uint64_t Q = P & ~ballot(dupl);
+        // Per lane: candidates of Q before it with distance <= its own,
+        // candidates of Q with a smaller key, and the lanes that repeat the
+        // vid of an earlier candidate of Q (same vid, same distance).
+        uint32_t cnt = 0, rank = 0;
+        uint64_t rep = 0;
+        for (uint64_t it = Q; it; it &= it - 1) {
+            const int i = __ffsll((unsigned long long)it) - 1;
+            const float di = readlane_f(myd, i);
+            const uint32_t vi = readlane_u(myv, i);
+            cnt += (int(lane) > i && di <= myd) ? 1u : 0u;
+            rank += key_less(di, vi, myd, myv) ? 1u : 0u;
+            rep |= ballot(vi == myv) & ((~0ull << i) << 1);
         }
-        wave_sync();
-        if (!A) return 1;
+        uint64_t A = 0;
+        if (!(rep & Q)) {
+            A = Q & ballot(cnt < L - min(le, L));
+            // rank among the accepted: take back the rejected keys below
+            for (uint64_t it = Q & ~A; it; it &= it - 1) {
+                const int r = __ffsll((unsigned long long)it) - 1;
+                rank -= key_less(readlane_f(myd, r), readlane_u(myv, r), myd, myv) ? 1u : 0u;
+            }
+        } else {
+            for (uint64_t it = Q; it; it &= it - 1) {
+                const int p = __ffsll((unsigned long long)it) - 1;
+                const float dp = readlane_f(myd, p);
+                const uint32_t vp = readlane_u(myv, p);
+                const uint32_t c = min(readlane_u(le, p), L) + uint32_t(__popcll(A & ballot(myd <= dp)));
+                if ((A & ballot(myv == vp)) == 0 && c < L) A |= 1ull << p;
+            }
+            rank = 0;
+            for (uint64_t it = A; it; it &= it - 1) {
+                const int a = __ffsll((unsigned long long)it) - 1;
+                rank += key_less(readlane_f(myd, a), readlane_u(myv, a), myd, myv) ? 1u : 0u;
+            }
+        }
+        if (!A) return false;
+        const bool inA = (A >> lane) & 1ull;
         const uint32_t m = uint32_t(__popcll(A));
-        if (n + m > CAP) return 0;
+        const uint32_t f = mypos + rank;  // final position of an accepted lane
         // Smallest accepted key first: the caller's next-pop row read can
         // then overlap the merge.
-        {
-            const bool in = (A >> lane) & 1ull;
-            const float dmin = wave_min(in ? myd : __int_as_float(0x7F800000));
-            uint64_t c = A & ballot(myd == dmin);
-            uint32_t vmin = readlane_u(myv, __ffsll((unsigned long long)c) - 1);
-            for (c &= c - 1; c; c &= c - 1) vmin = min(vmin, readlane_u(myv, __ffsll((unsigned long long)c) - 1));
-            on_first(dmin, vmin);
-        }
-        // rank among the accepted; accepted entries to LDS in rank order
-        uint32_t rank = 0;
-        for (uint64_t it = A; it; it &= it - 1) {
-            const int a = __ffsll((unsigned long long)it) - 1;
-            rank += key_less(readlane_f(myd, a), readlane_u(myv, a), myd, myv) ? 1u : 0u;
-        }
-        const bool inA = (A >> lane) & 1ull;
-        if (inA) {
-            w.cdist[rank] = myd;
-            w.cand[rank] = myv;
-        }
-        const uint32_t f = mypos + rank;  // final position of an accepted lane
-        uint64_t nm[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) nm[k] = 0;
-        for (uint64_t it = A; it; it &= it - 1) {
-            const uint32_t fa = readlane_u(f, __ffsll((unsigned long long)it) - 1);
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                if ((fa >> 6) == uint32_t(k)) nm[k] |= 1ull << (fa & 63u);
+        const int r0 = __ffsll((unsigned long long)ballot(inA && rank == 0)) - 1;
+        on_first(readlane_f(myd, r0), readlane_u(myv, r0));
+        const uint32_t fmin = readlane_u(f, r0);
+        // Accepted entries into the mirror at their final positions, marked
+        // by the top vid bit (mirror vids have it clear).
+        wave_sync();
+        if (inA && f < CAP) {
+            w.wd[f] = myd;
+            w.wv[f] = myv | kExplored;
         }
         wave_sync();
-        uint32_t below[K];  // new positions in blocks before k
-        uint32_t acc = 0, fmin = CAP;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            below[k] = acc;
-            acc += uint32_t(__popcll(nm[k]));
-            if (fmin == CAP && nm[k]) fmin = uint32_t(k) * 64 + uint32_t(__ffsll((unsigned long long)nm[k]) - 1);
-        }
         // Gather, high blocks first (block k reads the old blocks k and k-1).
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
             if (uint32_t(k) * 64 + 63 < fmin) continue;
-            const uint32_t c = below[k] + lanes_below(nm[k]);
-            const bool isnew = (nm[k] >> lane) & 1ull;
+            const float nd = w.wd[k * 64 + lane];
+            const uint32_t nv = w.wv[k * 64 + lane];
+            const bool isnew = (nv & kExplored) != 0;
+            const uint32_t below = k == 0 ? 0u : uint32_t(__popcll(ballot(inA && f < uint32_t(k) * 64)));
+            const uint32_t c = below + lanes_below(ballot(isnew));  // new entries before this position
             const uint32_t t = uint32_t(k) * 64 + lane - c;
             const int addr = int((t & 63u) << 2);
             float od = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(d[k])));
@@ -382,17 +392,15 @@ struct CandList {
                 od = here ? od : od1;
                 ov = here ? ov : ov1;
             }
-            const uint32_t ri = min(c, 63u);
-            const float nd = w.cdist[ri];
-            const uint32_t nv = w.cand[ri];
             d[k] = isnew ? nd : od;
-            v[k] = isnew ? nv : ov;
+            v[k] = isnew ? (nv & kVidMask) : ov;
         }
-        n += m;
+        const bool spill = n + m > CAP;
+        n = min(n + m, CAP);
         if (n >= L) lb = bound_d();
+        if (spill && readlane_f(d[K - 1], 63) == lb) overflow += 1;
         trim();
-        wave_sync();
-        return 2;
+        return true;
     }
 
     // pop_best_unexplored_entry: first unexplored entry; marks it explored.
@@ -557,33 +565,15 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
             const float myd = lane < cnt ? w.cdist[lane] : 0.f;
             const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
             wave_sync();
-            uint64_t pending = cnt == 64 ? ~0ull : ((1ull << cnt) - 1);
-            const uint64_t P = pending & (W.full() ? ballot(myd < W.lb) : ~0ull);
+            const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
             if (!P) continue;
-            const int merged = W.merge_chunk(myd, myv, P, w, [&](float d0, uint32_t v0) {
+            W.merge_chunk(myd, myv, P, w, [&](float d0, uint32_t v0) {
                 if (pre_v == 0xFFFFFFFFu || key_less(d0, v0, pre_d, pre_v)) {
                     pre_v = v0;
                     pre_d = d0;
                     pre_id = lane < a.stride0 ? a.ell0[size_t(v0) * a.stride0 + lane] : 0xFFFFFFFFu;
                 }
             });
-            if (merged) continue;
-            // One by one (distance ties with the list, or a possible overflow).
-            while (pending) {
-                const uint64_t m = pending & (W.full() ? ballot(myd < W.lb) : ~0ull);
-                if (!m) break;
-                const int i = __ffsll((unsigned long long)m) - 1;
-                pending &= ~((2ull << i) - 1ull);
-                if (i == 63) pending = 0;
-                const float dd = readlane_f(myd, i);
-                const uint32_t vv = readlane_u(myv, i);
-                if (W.insert(dd, vv) && (pre_v == 0xFFFFFFFFu || key_less(dd, vv, pre_d, pre_v))) {
-                    pre_v = vv;
-                    pre_d = dd;
-                    pre_id = lane < a.stride0 ? a.ell0[size_t(vv) * a.stride0 + lane] : 0xFFFFFFFFu;
-                }
-            }
-            W.trim();
         }
         if (!predicted) pre_v = 0xFFFFFFFFu;  // empty row: no read in flight
     }
