@@ -210,7 +210,7 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
         while (B < 32 && (uint64_t(1) << B) < idx->n) ++B;
         ARTEA_CHECK_ARG(B <= 28, "router: more than 2^28 vectors");
         a.hmask = uint32_t((uint64_t(1) << B) - 1);
-        a.vis_wide = B > 20 ? 1u : 0u;
+        a.vis_mode = B <= 20 ? 0u : (B <= 27 ? 1u : 2u);
     }
     KernelSet ks;
     uint32_t K;

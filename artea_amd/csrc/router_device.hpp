@@ -51,23 +51,38 @@ constexpr uint32_t kVidMask = 0x7FFFFFFFu;
 
 
 
-// Visited-cache key: h = v * odd mod 2^B is a bijection of [0, 2^B), so the
-// pair (slot = low bits of h, tag = 1 + the remaining bits) identifies v
-// exactly; tag 0 marks an empty slot.  With n <= 2^20 the 8 KiB hold 8192
-// slots of 8-bit tags, otherwise 4096 slots of 16-bit tags (n <= 2^28).
+// Visited cache: two-choice hashing over 8 KiB of codes.  h1 = v * A1 and
+// h2 = v * A2 mod 2^B (odd multipliers) are bijections of [0, 2^B), so a
+// slot (the low bits of h) and a quotient (the remaining bits) identify v
+// exactly.  Choice 1 stores quotient + 1 in its slot, choice 2 stores
+// quotient + 1 + (the number of choice-1 codes), so a stored code also says
+// which choice wrote it; 0 marks an empty slot.  Modes: 0 (n <= 2^20) 8192
+// slots of 8-bit codes, choice 1 = 1..128, choice 2 = 129..255 (a choice-2
+// quotient of 127 has no code: that vertex uses choice 1 only); 1 (n <= 2^27)
+// 4096 slots of 16-bit codes, 1..32768 and 32769..65535; 2 (n <= 2^28) 4096
+// 16-bit slots, choice 1 only (a code of 65536 is stored as the empty 0 and
+// never matches: that vertex is never cached).  A lookup hits only on v's own
+// code, so the cache never reports a vertex that was not inserted.  An
+// insertion takes choice 1 when empty, else choice 2 when empty, else
+// overwrites choice 1.
 struct VisKey {
-    uint32_t slot, tag;
+    uint32_t s1, t1, s2, t2;  // t2 == 0: no second choice
 };
-__device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t wide) {
-    const uint32_t h = (v * 0x9E3779B1u) & hmask;
-    return wide ? VisKey{h & 4095u, (h >> 12) + 1} : VisKey{h & 8191u, (h >> 13) + 1};
+__device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t mode) {
+    const uint32_t h1 = (v * 0x9E3779B1u) & hmask, h2 = (v * 0x85EBCA77u) & hmask;
+    if (mode == 0) {
+        const uint32_t q2 = h2 >> 13;
+        return VisKey{h1 & 8191u, (h1 >> 13) + 1, h2 & 8191u, q2 < 127 ? q2 + 129 : 0u};
+    }
+    const uint32_t q2 = h2 >> 12;
+    return VisKey{h1 & 4095u, (h1 >> 12) + 1, h2 & 4095u, (mode == 1 && q2 < 32767) ? q2 + 32769 : 0u};
 }
-__device__ __forceinline__ uint32_t vis_get(const unsigned char* vis, VisKey k, uint32_t wide) {
-    return wide ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[k.slot]) : uint32_t(vis[k.slot]);
+__device__ __forceinline__ uint32_t vis_load(const unsigned char* vis, uint32_t slot, uint32_t mode) {
+    return mode ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[slot]) : uint32_t(vis[slot]);
 }
-__device__ __forceinline__ void vis_put(unsigned char* vis, VisKey k, uint32_t wide) {
-    if (wide) reinterpret_cast<uint16_t*>(vis)[k.slot] = uint16_t(k.tag);
-    else vis[k.slot] = uint8_t(k.tag);
+__device__ __forceinline__ void vis_store(unsigned char* vis, uint32_t slot, uint32_t code, uint32_t mode) {
+    if (mode) reinterpret_cast<uint16_t*>(vis)[slot] = uint16_t(code);
+    else vis[slot] = uint8_t(code);
 }
 
 struct WaveLds {
@@ -82,12 +97,16 @@ struct WaveLds {
 // Collect the unvisited, valid ids of a 64-id chunk in row order into
 // cand[0..cnt), inserting them into the visited cache; returns cnt.
 __device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask,
-                                            uint32_t wide) {
+                                            uint32_t mode) {
     const bool valid = id < n;
-    const VisKey key = vis_key(id, hmask, wide);
-    const bool fresh = valid && vis_get(w.vis, key, wide) != key.tag;
+    const VisKey key = vis_key(id, hmask, mode);
+    const uint32_t c1 = vis_load(w.vis, key.s1, mode), c2 = vis_load(w.vis, key.s2, mode);
+    const bool fresh = valid && c1 != key.t1 && (key.t2 == 0 || c2 != key.t2);
     wave_sync();
-    if (fresh) vis_put(w.vis, key, wide);
+    if (fresh) {
+        const bool second = c1 != 0 && key.t2 != 0 && c2 == 0;
+        vis_store(w.vis, second ? key.s2 : key.s1, second ? key.t2 : key.t1, mode);
+    }
     const uint64_t m = ballot(fresh);
     if (fresh) w.cand[lanes_below(m)] = id;
     wave_sync();
@@ -477,7 +496,10 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     if (!(a.flags & ARTEA_L0_ONLY)) {
         for (uint32_t level = a.top; level >= 1; --level) {
             // Greedy walk on `level` (greedy_loop_body); seed marked visited.
-            if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
+            if (lane == 0) {
+                const VisKey k0 = vis_key(cur, a.hmask, a.vis_mode);
+                vis_store(w.vis, k0.s1, k0.t1, a.vis_mode);
+            }
             wave_sync();
             const bool early = (a.flags & ARTEA_EARLY_STOP) != 0;
             const float thr = early ? a.thresholds[level] : 0.f;
@@ -491,7 +513,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 for (uint32_t base = 0; base < a.strideU; base += 64) {
                     const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
                     if (ballot(id != 0xFFFFFFFFu) == 0) break;
-                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
+                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_mode);
                     if (cnt == 0) continue;
                     distances<CT, false, MF>(a, w, cnt, qn2);
                     ndc += cnt;
@@ -529,7 +551,10 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     CandList<K> W;
     W.init(L);
     W.insert(cd, cur);
-    if (lane == 0) vis_put(w.vis, vis_key(cur, a.hmask, a.vis_wide), a.vis_wide);
+    if (lane == 0) {
+        const VisKey k0 = vis_key(cur, a.hmask, a.vis_mode);
+        vis_store(w.vis, k0.s1, k0.t1, a.vis_mode);
+    }
     wave_sync();
     uint32_t pv;
     // Speculative read of the first 64 ids of the row the next pop will most
@@ -552,7 +577,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
         for (uint32_t base = 0; base < a.stride0; base += 64) {
             const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
             if (ballot(id != 0xFFFFFFFFu) == 0) break;
-            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_wide);
+            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_mode);
             if (!predicted) {
                 predicted = true;
                 pre_v = W.peek(&pre_d);

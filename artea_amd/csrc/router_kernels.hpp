@@ -11,7 +11,7 @@ namespace artea {
 namespace rk {
 
 constexpr int kWavesPerBlock = 4;
-constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit tags
+constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit codes
 
 // LDS of one wavefront: query row, visited cache, 64-entry candidate
 // staging (ids and distances), and the (distance, vid) mirror of a K-block
@@ -42,7 +42,7 @@ struct RouterArgs {
     float* out_dists;
     unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
     uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
-    uint32_t vis_wide;                            // 1: 4096 x 16-bit tags (B > 20), 0: 8192 x 8-bit
+    uint32_t vis_mode;                            // visited cache: 0 8192 x 8-bit (n <= 2^20), 1 4096 x 16-bit (n <= 2^27), 2 as 1 with one choice
     uint32_t* work_counter;                       // persistent mode when non-null
 };
 
