@@ -177,6 +177,17 @@ class DeviceIndex:
     def device_vectors(self) -> int:
         return lib().artea_index_device_vectors(self.h)
 
+    @property
+    def u8_rows(self) -> bool:
+        """Whether the index keeps exact one-byte rows (every value a whole number in [0, 255])."""
+        return bool(lib().artea_index_u8_rows(self.h))
+
+    def set_u8_rows(self, enable: bool) -> bool:
+        """Drop (False) or rebuild (True, when the values allow) the one-byte rows; returns whether they exist."""
+        on = C.c_int()
+        check(lib().artea_index_set_u8_rows(self.h, int(bool(enable)), C.byref(on)))
+        return bool(on.value)
+
     def export_graph(self) -> CompactHierarchicalGraph:
         """Host copy of the attached graph in the grouped CSR."""
         g = C.c_void_p()
@@ -314,14 +325,21 @@ class BatchedRouter:
         check(lib().artea_batched_router_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ndc": a.value, "expansions": b.value, "tail_overflows": c.value}
 
+    # Kernel variant names by slot: FP32 rows (0..3), and one-byte rows
+    # (4..7, while the index keeps them: every value a whole number in [0, 255]).
     VARIANTS = ("narrow", "wide", "wide-3waves", "wide-4waves")
+    VARIANTS_U8 = ("u8-64rows", "u8-32rows", "u8-16rows", "u8-64rows-4waves")
+
+    @property
+    def variant_names(self):
+        return self.VARIANTS + self.VARIANTS_U8 if self.index.u8_rows else self.VARIANTS
 
     def autotune(self, d_queries: int, nq: int, d_topk: int, d_queue: int, d_out_vids: int, d_out_dists: int,
                  flags: int = 0, thresholds=None):
         """Times every kernel variant on this batch and keeps the fastest;
         returns (chosen variant, per-variant milliseconds)."""
         thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.float32)
-        ms = np.zeros(len(self.VARIANTS), np.float32)
+        ms = np.zeros(len(self.variant_names), np.float32)
         ch = C.c_int(-1)
         check(lib().artea_batched_router_autotune(self.h, C.c_void_p(d_queries), nq, C.c_void_p(d_topk),
                                                   C.c_void_p(d_queue), flags, _ptr(thr) if thr is not None else None,

@@ -37,6 +37,8 @@ SIGNATURES = {
     "artea_index_snapshot_hgra": (i32, [P, C.c_char_p]),
     "artea_index_info": (i32, [P, P, P, P, P, P, P]),
     "artea_index_device_vectors": (P, [P]),
+    "artea_index_set_u8_rows": (i32, [P, i32, P]),
+    "artea_index_u8_rows": (i32, [P]),
     "artea_index_destroy": (i32, [P]),
     "artea_router_create": (i32, [P, u32, u32, PP]),
     "artea_router_destroy": (i32, [P]),

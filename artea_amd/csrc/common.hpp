@@ -34,6 +34,7 @@ struct artea_index {
     uint32_t n = 0, dim_pad = 0;
     int metric = 0;
     float* d_vecs = nullptr;
+    uint8_t* d_u8 = nullptr;  // 8-bit copy of d_vecs when every value is an integer in [0, 255]
     DeviceGraph dg;
     artea::HostCsr csr;  // host copy for snapshot / info
     hipStream_t stream = nullptr;

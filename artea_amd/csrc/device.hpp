@@ -254,6 +254,76 @@ __device__ __forceinline__ void quad_distance2(const float* __restrict__ q, cons
     }
 }
 
+// ---- one-byte rows -------------------------------------------------------
+// When every value of a row and of the query is a whole number in [0, 255]
+// and a row holds at most 256 values, every operation of the FP32 chains
+// above is exact (differences, squares and products are integers below 2^16,
+// partial sums stay below 2^24), so the chains' results are the exact
+// integer sums whatever their order.  Either zero qualifies: the chains'
+// accumulators start at +0, a zero product or square added to +0 gives +0,
+// and a difference of zeros is squared, so no -0 reaches a result.  The
+// one-byte path computes those sums in integers: a quad of lanes per row,
+// lane `sub` holding CT words (4*CT bytes) of it, v_dot4_u32_u8 for q.x and
+// x.x, quad sums through DPP.
+__device__ __forceinline__ bool byte_valued(float f) {
+    return f >= 0.f && f <= 255.f && f == __builtin_truncf(f);
+}
+
+__device__ __forceinline__ uint32_t quad_sum_u(uint32_t x) {
+    x += uint32_t(__builtin_amdgcn_mov_dpp(int(x), 0x4E, 0xF, 0xF, false));
+    x += uint32_t(__builtin_amdgcn_mov_dpp(int(x), 0xB1, 0xF, 0xF, false));
+    return x;
+}
+
+// CT words of one row slice (16-, 8- or 4-byte loads, as the slice's
+// alignment allows: slices start at 4*CT*sub bytes of a 16*CT-byte row).
+template <int CT>
+__device__ __forceinline__ void load_u8_slice(const uint32_t* __restrict__ p, uint32_t (&x)[CT]) {
+    if constexpr (CT % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < CT / 4; ++i) {
+            const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+            x[4 * i] = v.x, x[4 * i + 1] = v.y, x[4 * i + 2] = v.z, x[4 * i + 3] = v.w;
+        }
+    } else if constexpr (CT % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < CT / 2; ++i) {
+            const uint2 v = reinterpret_cast<const uint2*>(p)[i];
+            x[2 * i] = v.x, x[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < CT; ++i) x[i] = p[i];
+    }
+}
+
+// q.x and x.x of one row slice, summed over the quad.
+template <int CT>
+__device__ __forceinline__ void u8_dots(const uint32_t (&q)[CT], const uint32_t (&x)[CT], uint32_t* qx,
+                                        uint32_t* xx) {
+    uint32_t a = 0, b = 0;
+#pragma unroll
+    for (int k = 0; k < CT; ++k) {
+        a = __builtin_amdgcn_udot4(q[k], x[k], a, false);
+        b = __builtin_amdgcn_udot4(x[k], x[k], b, false);
+    }
+    *qx = quad_sum_u(a);
+    *xx = quad_sum_u(b);
+}
+
+// The metric value from the exact sums qn = q.q, qx = q.x, xx = x.x: the same
+// final operations as the FP32 chains apply to the same (exact) values.
+__device__ __forceinline__ float u8_finish(int metric, uint32_t qn, uint32_t qx, uint32_t xx) {
+    if (metric == kL2Sqr || metric == kL2) {
+        const float s = float(qn + xx - 2u * qx);
+        return metric == kL2 ? __builtin_sqrtf(s) : s;
+    }
+    if (metric == kDot) return -float(qx);
+    const float dot = float(qx), sq2 = float(xx), qn2 = float(qn);
+    const float denom = __builtin_sqrtf(qn2 * sq2);
+    return denom > 0.0f ? 1.0f - dot / denom : 1.0f;
+}
+
 // |q|^2 chain of the cosine kernel (same lane order), by a quad.
 __device__ __forceinline__ float quad_norm2(const float* __restrict__ q, int C, uint32_t sub) {
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;

@@ -124,6 +124,62 @@ __global__ void distance_matrix_kernel(int metric, const float* __restrict__ a, 
     if (ok && sub == 0) out[size_t(i) * nb + j] = d;
 }
 
+// 8-bit rows.  When every stored value is an integer in [0, 255] and a row
+// has at most 256 of them, every FP32 operation dist_func_t performs on a
+// query of the same kind is exact: differences, squares and products are
+// integers below 2^16, and every partial sum of at most 256 of them stays
+// below 2^24.  Distances computed from one-byte copies of the values (by any
+// summation order, in integers) then equal the FP32 chains bit for bit, and
+// the router reads a quarter of the bytes (dev::byte_valued: either zero
+// counts, NaN does not).
+
+__global__ void byte_check_kernel(const float* __restrict__ v, size_t count, uint32_t* __restrict__ ok) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+        if (!dev::byte_valued(v[i])) *ok = 0u;
+}
+
+__global__ void byte_pack_kernel(const float* __restrict__ v, size_t count, uint32_t* __restrict__ out) {
+    // four values per output word, little-endian byte order
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count / 4; i += size_t(gridDim.x) * blockDim.x) {
+        const float4 f = reinterpret_cast<const float4*>(v)[i];
+        out[i] = uint32_t(f.x) | (uint32_t(f.y) << 8) | (uint32_t(f.z) << 16) | (uint32_t(f.w) << 24);
+    }
+}
+
+// Builds (enable) or drops the 8-bit copy; returns whether the index has one.
+int set_u8_rows(artea_index* idx, bool enable, int* active) {
+    if (!enable || idx->dim_pad > 256 || idx->n == 0) {
+        if (idx->d_u8) ARTEA_HIP_TRY(hipFree(idx->d_u8));
+        idx->d_u8 = nullptr;
+        *active = 0;
+        return ARTEA_OK;
+    }
+    if (!idx->d_u8) {
+        const size_t count = size_t(idx->n) * idx->dim_pad;
+        uint32_t* d_ok = nullptr;
+        ARTEA_HIP_TRY(hipMalloc(&d_ok, sizeof(uint32_t)));
+        const uint32_t one = 1u;
+        uint32_t ok = 0u;
+        hipError_t e = hipMemcpy(d_ok, &one, sizeof(one), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(byte_check_kernel, dim3(2048), dim3(256), 0, nullptr, idx->d_vecs, count, d_ok);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(&ok, d_ok, sizeof(ok), hipMemcpyDeviceToHost);
+        (void)hipFree(d_ok);
+        ARTEA_HIP_TRY(e);
+        if (ok) {
+            ARTEA_HIP_TRY(hipMalloc(&idx->d_u8, count));
+            hipLaunchKernelGGL(byte_pack_kernel, dim3(2048), dim3(256), 0, nullptr, idx->d_vecs, count,
+                               reinterpret_cast<uint32_t*>(idx->d_u8));
+            ARTEA_HIP_TRY(hipGetLastError());
+            ARTEA_HIP_TRY(hipDeviceSynchronize());
+        }
+    }
+    *active = idx->d_u8 ? 1 : 0;
+    return ARTEA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -146,6 +202,14 @@ int artea_index_create(const float* vecs, uint32_t n, uint32_t dim_pad, int metr
             return ARTEA_EHIP;
         }
     }
+    {
+        int active = 0;
+        rc = set_u8_rows(idx, true, &active);
+        if (rc) {
+            artea_index_destroy(idx);
+            return rc;
+        }
+    }
     *out = idx;
     return ARTEA_OK;
 }
@@ -162,6 +226,14 @@ int artea_index_create_device(const float* d_vecs, uint32_t n, uint32_t dim_pad,
             artea_index_destroy(idx);
             set_error(std::string("artea_index_create_device: copy failed: ") + hipGetErrorString(e));
             return ARTEA_EHIP;
+        }
+    }
+    {
+        int active = 0;
+        rc = set_u8_rows(idx, true, &active);
+        if (rc) {
+            artea_index_destroy(idx);
+            return rc;
         }
     }
     *out = idx;
@@ -268,10 +340,22 @@ int artea_index_info(const artea_index* idx, uint32_t* n, uint32_t* dim_pad, uin
 
 const float* artea_index_device_vectors(const artea_index* idx) { return idx ? idx->d_vecs : nullptr; }
 
+int artea_index_set_u8_rows(artea_index* idx, int enable, int* active) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx, "artea_index_set_u8_rows: null index");
+    int on = 0;
+    const int rc = set_u8_rows(idx, enable != 0, &on);
+    if (active) *active = on;
+    return rc;
+}
+
+int artea_index_u8_rows(const artea_index* idx) { return idx && idx->d_u8 ? 1 : 0; }
+
 int artea_index_destroy(artea_index* idx) {
     if (!idx) return ARTEA_OK;
     free_graph(idx->dg);
     if (idx->d_vecs) (void)hipFree(idx->d_vecs);
+    if (idx->d_u8) (void)hipFree(idx->d_u8);
     if (idx->stream) (void)hipStreamDestroy(idx->stream);
     delete idx;
     return ARTEA_OK;

@@ -92,8 +92,13 @@ int pick_kernels(uint32_t L, uint32_t C, int metric, KernelSet* ks, uint32_t* K)
 
 // Persistent variant used until an autotune picks one: measured at
 // 1,000,000 x 128 L2 on MI355X, the unbounded 32-row kernel is fastest up
-// to 4 blocks and the 3-waves-per-SIMD budget above.
-int default_variant(uint32_t K) { return K <= 4 ? kVarWide : kVarWide3; }
+// to 4 blocks and the 3-waves-per-SIMD budget above; with one-byte rows the
+// unbounded kernel.
+int default_variant(uint32_t K, bool u8) { return u8 ? kNumVariants : (K <= 4 ? kVarWide : kVarWide3); }
+
+// Variant slots of the batched router: 0..3 the FP32-row kernels, 4..7 the
+// one-byte-row kernels (only while the index keeps one-byte rows).
+int num_slots(const artea_index* idx) { return idx->d_u8 ? 2 * kNumVariants : kNumVariants; }
 
 size_t lds_bytes(uint32_t D, uint32_t K) { return kWavesPerBlock * wave_lds_bytes(D, K); }
 
@@ -181,6 +186,7 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     }
     RouterArgs a{};
     a.vecs = idx->d_vecs;
+    a.vecs_u8 = reinterpret_cast<const uint32_t*>(idx->d_u8);
     a.n = idx->n;
     a.D = idx->dim_pad;
     a.C = idx->dim_pad / 16;
@@ -216,8 +222,13 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     uint32_t K;
     int rc = pick_kernels(queue, a.C, idx->metric, &ks, &K);
     if (rc) return rc;
-    const int variant = r->variant >= 0 ? r->variant : default_variant(K);
-    const KernelFn fn = persistent ? ks.persistent[variant] : ks.plain;
+    const bool has_u8 = idx->d_u8 != nullptr;
+    int slot = r->variant >= 0 ? r->variant : default_variant(K, has_u8);
+    if (!has_u8 && slot >= kNumVariants) slot -= kNumVariants;  // the one-byte rows were dropped
+    const bool u8 = slot >= kNumVariants;
+    const int variant = slot % kNumVariants;
+    const KernelFn fn = persistent ? (u8 ? ks.persistent_u8[variant] : ks.persistent[variant])
+                                   : (has_u8 ? ks.plain_u8 : ks.plain);
     const size_t lds = lds_bytes(a.D, K);
     if (lds > 65536) {
         // above the default dynamic-LDS limit (large queues, wide rows)
@@ -386,7 +397,7 @@ int artea_batched_router_query_device(artea_batched_router* r, const float* d_q,
 int artea_batched_router_set_variant(artea_batched_router* r, int variant) {
     artea::clear_error();
     ARTEA_CHECK_ARG(r, "artea_batched_router_set_variant: null router");
-    ARTEA_CHECK_ARG(variant >= -1 && variant < kNumVariants, "artea_batched_router_set_variant: unknown variant");
+    ARTEA_CHECK_ARG(variant >= -1 && variant < num_slots(r->base.idx), "artea_batched_router_set_variant: unknown variant");
     r->base.variant = variant;
     return ARTEA_OK;
 }
@@ -397,7 +408,10 @@ int artea_batched_router_get_variant(artea_batched_router* r, int* variant) {
     uint32_t K = 0;
     int rc = pick_kernels(r->max_queue, r->base.idx->dim_pad / 16, r->base.idx->metric, nullptr, &K);
     if (rc) return rc;
-    *variant = r->base.variant >= 0 ? r->base.variant : default_variant(K);
+    const bool has_u8 = r->base.idx->d_u8 != nullptr;
+    int slot = r->base.variant >= 0 ? r->base.variant : default_variant(K, has_u8);
+    if (!has_u8 && slot >= kNumVariants) slot -= kNumVariants;
+    *variant = slot;
     return ARTEA_OK;
 }
 
@@ -418,7 +432,8 @@ int artea_batched_router_autotune(artea_batched_router* r, const float* d_q, uin
     const int saved = r->base.variant;
     int best = -1, rc = ARTEA_OK;
     float best_ms = 0.f;
-    for (int v = 0; v < kNumVariants && rc == ARTEA_OK; ++v) {
+    const int slots = num_slots(r->base.idx);
+    for (int v = 0; v < slots && rc == ARTEA_OK; ++v) {
         r->base.variant = v;
         // One untimed launch (occupancy query, caches), then one timed launch.
         rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,

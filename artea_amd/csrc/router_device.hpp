@@ -154,6 +154,97 @@ __device__ __forceinline__ void distances(const RouterArgs& a, const WaveLds& w,
     wave_sync();
 }
 
+// One-byte state of a query (device.hpp, "one-byte rows"): this lane's quad
+// slice of the query as bytes and q.q, used when the index keeps one-byte
+// rows and every value of the query is a whole number in [0, 255].
+template <int CT>
+struct U8Query {
+    bool on;
+    uint32_t qn;
+    uint32_t qw[CT > 0 ? CT : 1];
+};
+
+template <int CT, bool U8>
+__device__ __forceinline__ U8Query<CT> u8_query(const RouterArgs& a, const WaveLds& w) {
+    U8Query<CT> u;
+    u.on = false;
+    u.qn = 0;
+    if constexpr (CT > 0 && U8) {
+        if (a.vecs_u8) {
+            const uint32_t lane = lane_id(), sub = lane & 3u;
+            bool bad = false;
+            for (uint32_t i = lane; i < a.D; i += 64) bad |= !byte_valued(w.q[i]);
+            u.on = ballot(bad) == 0;
+            if (u.on) {
+                uint32_t s = 0;
+#pragma unroll
+                for (int k = 0; k < CT; ++k) {
+                    const float4 f = *reinterpret_cast<const float4*>(w.q + sub * 4 * CT + 4 * k);
+                    u.qw[k] = uint32_t(f.x) | (uint32_t(f.y) << 8) | (uint32_t(f.z) << 16) | (uint32_t(f.w) << 24);
+                    s = __builtin_amdgcn_udot4(u.qw[k], u.qw[k], s, false);
+                }
+                u.qn = quad_sum_u(s);
+            }
+        }
+    }
+    return u;
+}
+
+// Distances of cand[0..cnt) into cdist[0..cnt) from the one-byte rows: a
+// quad per row, 16 rows per group, R groups' loads issued before any sum.
+// Equal to distances() bit for bit on such queries (exact arithmetic).
+template <int CT, int R>
+__device__ __forceinline__ void distances_u8(const RouterArgs& a, const WaveLds& w, uint32_t cnt, const U8Query<CT>& u,
+                                             int metric) {
+    const uint32_t lane = lane_id(), sub = lane & 3u;
+    for (uint32_t g = 0; g < cnt; g += 16 * R) {
+        uint32_t x[R][CT];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (g + 16u * r >= cnt) break;
+            const uint32_t row = min(g + (lane >> 2) + 16u * r, cnt - 1);  // past the end: re-read the last row
+            load_u8_slice<CT>(a.vecs_u8 + size_t(w.cand[row]) * (4 * CT) + sub * CT, x[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (g + 16u * r >= cnt) break;
+            uint32_t qx, xx;
+            u8_dots<CT>(u.qw, x[r], &qx, &xx);
+            const uint32_t row = g + (lane >> 2) + 16u * r;
+            if (sub == 0 && row < cnt) w.cdist[row] = u8_finish(metric, u.qn, qx, xx);
+        }
+    }
+    wave_sync();
+}
+
+// The query's distances to cand[0..cnt): one-byte rows when it can use them.
+template <int CT, bool WIDE, int MF, int R>
+__device__ __forceinline__ void any_distances(const RouterArgs& a, const WaveLds& w, uint32_t cnt, const U8Query<CT>& u,
+                                              int metric, float qn2) {
+    if constexpr (CT > 0) {
+        if (u.on) {
+            distances_u8<CT, R>(a, w, cnt, u, metric);
+            return;
+        }
+    }
+    distances<CT, WIDE, MF>(a, w, cnt, qn2);
+}
+
+// Distance of one row (every lane of quad 0 returns it).
+template <int CT>
+__device__ __forceinline__ float row_distance(const RouterArgs& a, const WaveLds& w, const U8Query<CT>& u, uint32_t v,
+                                              int metric, float qn2) {
+    if constexpr (CT > 0) {
+        if (u.on) {
+            uint32_t x[CT], qx, xx;
+            load_u8_slice<CT>(a.vecs_u8 + size_t(v) * (4 * CT) + (lane_id() & 3u) * CT, x);
+            u8_dots<CT>(u.qw, x, &qx, &xx);
+            return u8_finish(metric, u.qn, qx, xx);
+        }
+    }
+    return quad_distance<CT>(w.q, a.vecs + size_t(v) * a.D, int(a.C), metric, lane_id() & 3u, qn2);
+}
+
 __device__ __forceinline__ const uint32_t* upper_row(const RouterArgs& a, uint32_t v, uint32_t level) {
     const uint2 vi = a.vinfo[v];
     if (vi.x == 0xFFFFFFFFu || level > vi.x || level == 0) return nullptr;
@@ -469,7 +560,9 @@ struct QueryStats {
     uint32_t ndc, expansions, overflow;
 };
 
-template <int K, int CT, int MF, bool WIDE>
+// U8R > 0: the kernel also carries the one-byte path (used by queries that
+// can), with U8R groups of 16 rows per distance pass in the beam.
+template <int K, int CT, int MF, bool WIDE, int U8R>
 __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveLds& w, uint32_t qi) {
     const uint32_t lane = lane_id();
     const int metric = family_metric<MF>(a.metric);
@@ -486,11 +579,11 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
     wave_sync();
     const float qn2 = MF == 2 ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
+    const U8Query<CT> u = u8_query<CT, (U8R > 0)>(a, w);
 
     // Entry point distance (all lanes compute it; quad 0's value is used).
     uint32_t cur = a.entry;
-    float cd = quad_distance<CT>(w.q, a.vecs + size_t(cur) * a.D, int(a.C), metric, lane & 3u, qn2);
-    cd = readlane_f(cd, 0);
+    float cd = readlane_f(row_distance<CT>(a, w, u, cur, metric, qn2), 0);
     uint32_t ndc = 1, expansions = 0;
 
     if (!(a.flags & ARTEA_L0_ONLY)) {
@@ -515,7 +608,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                     if (ballot(id != 0xFFFFFFFFu) == 0) break;
                     const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_mode);
                     if (cnt == 0) continue;
-                    distances<CT, false, MF>(a, w, cnt, qn2);
+                    any_distances<CT, false, MF, 1>(a, w, cnt, u, metric, qn2);
                     ndc += cnt;
                     // First strict minimum in row order below the running best.
                     float dd = lane < cnt ? w.cdist[lane] : __int_as_float(0x7F800000);
@@ -585,7 +678,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                                                                      : 0xFFFFFFFFu;
             }
             if (cnt == 0) continue;
-            distances<CT, WIDE, MF>(a, w, cnt, qn2);
+            any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
             ndc += cnt;
             const float myd = lane < cnt ? w.cdist[lane] : 0.f;
             const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
@@ -647,12 +740,12 @@ __device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long lon
 }
 
 // Plain router: one wavefront per query, the grid covers the batch.
-template <int K, int CT, int MF>
+template <int K, int CT, int MF, int U8R>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
     const WaveLds w = wave_lds<K>(a);
     const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
     if (qi >= a.nq) return;
-    const QueryStats st = route_one<K, CT, MF, true>(a, w, qi);
+    const QueryStats st = route_one<K, CT, MF, (U8R == 0), U8R>(a, w, qi);
     add_stats(a, st.ndc, st.expansions, st.overflow);
 }
 
@@ -661,8 +754,9 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
 // queries of uneven work (mixed topk / queue, hop counts) balance across
 // waves and no wave idles at a launch tail.  WAVES = the minimum resident
 // waves per SIMD the register allocation must allow (1 = no constraint);
-// WIDE = 32 rows per distance pass instead of 16.
-template <int K, int CT, int MF, int WAVES, bool WIDE>
+// WIDE = 32 rows per FP32 distance pass instead of 16; U8R > 0 = with the
+// one-byte path, 16 * U8R rows per pass.
+template <int K, int CT, int MF, int WAVES, bool WIDE, int U8R>
 __global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_kernel(RouterArgs a) {
     const WaveLds w = wave_lds<K>(a);
     while (true) {
@@ -670,7 +764,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_
         if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);
         qi = readfirst_u(qi);
         if (qi >= a.nq) break;
-        const QueryStats st = route_one<K, CT, MF, WIDE>(a, w, qi);
+        const QueryStats st = route_one<K, CT, MF, WIDE, U8R>(a, w, qi);
         add_stats(a, st.ndc, st.expansions, st.overflow);
     }
 }

@@ -22,6 +22,7 @@ __host__ __device__ inline size_t wave_lds_bytes(uint32_t D, uint32_t K) {
 
 struct RouterArgs {
     const float* vecs;
+    const uint32_t* vecs_u8;                      // one-byte rows (4*C words each), or null
     uint32_t n, D, C;
     int metric;
     const uint32_t* ell0;
@@ -61,6 +62,8 @@ enum PersistentVariant : int {
 struct KernelSet {
     KernelFn plain;
     KernelFn persistent[kNumVariants];
+    KernelFn plain_u8;  // with the one-byte path (indexes that keep one-byte rows)
+    KernelFn persistent_u8[kNumVariants];
 };
 
 // Metric family 0 = L2SQR / L2, 1 = DOT, 2 = COSINE; K = 64-entry register

@@ -111,6 +111,19 @@ int artea_index_info(const artea_index* idx, uint32_t* n, uint32_t* dim_pad, uin
                      uint32_t* entry_vid, uint32_t* ul_max, uint32_t* bl_max);
 /* Device pointer of the base vectors (n x dim_pad floats). */
 const float* artea_index_device_vectors(const artea_index* idx);
+/* 8-bit rows (no counterpart in the reference; a storage choice of this
+ * library).  When every base value is a whole number in [0, 255] and
+ * dim_pad <= 256, the index also keeps a one-byte copy of each row, built
+ * when the index is created.  The router then computes the distances of
+ * every query whose values are of the same kind from the one-byte rows, in
+ * exact integer arithmetic: for such values every FP32 operation of
+ * dist_func_t is exact, so the distances, and hence the results, equal the
+ * FP32 chains' bit for bit, from a quarter of the bytes.  Other queries of
+ * the same batch use the FP32 rows.  enable = 0 drops the copy, enable = 1
+ * builds it again when the values allow; *active (optional) tells whether
+ * the index has it.  artea_index_u8_rows returns 1 when it has, else 0. */
+int artea_index_set_u8_rows(artea_index* idx, int enable, int* active);
+int artea_index_u8_rows(const artea_index* idx);
 int artea_index_destroy(artea_index* idx);
 
 /* ----------------------------------------------------------------- router */
@@ -159,22 +172,28 @@ int artea_batched_router_query_device(artea_batched_router* r, const float* d_qu
 int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint64_t* expansions,
                                     uint64_t* tail_overflows);
 
-/* Kernel variants of the batched router (no reference counterpart): the
- * distance gather width (16 or 32 candidate rows per pass) and the register
- * budget the kernel is compiled for (no bound, 3 or 4 resident waves per
- * SIMD).  Every variant returns the same results; only the speed differs. */
+/* Kernel variants of the batched router (no reference counterpart).  Slots
+ * 0..3 read the FP32 rows: the distance gather width (16 or 32 candidate
+ * rows per pass) and the register budget the kernel is compiled for (no
+ * bound, 3 or 4 resident waves per SIMD).  While the index keeps one-byte
+ * rows (artea_index_set_u8_rows) slots 4..7 exist too: the integer distance
+ * path with 64, 32 or 16 rows per pass, and 64 under a 4-wave budget.
+ * Every variant returns the same results; only the speed differs. */
 #define ARTEA_ROUTER_VARIANTS 4
+#define ARTEA_ROUTER_VARIANTS_U8 8
 /* Autotuner: times each variant on this batch (one untimed and one timed
  * launch each, on the router's stream, synchronously) and keeps the fastest
- * for the router's later queries.  ms (optional) receives
- * ARTEA_ROUTER_VARIANTS launch times in milliseconds, *chosen (optional)
- * the variant kept.  The output buffers hold the batch's results after. */
+ * for the router's later queries.  ms (optional) receives one launch time in
+ * milliseconds per slot (ARTEA_ROUTER_VARIANTS_U8 of them when the index
+ * keeps one-byte rows, else ARTEA_ROUTER_VARIANTS), *chosen (optional) the
+ * slot kept.  The output buffers hold the batch's results after. */
 int artea_batched_router_autotune(artea_batched_router* r, const float* d_queries, uint32_t nq,
                                   const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
                                   const float* thresholds, uint32_t* d_out_vids, float* d_out_dists, float* ms,
                                   int* chosen);
-/* Pins a variant (0 .. ARTEA_ROUTER_VARIANTS-1), or -1 for the built-in
- * default for the router's queue size; get returns the variant in use. */
+/* Pins a variant slot (0 .. 3, or 0 .. 7 with one-byte rows), or -1 for the
+ * built-in default for the router's queue size; get returns the slot in use
+ * (a one-byte slot reads as its FP32 counterpart once the rows are dropped). */
 int artea_batched_router_set_variant(artea_batched_router* r, int variant);
 int artea_batched_router_get_variant(artea_batched_router* r, int* variant);
 

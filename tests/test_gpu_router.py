@@ -303,7 +303,7 @@ def test_batched_router_variants_and_autotune_agree(orc, metric, kind, dim):
         od = DeviceBuffer.empty((nq, 10), np.float32)
         pv, pd = aa.HierarchicalGraphRouter(idx, None, 10, L).batch_query(qs)
         br = aa.BatchedRouter(idx, 10, L)
-        for v in range(len(aa.BatchedRouter.VARIANTS)):
+        for v in range(len(br.variant_names)):
             br.set_variant(v)
             assert br.variant() == v
             br.query_device(tq.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr, stream=s.h)
