@@ -10,7 +10,11 @@
 namespace artea {
 namespace rk {
 
-constexpr int kWavesPerBlock = 4;
+// Waves (queries) per block.  Two rather than four: a block's LDS is then
+// small enough that 14 waves fit a CU's 160 KiB at D = 128 with a 4-block
+// candidate list (12 with four-wave blocks), which the one-byte kernels'
+// register use (under 128 VGPRs) can take.
+constexpr int kWavesPerBlock = 2;
 constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit codes
 
 // LDS of one wavefront: query row, visited cache, 64-entry candidate

@@ -68,8 +68,8 @@ def sweep_queues(aa, idx, q_d, nq, gt_i, topk, queues, stream, target):
         r = recall(pred, gt_i, topk)
         st = br.last_stats()
         rows.append({"queue": L, "recall@%d" % topk: round(r, 4), "qps": round(nq / t), "ms": round(t * 1e3, 3),
-                     "ndc_per_query": round(st["ndc"] / nq, 1), "variant": aa.BatchedRouter.VARIANTS[variant]})
-        log("queue %d: recall@%d %.4f, %.0f q/s (%s)" % (L, topk, r, nq / t, aa.BatchedRouter.VARIANTS[variant]))
+                     "ndc_per_query": round(st["ndc"] / nq, 1), "variant": br.variant_names[variant]})
+        log("queue %d: recall@%d %.4f, %.0f q/s (%s)" % (L, topk, r, nq / t, br.variant_names[variant]))
     ok = [x for x in rows if x["recall@%d" % topk] >= target]
     return rows, (ok[0] if ok else rows[-1])
 
@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--f32-rows", action="store_true",
+                    help="search the FP32 rows only (drop the exact one-byte copy of whole-number data)")
     ap.add_argument("--serve-requests", type=int, default=40000,
                     help="requests per offered-load level of the serving leg (0 = skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -256,6 +258,9 @@ def main():
     log("device %s, %d CUs" % (dev["arch"], dev["compute_units"]))
 
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    if args.f32_rows:
+        idx.set_u8_rows(False)
+    log("one-byte rows: %s" % idx.u8_rows)
     t0 = time.time()
     if args.graph_cache and os.path.exists(args.graph_cache):
         idx.load_hgra(args.graph_cache)
@@ -347,14 +352,14 @@ def main():
         sweep.append({"queue": L, "recall@100": round(r100, 4), "recall@10": round(r10, 4),
                       "batched_qps": round(nq / tb), "plain_qps": round(nq / tp), "ndc_per_query":
                       round(st["ndc"] / nq, 1), "batched_equals_plain": same, "tail_overflows": st["tail_overflows"],
-                      "variant": aa.BatchedRouter.VARIANTS[variant],
-                      "variant_ms": {aa.BatchedRouter.VARIANTS[i]: round(float(x), 3) for i, x in enumerate(var_ms)}})
+                      "variant": br.variant_names[variant],
+                      "variant_ms": {br.variant_names[i]: round(float(x), 3) for i, x in enumerate(var_ms)}})
         log("queue %d: recall@100 %.4f recall@10 %.4f batched %.0f q/s plain %.0f q/s ndc %.0f same %s variant %s" % (
-            L, r100, r10, nq / tb, nq / tp, st["ndc"] / nq, same, aa.BatchedRouter.VARIANTS[variant]))
-        routers[L] = (br, p10, p100, run_batched, run_plain, st, graph)
+            L, r100, r10, nq / tb, nq / tp, st["ndc"] / nq, same, br.variant_names[variant]))
+        routers[L] = (br, p10, p100, run_batched, run_plain, st, graph, var_ms, ql_d)
     ok = [s for s in sweep if s["recall@100"] >= args.target_recall]
     chosen = ok[0]["queue"] if ok else sweep[-1]["queue"]
-    br, p10, p100, run_batched, run_plain, st, graph = routers[chosen]
+    br, p10, p100, run_batched, run_plain, st, graph, var_ms, ql_d = routers[chosen]
 
     for _ in range(args.warmup):
         run_batched()
@@ -368,6 +373,29 @@ def main():
     log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
     pred = ov.numpy(stream)
     pdist = od.numpy(stream)
+    slot = br.variant()
+    u8 = idx.u8_rows and slot >= len(aa.BatchedRouter.VARIANTS)
+    f32_rows = None
+    if u8:
+        # The same batch on the FP32 rows (the fastest FP32 slot of the
+        # autotune): equal results, its throughput reported beside.
+        f32_slot = int(np.argmin(np.asarray(var_ms[: len(aa.BatchedRouter.VARIANTS)])))
+        br.set_variant(f32_slot)
+        gf = br.capture(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr)
+
+        def run_f32(gf=gf):
+            gf.launch(sh)
+
+        for _ in range(args.warmup):
+            run_f32()
+        t_f32 = time_it(run_f32, args.steps)
+        same_f32 = bool(np.array_equal(ov.numpy(stream), pred) and
+                        np.array_equal(od.numpy(stream).view(np.uint32), pdist.view(np.uint32)))
+        br.set_variant(slot)
+        f32_rows = {"qps": round(nq / t_f32, 1), "ms_per_step": round(t_f32 * 1e3, 4),
+                    "variant": br.variant_names[f32_slot], "results_equal": same_f32}
+        log("FP32 rows: %.0f q/s (%.3f ms/step, %s), results equal %s" % (nq / t_f32, t_f32 * 1e3,
+                                                                           br.variant_names[f32_slot], same_f32))
     # FP32 router distances against FP64 recomputation (same ids), on the
     # top-100 queries: the integer-valued data makes every FP32 partial sum
     # exact, so the expected error is zero.
@@ -387,17 +415,24 @@ def main():
         log("calibration copy of %d bytes done" % base.nbytes)
 
     # Algorithmic bytes of one launch: per query the query row and the result
-    # rows, per distance evaluation one base row (4*D bytes), per beam
-    # expansion one neighbour row (4 bytes per stored neighbour id).
+    # rows, per distance evaluation one base row (4*D bytes, or D with
+    # one-byte rows), per beam expansion one neighbour row (4 bytes per
+    # stored neighbour id).
     row_bytes = 4 * 64
-    alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
+    vec_bytes = D if u8 else 4 * D
+    alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * vec_bytes + st["expansions"] * row_bytes
     achieved = alg_bytes / t_step / 1e9
+    if f32_rows is not None:
+        fb = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
+        f32_rows["roofline_frac"] = round(fb / (f32_rows["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (calibrated on a copy kernel), committed under profiles/.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "router_c1_traffic.json")
     if os.path.exists(tpath) and args.n == 1_000_000 and nq == 10_000 and chosen == 200:
-        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
+        tj = json.load(open(tpath))
+        if tj.get("rows", "f32") == ("u8" if u8 else "f32"):
+            traffic = tj["traffic_bytes_per_launch"]
     log("roofline: %.1f GB algorithmic per launch, %.1f GB/s (%.3f of peak)" % (alg_bytes / 1e9, achieved,
                                                                           achieved / HBM_PEAK_GBS))
 
@@ -464,7 +499,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "u8" if u8 else "f32",
         "data": "synthetic (seeded 256-component integer Gaussian mixture; index built on the GPU)",
         "config": {"workload": "batched search, 1,000,000 x 128 base, 10,000 queries, L2, topk mixed {10,100}",
                    "n": args.n, "dim": D, "nq": nq, "queue": chosen, "parallelism": "single GPU"},
@@ -475,7 +510,10 @@ def main():
         "sweep": sweep,
         "ground_truth": {"fp64_kernel_s": round(gt_fp64_s, 3), "bf16_mfma_s": round(gt_mfma_s, 3),
                          "identical": gt_same, "kept_pairs_per_query": round(gst.get("kept_pairs", 0) / nq, 1)},
-        "router_variant": aa.BatchedRouter.VARIANTS[br.variant()],
+        "router_variant": br.variant_names[br.variant()],
+        "rows": ("one-byte rows (every base and query value a whole number in [0, 255]; integer distances equal "
+                 "the FP32 chains bit for bit)" if u8 else "FP32 rows"),
+        "f32_rows": f32_rows,
         "launch": "HIP graph replay (counter-reset kernel + persistent router kernel)",
         "distance_max_rel_err_vs_fp64": dist_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
