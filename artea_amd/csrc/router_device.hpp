@@ -51,45 +51,47 @@ constexpr uint32_t kVidMask = 0x7FFFFFFFu;
 
 
 
-// Visited cache: two-choice hashing over 8 KiB of codes.  h1 = v * A1 and
+// Visited cache: two-choice hashing over 7.5 KiB of codes.  h1 = v * A1 and
 // h2 = v * A2 mod 2^B (odd multipliers) are bijections of [0, 2^B), so a
-// slot (the low bits of h) and a quotient (the remaining bits) identify v
-// exactly.  Choice 1 stores quotient + 1 in its slot, choice 2 stores
-// quotient + 1 + (the number of choice-1 codes), so a stored code also says
-// which choice wrote it; 0 marks an empty slot.  Modes: 0 (n <= 2^20) 8192
-// slots of 8-bit codes, choice 1 = 1..128, choice 2 = 129..255 (a choice-2
-// quotient of 127 has no code: that vertex uses choice 1 only); 1 (n <= 2^27)
-// 4096 slots of 16-bit codes, 1..32768 and 32769..65535; 2 (n <= 2^28) 4096
-// 16-bit slots, choice 1 only (a code of 65536 is stored as the empty 0 and
-// never matches: that vertex is never cached).  A lookup hits only on v's own
-// code, so the cache never reports a vertex that was not inserted.  An
-// insertion takes choice 1 when empty, else choice 2 when empty, else
-// overwrites choice 1.
+// slot (h mod S) and a quotient (h div S) identify v exactly.  Choice 1
+// stores quotient + 1 in its slot, choice 2 stores quotient + 1 + (the
+// number of choice-1 codes), so a stored code also says which choice wrote
+// it; 0 marks an empty slot.  Modes: 0 (n <= 2^20) S = 7680 slots of 8-bit
+// codes, choice 1 = 1..137, choice 2 = 138..255 (vertices whose choice-2
+// quotient exceeds 117 use choice 1 only); 1 (n <= 2^27) S = 3840 slots of
+// 16-bit codes, 1..34953 and 34954..65535; 2 (n <= 2^28) as 1 with choice 1
+// only, and a vertex whose code exceeds 16 bits is never stored.  A lookup
+// hits only on v's own code, so the cache never reports a vertex that was
+// not inserted.  An insertion takes choice 1 when empty, else choice 2 when
+// empty, else overwrites choice 1.
 struct VisKey {
     uint32_t s1, t1, s2, t2;  // t2 == 0: no second choice
 };
 __device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t mode) {
     const uint32_t h1 = (v * 0x9E3779B1u) & hmask, h2 = (v * 0x85EBCA77u) & hmask;
     if (mode == 0) {
-        const uint32_t q2 = h2 >> 13;
-        return VisKey{h1 & 8191u, (h1 >> 13) + 1, h2 & 8191u, q2 < 127 ? q2 + 129 : 0u};
+        const uint32_t q1 = h1 / 7680u, q2 = h2 / 7680u;
+        return VisKey{h1 - q1 * 7680u, q1 + 1, h2 - q2 * 7680u, q2 <= 117u ? q2 + 138u : 0u};
     }
-    const uint32_t q2 = h2 >> 12;
-    return VisKey{h1 & 4095u, (h1 >> 12) + 1, h2 & 4095u, (mode == 1 && q2 < 32767) ? q2 + 32769 : 0u};
+    const uint32_t q1 = h1 / 3840u, q2 = h2 / 3840u;
+    return VisKey{h1 - q1 * 3840u, q1 + 1, h2 - q2 * 3840u, (mode == 1 && q2 <= 30581u) ? q2 + 34954u : 0u};
 }
 __device__ __forceinline__ uint32_t vis_load(const unsigned char* vis, uint32_t slot, uint32_t mode) {
     return mode ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[slot]) : uint32_t(vis[slot]);
 }
 __device__ __forceinline__ void vis_store(unsigned char* vis, uint32_t slot, uint32_t code, uint32_t mode) {
-    if (mode) reinterpret_cast<uint16_t*>(vis)[slot] = uint16_t(code);
-    else vis[slot] = uint8_t(code);
+    if (mode) {
+        if (code <= 0xFFFFu) reinterpret_cast<uint16_t*>(vis)[slot] = uint16_t(code);
+    } else {
+        vis[slot] = uint8_t(code);
+    }
 }
 
 struct WaveLds {
     float* q;         // D floats
     unsigned char* vis;  // kVisBytes
-    uint32_t* cand;   // 64
-    float* cdist;     // 64
+    uint32_t* cand;   // 64 (= wv[0..64): staging between collect and a merge)
+    float* cdist;     // 64 (= wd[0..64))
     float* wd;        // 64 K: candidate-list distances by position (merge_chunk)
     uint32_t* wv;     // 64 K: candidate-list vids by position
 };
@@ -721,12 +723,12 @@ __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     const size_t per_wave = wave_lds_bytes(a.D, K);
     unsigned char* base = smem + wid * per_wave;
     WaveLds w;
-    w.q = reinterpret_cast<float*>(base);
-    w.vis = base + size_t(a.D) * 4;
-    w.cand = reinterpret_cast<uint32_t*>(w.vis + kVisBytes);
-    w.cdist = reinterpret_cast<float*>(w.cand + 64);
-    w.wd = w.cdist + 64;
+    w.wd = reinterpret_cast<float*>(base);
     w.wv = reinterpret_cast<uint32_t*>(w.wd + 64 * K);
+    w.cdist = w.wd;
+    w.cand = w.wv;
+    w.vis = reinterpret_cast<unsigned char*>(w.wv + 64 * K);
+    w.q = reinterpret_cast<float*>(w.vis + kVisBytes);
     return w;
 }
 

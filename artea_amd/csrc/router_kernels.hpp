@@ -15,13 +15,15 @@ namespace rk {
 // candidate list (12 with four-wave blocks), which the one-byte kernels'
 // register use (under 128 VGPRs) can take.
 constexpr int kWavesPerBlock = 2;
-constexpr uint32_t kVisBytes = 8192;  // lossy visited cache per wave: 8192 x 8-bit or 4096 x 16-bit codes
+constexpr uint32_t kVisBytes = 7680;  // lossy visited cache per wave: 7680 x 8-bit or 3840 x 16-bit codes
 
-// LDS of one wavefront: query row, visited cache, 64-entry candidate
-// staging (ids and distances), and the (distance, vid) mirror of a K-block
-// candidate list.
+// LDS of one wavefront: the (distance, vid) mirror of a K-block candidate
+// list (the first 64 entries of each half double as the expansion chunk's
+// candidate staging, whose use never overlaps a merge), the visited cache
+// and the query row.  At D = 128 and K = 4 that is 10 KiB, so 16 waves fit
+// a CU's 160 KiB.
 __host__ __device__ inline size_t wave_lds_bytes(uint32_t D, uint32_t K) {
-    return size_t(D) * 4 + kVisBytes + 512 + size_t(K) * 512;
+    return size_t(K) * 512 + kVisBytes + (size_t(D) * 4 + 15) / 16 * 16;
 }
 
 struct RouterArgs {
@@ -47,7 +49,7 @@ struct RouterArgs {
     float* out_dists;
     unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
     uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
-    uint32_t vis_mode;                            // visited cache: 0 8192 x 8-bit (n <= 2^20), 1 4096 x 16-bit (n <= 2^27), 2 as 1 with one choice
+    uint32_t vis_mode;                            // visited cache: 0 7680 x 8-bit (n <= 2^20), 1 3840 x 16-bit (n <= 2^27), 2 as 1 with one choice
     uint32_t* work_counter;                       // persistent mode when non-null
 };
 
