@@ -116,12 +116,24 @@ __device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint3
 }
 
 // Distances of cand[0..cnt) into cdist[0..cnt).
-// The metric a kernel of metric family MF evaluates: 0 = L2SQR / L2 (sqrt
-// chosen at run time), 1 = DOT, 2 = COSINE.  Fixing the family at compile
-// time keeps the other families' register demand out of the kernel.
+// The chain a kernel of metric family MF evaluates: 0 = L2SQR (and L2, whose
+// square root is taken in a pass of its own, l2_pass: selected per row inside
+// the chains, the correctly rounded sqrt would be computed for every L2SQR
+// row too), 1 = DOT, 2 = COSINE.  Fixing the family at compile time keeps the
+// other families' register demand out of the kernel.
 template <int MF>
-__device__ __forceinline__ int family_metric(int metric) {
-    return MF == 0 ? (metric == kL2 ? int(kL2) : int(kL2Sqr)) : (MF == 1 ? int(kDot) : int(kCosine));
+__device__ __forceinline__ int family_metric(int) {
+    return MF == 0 ? int(kL2Sqr) : (MF == 1 ? int(kDot) : int(kCosine));
+}
+
+// L2 = sqrt of the L2SQR chain, bit for bit as dist_func_t computes it.
+template <int MF>
+__device__ __forceinline__ void l2_pass(const RouterArgs& a, const WaveLds& w, uint32_t cnt) {
+    if (MF == 0 && a.metric == kL2) {
+        const uint32_t lane = lane_id();
+        if (lane < cnt) w.cdist[lane] = __builtin_sqrtf(w.cdist[lane]);
+        wave_sync();
+    }
 }
 
 template <int CT, bool WIDE, int MF>
@@ -223,13 +235,15 @@ __device__ __forceinline__ void distances_u8(const RouterArgs& a, const WaveLds&
 template <int CT, bool WIDE, int MF, int R>
 __device__ __forceinline__ void any_distances(const RouterArgs& a, const WaveLds& w, uint32_t cnt, const U8Query<CT>& u,
                                               int metric, float qn2) {
+    bool done = false;
     if constexpr (CT > 0) {
         if (u.on) {
             distances_u8<CT, R>(a, w, cnt, u, metric);
-            return;
+            done = true;
         }
     }
-    distances<CT, WIDE, MF>(a, w, cnt, qn2);
+    if (!done) distances<CT, WIDE, MF>(a, w, cnt, qn2);
+    l2_pass<MF>(a, w, cnt);
 }
 
 // Distance of one row (every lane of quad 0 returns it).
@@ -586,6 +600,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // Entry point distance (all lanes compute it; quad 0's value is used).
     uint32_t cur = a.entry;
     float cd = readlane_f(row_distance<CT>(a, w, u, cur, metric, qn2), 0);
+    if (MF == 0 && a.metric == kL2) cd = __builtin_sqrtf(cd);
     uint32_t ndc = 1, expansions = 0;
 
     if (!(a.flags & ARTEA_L0_ONLY)) {

@@ -55,8 +55,9 @@ class PinnedBuffer:
 
 @dataclass
 class _Launch:
-    ids: np.ndarray
-    event: Event
+    ids: np.ndarray   # request ids of one queue-size class segment
+    rows: int         # first row of the segment in the lane's buffers
+    event: Event      # recorded after the segment's result copies
 
 
 class _Lane:
@@ -76,7 +77,7 @@ class _Lane:
         self.dv = DeviceBuffer.empty((budget, max_topk), np.uint32)
         self.dd = DeviceBuffer.empty((budget, max_topk), np.float32)
         self.max_topk = max_topk
-        self.inflight: Optional[_Launch] = None
+        self.inflight: List[_Launch] = []
 
     def cls(self, q: int) -> int:
         # smallest router class that holds the queue and the row stride
@@ -95,6 +96,9 @@ class _Lane:
         check(lib().artea_memcpy_htod(C.c_void_p(self.dk.ptr), C.c_void_p(self.hk.ptr), m * 4, C.c_void_p(s)))
         check(lib().artea_memcpy_htod(C.c_void_p(self.dl.ptr), C.c_void_p(self.hl.ptr), m * 4, C.c_void_p(s)))
         classes = np.array([self.cls(q) for q in self.hl.array[:m]])
+        # Smallest classes first, each with its own result copies and event,
+        # so a request completes when its class does, not when the lane's
+        # slowest class does.
         start = 0
         while start < m:
             c = classes[start]
@@ -102,20 +106,26 @@ class _Lane:
             self.routers[c].query_device(self.dq.ptr + start * d * 4, end - start, self.dk.ptr + start * 4,
                                          self.dl.ptr + start * 4, self.dv.ptr + start * self.max_topk * 4,
                                          self.dd.ptr + start * self.max_topk * 4, stream=s)
+            off, nb = start * self.max_topk * 4, (end - start) * self.max_topk * 4
+            check(lib().artea_memcpy_dtoh(C.c_void_p(self.hv.ptr + off), C.c_void_p(self.dv.ptr + off), nb,
+                                          C.c_void_p(s)))
+            check(lib().artea_memcpy_dtoh(C.c_void_p(self.hd.ptr + off), C.c_void_p(self.dd.ptr + off), nb,
+                                          C.c_void_p(s)))
+            ev = Event()
+            ev.record(self.stream)
+            self.inflight.append(_Launch(ids[start:end], start, ev))
             start = end
-        nb = m * self.max_topk * 4
-        check(lib().artea_memcpy_dtoh(C.c_void_p(self.hv.ptr), C.c_void_p(self.dv.ptr), nb, C.c_void_p(s)))
-        check(lib().artea_memcpy_dtoh(C.c_void_p(self.hd.ptr), C.c_void_p(self.dd.ptr), nb, C.c_void_p(s)))
-        ev = Event()
-        ev.record(self.stream)
-        self.inflight = _Launch(ids, ev)
 
-    def poll(self) -> bool:
-        if self.inflight is None:
-            return False
-        done = C.c_int()
-        check(lib().artea_event_query(C.c_void_p(self.inflight.event.h), C.byref(done)))
-        return bool(done.value)
+    def poll(self) -> List[_Launch]:
+        """Segments completed since the last poll, in launch order."""
+        out = []
+        while self.inflight:
+            done = C.c_int()
+            check(lib().artea_event_query(C.c_void_p(self.inflight[0].event.h), C.byref(done)))
+            if not done.value:
+                break
+            out.append(self.inflight.pop(0))
+        return out
 
 
 class Server:
@@ -147,23 +157,22 @@ class Server:
             now = time.perf_counter() - t0
             progressed = False
             for ln in self.lanes:
-                if ln.inflight is not None and ln.poll():
+                for seg in ln.poll():
                     t = time.perf_counter() - t0
-                    ids = ln.inflight.ids
+                    ids, r0 = seg.ids, seg.rows
                     m = len(ids)
-                    out_v[ids] = ln.hv.array[:m]
-                    out_d[ids] = ln.hd.array[:m]
+                    out_v[ids] = ln.hv.array[r0:r0 + m]
+                    out_d[ids] = ln.hd.array[r0:r0 + m]
                     lat[ids] = t - arrivals[ids]
                     done += m
-                    ln.inflight = None
                     progressed = True
-                if ln.inflight is None and nxt < n and arrivals[nxt] <= now:
+                if not ln.inflight and nxt < n and arrivals[nxt] <= now:
                     end = min(n, int(np.searchsorted(arrivals, now, side="right")), nxt + ln.budget)
                     ln.launch(np.arange(nxt, end), queries, topk, queue)
                     nxt = end
                     progressed = True
             if not progressed:
-                if nxt < n and all(ln.inflight is None for ln in self.lanes):
+                if nxt < n and all(not ln.inflight for ln in self.lanes):
                     wait = arrivals[nxt] - (time.perf_counter() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 1e-3))
