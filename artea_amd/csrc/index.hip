@@ -169,11 +169,17 @@ int set_u8_rows(artea_index* idx, bool enable, int* active) {
         (void)hipFree(d_ok);
         ARTEA_HIP_TRY(e);
         if (ok) {
-            ARTEA_HIP_TRY(hipMalloc(&idx->d_u8, count));
+            uint8_t* d = nullptr;
+            ARTEA_HIP_TRY(hipMalloc(&d, count));
             hipLaunchKernelGGL(byte_pack_kernel, dim3(2048), dim3(256), 0, nullptr, idx->d_vecs, count,
-                               reinterpret_cast<uint32_t*>(idx->d_u8));
-            ARTEA_HIP_TRY(hipGetLastError());
-            ARTEA_HIP_TRY(hipDeviceSynchronize());
+                               reinterpret_cast<uint32_t*>(d));
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+            if (e != hipSuccess) {
+                (void)hipFree(d);  // never attach a copy that may not hold the values
+                ARTEA_HIP_TRY(e);
+            }
+            idx->d_u8 = d;
         }
     }
     *active = idx->d_u8 ? 1 : 0;

@@ -121,7 +121,9 @@ const float* artea_index_device_vectors(const artea_index* idx);
  * FP32 chains' bit for bit, from a quarter of the bytes.  Other queries of
  * the same batch use the FP32 rows.  enable = 0 drops the copy, enable = 1
  * builds it again when the values allow; *active (optional) tells whether
- * the index has it.  artea_index_u8_rows returns 1 when it has, else 0. */
+ * the index has it.  artea_index_u8_rows returns 1 when it has, else 0.
+ * Launch graphs (artea_batched_router_capture) record the rows in use when
+ * they are captured: capture again after dropping or rebuilding the copy. */
 int artea_index_set_u8_rows(artea_index* idx, int enable, int* active);
 int artea_index_u8_rows(const artea_index* idx);
 int artea_index_destroy(artea_index* idx);
