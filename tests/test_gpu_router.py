@@ -115,10 +115,16 @@ def small_l2(orc):
     return g, vecs, qs
 
 
-@pytest.mark.parametrize("topk,queue", [(1, 1), (10, 10), (10, 64), (100, 100), (100, 200), (50, 400), (100, 800)])
-def test_random_index_parity_l2sqr(orc, small_l2, topk, queue):
+@pytest.mark.parametrize("rows", ["u8", "f32"])
+@pytest.mark.parametrize("topk,queue", [(1, 1), (10, 10), (10, 64), (100, 100), (100, 200), (50, 400), (100, 800),
+                                        (100, 1008)])
+def test_random_index_parity_l2sqr(orc, small_l2, topk, queue, rows):
+    # The data are whole numbers in [0, 255]: the index keeps one-byte rows
+    # unless they are dropped.  queue 1008 leaves 16 entries of tail room in
+    # the 16-block list, so merges that run past it are frequent.
     g, vecs, qs = small_l2
     idx = device_index(orc, g, vecs, 0)
+    assert idx.set_u8_rows(rows == "u8") == (rows == "u8")
     r = aa.HierarchicalGraphRouter(idx, None, topk, queue)
     gv, gd = r.batch_query(qs)
     ov, od, ndc = orc.batch_query(g, vecs, orc.L2SQR, qs, topk, queue)
