@@ -100,7 +100,7 @@ int default_variant(uint32_t K, bool u8) { return u8 ? kNumVariants : (K <= 4 ? 
 // one-byte-row kernels (only while the index keeps one-byte rows).
 int num_slots(const artea_index* idx) { return idx->d_u8 ? 2 * kNumVariants : kNumVariants; }
 
-size_t lds_bytes(uint32_t D, uint32_t K) { return kWavesPerBlock * wave_lds_bytes(D, K); }
+size_t lds_bytes(uint32_t D, uint32_t K, bool u8) { return kWavesPerBlock * wave_lds_bytes(D, K, u8); }
 
 }  // namespace
 
@@ -229,7 +229,8 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     const int variant = slot % kNumVariants;
     const KernelFn fn = persistent ? (u8 ? ks.persistent_u8[variant] : ks.persistent[variant])
                                    : (has_u8 ? ks.plain_u8 : ks.plain);
-    const size_t lds = lds_bytes(a.D, K);
+    const bool fn_u8 = (persistent ? u8 : has_u8) && ks.u8_kernels;  // the kernel's LDS layout
+    const size_t lds = lds_bytes(a.D, K, fn_u8);
     if (lds > 65536) {
         // above the default dynamic-LDS limit (large queues, wide rows)
         ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -51,30 +51,34 @@ constexpr uint32_t kVidMask = 0x7FFFFFFFu;
 
 
 
-// Visited cache: two-choice hashing over 7.5 KiB of codes.  h1 = v * A1 and
-// h2 = v * A2 mod 2^B (odd multipliers) are bijections of [0, 2^B), so a
-// slot (h mod S) and a quotient (h div S) identify v exactly.  Choice 1
-// stores quotient + 1 in its slot, choice 2 stores quotient + 1 + (the
-// number of choice-1 codes), so a stored code also says which choice wrote
-// it; 0 marks an empty slot.  Modes: 0 (n <= 2^20) S = 7680 slots of 8-bit
-// codes, choice 1 = 1..137, choice 2 = 138..255 (vertices whose choice-2
-// quotient exceeds 117 use choice 1 only); 1 (n <= 2^27) S = 3840 slots of
-// 16-bit codes, 1..34953 and 34954..65535; 2 (n <= 2^28) as 1 with choice 1
-// only, and a vertex whose code exceeds 16 bits is never stored.  A lookup
-// hits only on v's own code, so the cache never reports a vertex that was
-// not inserted.  An insertion takes choice 1 when empty, else choice 2 when
-// empty, else overwrites choice 1.
+// Visited cache: two-choice hashing over vis_bytes(K, U8) bytes of codes.
+// h1 = v * A1 and h2 = v * A2 mod 2^B (odd multipliers) are bijections of
+// [0, 2^B), so a slot (h mod S) and a quotient (h div S) identify v exactly.
+// Choice 1 stores quotient + 1 in its slot, choice 2 stores quotient + 1 +
+// (the number of choice-1 codes), so a stored code also says which choice
+// wrote it; 0 marks an empty slot.  Modes: 0 (n <= 2^20) S = vis_bytes(K, U8)
+// slots of 8-bit codes; 1 (n <= 2^27) S = vis_bytes(K, U8) / 2 slots of 16-bit
+// codes; 2 (n <= 2^28) as 1 with choice 1 only, and a vertex whose code
+// exceeds 16 bits is never stored.  The choice-2 codes fill what the code
+// width leaves above the choice-1 codes, so vertices with a larger choice-2
+// quotient use choice 1 only.  A lookup hits only on v's own code, so the
+// cache never reports a vertex that was not inserted.  An insertion takes
+// choice 1 when empty, else choice 2 when empty, else overwrites choice 1.
 struct VisKey {
     uint32_t s1, t1, s2, t2;  // t2 == 0: no second choice
 };
+template <int K, bool U8>
 __device__ __forceinline__ VisKey vis_key(uint32_t v, uint32_t hmask, uint32_t mode) {
+    constexpr uint32_t S8 = vis_bytes(K, U8), S16 = vis_bytes(K, U8) / 2;
+    constexpr uint32_t C8 = ((1u << 20) - 1) / S8 + 1, C16 = ((1u << 27) - 1) / S16 + 1;  // choice-1 codes
+    static_assert(C8 < 255 && C16 < 65535, "visited cache too small for its code width");
     const uint32_t h1 = (v * 0x9E3779B1u) & hmask, h2 = (v * 0x85EBCA77u) & hmask;
     if (mode == 0) {
-        const uint32_t q1 = h1 / 7680u, q2 = h2 / 7680u;
-        return VisKey{h1 - q1 * 7680u, q1 + 1, h2 - q2 * 7680u, q2 <= 117u ? q2 + 138u : 0u};
+        const uint32_t q1 = h1 / S8, q2 = h2 / S8;
+        return VisKey{h1 - q1 * S8, q1 + 1, h2 - q2 * S8, q2 + C8 + 1 <= 255u ? q2 + C8 + 1 : 0u};
     }
-    const uint32_t q1 = h1 / 3840u, q2 = h2 / 3840u;
-    return VisKey{h1 - q1 * 3840u, q1 + 1, h2 - q2 * 3840u, (mode == 1 && q2 <= 30581u) ? q2 + 34954u : 0u};
+    const uint32_t q1 = h1 / S16, q2 = h2 / S16;
+    return VisKey{h1 - q1 * S16, q1 + 1, h2 - q2 * S16, (mode == 1 && q2 + C16 + 1 <= 65535u) ? q2 + C16 + 1 : 0u};
 }
 __device__ __forceinline__ uint32_t vis_load(const unsigned char* vis, uint32_t slot, uint32_t mode) {
     return mode ? uint32_t(reinterpret_cast<const uint16_t*>(vis)[slot]) : uint32_t(vis[slot]);
@@ -89,7 +93,7 @@ __device__ __forceinline__ void vis_store(unsigned char* vis, uint32_t slot, uin
 
 struct WaveLds {
     float* q;         // D floats
-    unsigned char* vis;  // kVisBytes
+    unsigned char* vis;  // vis_bytes(K, U8)
     uint32_t* cand;   // 64 (= wv[0..64): staging between collect and a merge)
     float* cdist;     // 64 (= wd[0..64))
     float* wd;        // 64 K: candidate-list distances by position (merge_chunk)
@@ -98,10 +102,11 @@ struct WaveLds {
 
 // Collect the unvisited, valid ids of a 64-id chunk in row order into
 // cand[0..cnt), inserting them into the visited cache; returns cnt.
+template <int K, bool U8>
 __device__ __forceinline__ uint32_t collect(const WaveLds& w, uint32_t id, uint32_t n, uint32_t hmask,
                                             uint32_t mode) {
     const bool valid = id < n;
-    const VisKey key = vis_key(id, hmask, mode);
+    const VisKey key = vis_key<K, U8>(id, hmask, mode);
     const uint32_t c1 = vis_load(w.vis, key.s1, mode), c2 = vis_load(w.vis, key.s2, mode);
     const bool fresh = valid && c1 != key.t1 && (key.t2 == 0 || c2 != key.t2);
     wave_sync();
@@ -592,7 +597,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // Query row into LDS.
     const float* qg = a.queries + size_t(qi) * a.D;
     for (uint32_t i = lane; i < a.D; i += 64) w.q[i] = qg[i];
-    for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+    for (uint32_t i = lane; i < vis_bytes(K, (U8R > 0)) / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
     wave_sync();
     const float qn2 = MF == 2 ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
     const U8Query<CT> u = u8_query<CT, (U8R > 0)>(a, w);
@@ -607,7 +612,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
         for (uint32_t level = a.top; level >= 1; --level) {
             // Greedy walk on `level` (greedy_loop_body); seed marked visited.
             if (lane == 0) {
-                const VisKey k0 = vis_key(cur, a.hmask, a.vis_mode);
+                const VisKey k0 = vis_key<K, (U8R > 0)>(cur, a.hmask, a.vis_mode);
                 vis_store(w.vis, k0.s1, k0.t1, a.vis_mode);
             }
             wave_sync();
@@ -623,7 +628,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 for (uint32_t base = 0; base < a.strideU; base += 64) {
                     const uint32_t id = (base + lane < a.strideU) ? row[base + lane] : 0xFFFFFFFFu;
                     if (ballot(id != 0xFFFFFFFFu) == 0) break;
-                    const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_mode);
+                    const uint32_t cnt = collect<K, (U8R > 0)>(w, id, a.n, a.hmask, a.vis_mode);
                     if (cnt == 0) continue;
                     any_distances<CT, false, MF, 1>(a, w, cnt, u, metric, qn2);
                     ndc += cnt;
@@ -652,7 +657,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
                 cd = best_d;
             }
             // visited.clear() after every level (:228-236).
-            for (uint32_t i = lane; i < kVisBytes / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
+            for (uint32_t i = lane; i < vis_bytes(K, (U8R > 0)) / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
             wave_sync();
         }
     }
@@ -662,7 +667,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     W.init(L);
     W.insert(cd, cur);
     if (lane == 0) {
-        const VisKey k0 = vis_key(cur, a.hmask, a.vis_mode);
+        const VisKey k0 = vis_key<K, (U8R > 0)>(cur, a.hmask, a.vis_mode);
         vis_store(w.vis, k0.s1, k0.t1, a.vis_mode);
     }
     wave_sync();
@@ -687,7 +692,7 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
         for (uint32_t base = 0; base < a.stride0; base += 64) {
             const uint32_t id = base == 0 ? first : ((base + lane < a.stride0) ? row[base + lane] : 0xFFFFFFFFu);
             if (ballot(id != 0xFFFFFFFFu) == 0) break;
-            const uint32_t cnt = collect(w, id, a.n, a.hmask, a.vis_mode);
+            const uint32_t cnt = collect<K, (U8R > 0)>(w, id, a.n, a.hmask, a.vis_mode);
             if (!predicted) {
                 predicted = true;
                 pre_v = W.peek(&pre_d);
@@ -731,11 +736,11 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     return QueryStats{ndc, expansions, W.overflow};
 }
 
-template <int K>
+template <int K, bool U8>
 __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t wid = threadIdx.x / 64;
-    const size_t per_wave = wave_lds_bytes(a.D, K);
+    const size_t per_wave = wave_lds_bytes(a.D, K, U8);
     unsigned char* base = smem + wid * per_wave;
     WaveLds w;
     w.wd = reinterpret_cast<float*>(base);
@@ -743,7 +748,7 @@ __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     w.cdist = w.wd;
     w.cand = w.wv;
     w.vis = reinterpret_cast<unsigned char*>(w.wv + 64 * K);
-    w.q = reinterpret_cast<float*>(w.vis + kVisBytes);
+    w.q = reinterpret_cast<float*>(w.vis + vis_bytes(K, U8));
     return w;
 }
 
@@ -759,7 +764,7 @@ __device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long lon
 // Plain router: one wavefront per query, the grid covers the batch.
 template <int K, int CT, int MF, int U8R>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds<K>(a);
+    const WaveLds w = wave_lds<K, (U8R > 0)>(a);
     const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
     if (qi >= a.nq) return;
     const QueryStats st = route_one<K, CT, MF, (U8R == 0), U8R>(a, w, qi);
@@ -775,7 +780,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
 // one-byte path, 16 * U8R rows per pass.
 template <int K, int CT, int MF, int WAVES, bool WIDE, int U8R>
 __global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_kernel(RouterArgs a) {
-    const WaveLds w = wave_lds<K>(a);
+    const WaveLds w = wave_lds<K, (U8R > 0)>(a);
     while (true) {
         uint32_t qi = 0;
         if (lane_id() == 0) qi = atomicAdd(a.work_counter, 1u);

@@ -27,6 +27,7 @@ KernelSet make_set() {
     s.persistent_u8[1] = router_persistent_kernel<K, CT, MF, 1, false, 2>;
     s.persistent_u8[2] = router_persistent_kernel<K, CT, MF, 1, false, 1>;
     s.persistent_u8[3] = router_persistent_kernel<K, CT, MF, 4, false, 4>;
+    s.u8_kernels = true;
     return s;
 }
 
@@ -38,6 +39,7 @@ KernelSet make_generic() {
     s.plain = s.plain_u8 = router_kernel<K, 0, MF, 0>;
     for (int v = 0; v < kNumVariants; ++v)
         s.persistent[v] = s.persistent_u8[v] = router_persistent_kernel<K, 0, MF, 1, false, 0>;
+    s.u8_kernels = false;
     return s;
 }
 

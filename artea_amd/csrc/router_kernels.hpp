@@ -15,15 +15,26 @@ namespace rk {
 // candidate list (12 with four-wave blocks), which the one-byte kernels'
 // register use (under 128 VGPRs) can take.
 constexpr int kWavesPerBlock = 2;
-constexpr uint32_t kVisBytes = 7680;  // lossy visited cache per wave: 7680 x 8-bit or 3840 x 16-bit codes
+// Lossy visited cache per wave, sized with the candidate list and the row
+// kind.  One-byte-row kernels use under 128 VGPRs, so LDS bounds their
+// residency: 7.5 KiB with lists of up to 4 blocks, 5.5 KiB up to 8, 4.1 KiB
+// above keep a wave near 10-11 KiB (14-16 waves per CU at D = 128; a smaller
+// cache there costs fewer recomputed distances than the waves it buys).
+// FP32-row kernels are held to 12 waves per CU by their registers, so their
+// caches take the LDS up to that: 10, 8.5, 6 and 4.5 KiB.  The cache holds
+// vis_bytes(K, U8) 8-bit codes, or half as many 16-bit codes.
+__host__ __device__ constexpr uint32_t vis_bytes(uint32_t K, bool u8) {
+    return u8 ? (K <= 4 ? 7680u : (K <= 8 ? 5632u : 4224u))
+              : (K <= 4 ? 10240u : (K <= 8 ? 8704u : (K <= 13 ? 6144u : 4608u)));
+}
 
 // LDS of one wavefront: the (distance, vid) mirror of a K-block candidate
 // list (the first 64 entries of each half double as the expansion chunk's
 // candidate staging, whose use never overlaps a merge), the visited cache
-// and the query row.  At D = 128 and K = 4 that is 10 KiB, so 16 waves fit
-// a CU's 160 KiB.
-__host__ __device__ inline size_t wave_lds_bytes(uint32_t D, uint32_t K) {
-    return size_t(K) * 512 + kVisBytes + (size_t(D) * 4 + 15) / 16 * 16;
+// and the query row.  At D = 128 and K = 4 that is 10 KiB with one-byte rows,
+// so 16 waves fit a CU's 160 KiB.
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t D, uint32_t K, bool u8) {
+    return size_t(K) * 512 + vis_bytes(K, u8) + (size_t(D) * 4 + 15) / 16 * 16;
 }
 
 struct RouterArgs {
@@ -49,7 +60,7 @@ struct RouterArgs {
     float* out_dists;
     unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
     uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
-    uint32_t vis_mode;                            // visited cache: 0 7680 x 8-bit (n <= 2^20), 1 3840 x 16-bit (n <= 2^27), 2 as 1 with one choice
+    uint32_t vis_mode;                            // visited cache codes: 0 8-bit (n <= 2^20), 1 16-bit (n <= 2^27), 2 as 1 with one choice
     uint32_t* work_counter;                       // persistent mode when non-null
 };
 
@@ -70,6 +81,7 @@ struct KernelSet {
     KernelFn persistent[kNumVariants];
     KernelFn plain_u8;  // with the one-byte path (indexes that keep one-byte rows)
     KernelFn persistent_u8[kNumVariants];
+    bool u8_kernels;    // the *_u8 entries carry the one-byte path (and its LDS layout)
 };
 
 // Metric family 0 = L2SQR / L2, 1 = DOT, 2 = COSINE; K = 64-entry register
