@@ -166,7 +166,17 @@ __device__ __forceinline__ void distances(const RouterArgs& a, const WaveLds& w,
         const uint32_t r = g + (lane >> 2);
         if (r < cnt) {
             const uint32_t v = w.cand[r];
-            const float d = quad_distance<CT>(w.q, a.vecs + size_t(v) * a.D, int(a.C), metric, sub, qn2);
+            float d;
+            if constexpr (CT > 0) {
+                // every chunk's load issued before the chain starts
+                const float* x = a.vecs + size_t(v) * a.D;
+                float4 xv[CT];
+#pragma unroll
+                for (int c = 0; c < CT; ++c) xv[c] = *reinterpret_cast<const float4*>(x + c * 16 + sub * 4);
+                d = quad_finish<CT>(w.q, xv, metric, sub, qn2);
+            } else {
+                d = quad_distance<CT>(w.q, a.vecs + size_t(v) * a.D, int(a.C), metric, sub, qn2);
+            }
             if (sub == 0) w.cdist[r] = d;
         }
     }
