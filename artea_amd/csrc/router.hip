@@ -6,29 +6,11 @@
 // beam search with StdCandidateQueue (router/data_structures/
 // std_candidate_queue.hpp:358-513) and results by (distance, vid).
 //
-// Device design:
-//  * The candidate list W is held in registers: K blocks of 64 (distance,
-//    vid|explored) pairs, entry i in block i/64, lane i%64, sorted by
-//    (distance, vid).  Positions [0, L) are the reference's top-L max-heap T;
-//    positions >= L hold evicted, still-unexplored entries whose distance
-//    equals the current lower bound (the only stale members of the
-//    reference's unexplored heap U that can still be popped: U's top is
-//    popped only while its distance is <= lb, and lb never grows once T is
-//    full).  Popping = first unexplored entry of W.
-//  * try_push's distance-only rejection (`|T| >= L && d >= lb`) is applied to
-//    the candidates of an expansion in CSR order; only candidates below the
-//    current bound are visited serially (ballot + find-first-set).
-//  * The visited table is a lossy direct-mapped cache of vids in LDS.  It
-//    never reports a vertex that was not inserted, and a vertex it forgets
-//    is recomputed: in the beam the recomputed candidate is either still in
-//    W (dropped as a duplicate key) or has distance >= lb with T full (the
-//    rule rejects it), and in a greedy walk a revisited vertex can never be
-//    strictly closer than the current best.  Results therefore equal the
-//    reference's for every cache size; only the distance count can grow.
-//  * Distances: quads of lanes per candidate row, 32 rows per pass (every
-//    row's loads issued before any reduction), see device.hpp
-//    quad_distance2.  Insertions shift the register blocks with DPP
-//    wave_shr:1, and the tail is trimmed once per expansion chunk.
+// This file is the host side: kernel choice by metric family, list size and
+// row width (pick_kernels), launch sizing (LDS per wave, occupancy-sized
+// persistent grids), the batched router's variant slots, autotuner and
+// launch graphs, and the C-ABI entry points.  The device design is described
+// in router_device.hpp.
 #include <algorithm>
 #include <cfloat>
 #include <cstring>

@@ -21,7 +21,7 @@
 //    the 64 candidates of an expansion chunk against the current bound; the
 //    survivors are decided and merged into W together (merge_chunk), with
 //    the same outcome as pushing them one by one in CSR order.
-//  * The visited table is a lossy direct-mapped cache of vids in LDS.  It
+//  * The visited table is a lossy two-choice cache of vids in LDS.  It
 //    never reports a vertex that was not inserted, and a vertex it forgets
 //    is recomputed: in the beam the recomputed candidate is either still in
 //    W (dropped as a duplicate key) or has distance >= lb with T full (the
@@ -30,7 +30,8 @@
 //    reference's for every cache size; only the distance count can grow.
 //  * Distances: quads of lanes per candidate row, 32 rows per pass (every
 //    row's loads issued before any reduction), see device.hpp
-//    quad_distance2.  A merge moves every register block in one
+//    quad_distance2; from one-byte rows (whole-number data) in exact
+//    integer arithmetic, 16-64 rows per pass (distances_u8).  A merge moves every register block in one
 //    ds_bpermute gather, and the tail is trimmed once per expansion chunk.
 #pragma once
 
