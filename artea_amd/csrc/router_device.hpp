@@ -31,8 +31,9 @@
 //  * Distances: quads of lanes per candidate row, 32 rows per pass (every
 //    row's loads issued before any reduction), see device.hpp
 //    quad_distance2; from one-byte rows (whole-number data) in exact
-//    integer arithmetic, 16-64 rows per pass (distances_u8).  A merge moves every register block in one
-//    ds_bpermute gather, and the tail is trimmed once per expansion chunk.
+//    integer arithmetic, 16-64 rows per pass (distances_u8).  A merge
+//    moves every register block in one ds_bpermute gather, and the tail is
+//    trimmed once per expansion chunk.
 #pragma once
 
 #include <cfloat>
