@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--queues", type=str, default="100,200,400,800")
     ap.add_argument("--target-recall", type=float, default=0.95)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bounded CPU-baseline sample")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="streams the timed steps go round (consecutive batches overlap); 1 = one batch at a time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--f32-rows", action="store_true",
                     help="search the FP32 rows only (drop the exact one-byte copy of whole-number data)")
@@ -361,41 +363,93 @@ def main():
     chosen = ok[0]["queue"] if ok else sweep[-1]["queue"]
     br, p10, p100, run_batched, run_plain, st, graph, var_ms, ql_d = routers[chosen]
 
+    # One batch on one stream: the kernel's launch duration (the roofline's
+    # time) and the batch latency.
     for _ in range(args.warmup):
         run_batched()
-    t_step = time_it(run_batched, args.steps)
+    t_batch = time_it(run_batched, args.steps)
     st = br.last_stats()
     for _ in range(args.warmup):
         run_plain()
-    t_plain = time_it(run_plain, args.steps)
-    qps = nq / t_step
-    qps_plain = nq / t_plain
-    log("chosen queue %d: batched %.0f q/s (%.3f ms/step), plain %.0f q/s" % (chosen, qps, t_step * 1e3, qps_plain))
+    t_plain_batch = time_it(run_plain, args.steps)
     pred = ov.numpy(stream)
     pdist = od.numpy(stream)
     slot = br.variant()
+
+    # Timed steps: consecutive batches are independent, so the steps go round
+    # `--streams` streams, each with its own routers (work counters), outputs
+    # and launch graph, and one batch's tail overlaps the next batch's start,
+    # as a serving loop's lanes do.  The plain routers are timed the same way.
+    lanes = [(stream, run_batched, run_plain, ov, od, br)]
+    for _ in range(1, max(1, args.streams)):
+        s_j = Stream()
+        br_j = aa.BatchedRouter(idx, 100, chosen)
+        br_j.set_variant(slot)
+        ov_j, od_j = DeviceBuffer.empty((nq, 100), np.uint32), DeviceBuffer.empty((nq, 100), np.float32)
+        g_j = br_j.capture(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov_j.ptr, od_j.ptr)
+        p10_j = aa.HierarchicalGraphRouter(idx, None, 10, chosen)
+        p100_j = aa.HierarchicalGraphRouter(idx, None, 100, chosen)
+        b10 = (DeviceBuffer.empty((len(sel10), 10), np.uint32), DeviceBuffer.empty((len(sel10), 10), np.float32))
+        b100 = (DeviceBuffer.empty((len(sel100), 100), np.uint32), DeviceBuffer.empty((len(sel100), 100), np.float32))
+
+        def run_b(g_j=g_j, s_j=s_j, br_j=br_j):
+            g_j.launch(s_j.h)
+
+        def run_p(p10_j=p10_j, p100_j=p100_j, s_j=s_j, b10=b10, b100=b100):
+            p10_j.batch_query_device(q10.ptr, len(sel10), b10[0].ptr, b10[1].ptr, stream=s_j.h)
+            p100_j.batch_query_device(q100.ptr, len(sel100), b100[0].ptr, b100[1].ptr, stream=s_j.h)
+
+        lanes.append((s_j, run_b, run_p, ov_j, od_j, br_j))
+
+    def time_lanes(fns, reps):
+        for fn in fns:
+            for _ in range(args.warmup):
+                fn()
+        for ln in lanes:
+            ln[0].synchronize()
+        t0 = time.perf_counter()
+        for i in range(reps):
+            fns[i % len(fns)]()
+        for ln in lanes:
+            ln[0].synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    def written_equal(a, b):
+        # A top-k query writes the first k slots of its row; the rest are untouched.
+        return bool(np.array_equal(a[sel100, :100], b[sel100, :100]) and np.array_equal(a[sel10, :10], b[sel10, :10]))
+
+    t_step = time_lanes([ln[1] for ln in lanes], args.steps)
+    t_plain = time_lanes([ln[2] for ln in lanes], args.steps)
+    lanes_equal = all(written_equal(ln[3].numpy(ln[0]), pred) and
+                      written_equal(ln[4].numpy(ln[0]).view(np.uint32), pdist.view(np.uint32)) for ln in lanes)
+    qps = nq / t_step
+    qps_plain = nq / t_plain
+    log("chosen queue %d: batched %.0f q/s (%.3f ms/step over %d streams; one batch alone %.3f ms), plain %.0f q/s "
+        "(alone %.3f ms)" % (chosen, qps, t_step * 1e3, len(lanes), t_batch * 1e3, qps_plain, t_plain_batch * 1e3))
     u8 = idx.u8_rows and slot >= len(aa.BatchedRouter.VARIANTS)
     f32_rows = None
     if u8:
         # The same batch on the FP32 rows (the fastest FP32 slot of the
         # autotune): equal results, its throughput reported beside.
         f32_slot = int(np.argmin(np.asarray(var_ms[: len(aa.BatchedRouter.VARIANTS)])))
-        br.set_variant(f32_slot)
-        gf = br.capture(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov.ptr, od.ptr)
-
-        def run_f32(gf=gf):
-            gf.launch(sh)
-
+        f32_fns = []
+        for s_j, _, _, ov_j, od_j, br_j in lanes:
+            br_j.set_variant(f32_slot)
+            gf = br_j.capture(q_d.ptr, nq, tk_d.ptr, ql_d.ptr, ov_j.ptr, od_j.ptr)
+            f32_fns.append(lambda gf=gf, s_j=s_j: gf.launch(s_j.h))
         for _ in range(args.warmup):
-            run_f32()
-        t_f32 = time_it(run_f32, args.steps)
-        same_f32 = bool(np.array_equal(ov.numpy(stream), pred) and
-                        np.array_equal(od.numpy(stream).view(np.uint32), pdist.view(np.uint32)))
-        br.set_variant(slot)
+            f32_fns[0]()
+        t_f32_batch = time_it(f32_fns[0], args.steps)
+        t_f32 = time_lanes(f32_fns, args.steps)
+        same_f32 = all(written_equal(ln[3].numpy(ln[0]), pred) and
+                       written_equal(ln[4].numpy(ln[0]).view(np.uint32), pdist.view(np.uint32)) for ln in lanes)
+        for ln in lanes:
+            ln[5].set_variant(slot)
         f32_rows = {"qps": round(nq / t_f32, 1), "ms_per_step": round(t_f32 * 1e3, 4),
+                    "batch_ms": round(t_f32_batch * 1e3, 4),
                     "variant": br.variant_names[f32_slot], "results_equal": same_f32}
-        log("FP32 rows: %.0f q/s (%.3f ms/step, %s), results equal %s" % (nq / t_f32, t_f32 * 1e3,
-                                                                           br.variant_names[f32_slot], same_f32))
+        log("FP32 rows: %.0f q/s (%.3f ms/step; one batch alone %.3f ms, %s), results equal %s" % (
+            nq / t_f32, t_f32 * 1e3, t_f32_batch * 1e3, br.variant_names[f32_slot], same_f32))
     # FP32 router distances against FP64 recomputation (same ids), on the
     # top-100 queries: the integer-valued data makes every FP32 partial sum
     # exact, so the expected error is zero.
@@ -421,10 +475,10 @@ def main():
     row_bytes = 4 * 64
     vec_bytes = D if u8 else 4 * D
     alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * vec_bytes + st["expansions"] * row_bytes
-    achieved = alg_bytes / t_step / 1e9
+    achieved = alg_bytes / t_batch / 1e9
     if f32_rows is not None:
         fb = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
-        f32_rows["roofline_frac"] = round(fb / (f32_rows["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        f32_rows["roofline_frac"] = round(fb / (f32_rows["batch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (calibrated on a copy kernel), committed under profiles/.
     traffic = None
@@ -474,6 +528,10 @@ def main():
         ns = args.serve_requests
         pick = np.arange(ns) % nq
         sq, stopk, squeue = queries[pick], topk_q[pick], np.full(ns, chosen, np.uint32)
+        # Untimed warm-up: every lane's routers launch once (first launches
+        # load kernels and size the persistent grids).
+        nw = min(ns, 4 * 2048)
+        srv.run(sq[:nw], stopk[:nw], squeue[:nw], np.zeros(nw))
         loads = []
         same = True
         for i, frac in enumerate((0.25, 0.5, 0.75, 0.9)):
@@ -496,6 +554,10 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(t_step * 1e3, 4),
+        "streams": len(lanes),
+        "batch_ms": round(t_batch * 1e3, 4),
+        "plain_batch_ms": round(t_plain_batch * 1e3, 4),
+        "streams_results_equal": lanes_equal,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -514,12 +576,13 @@ def main():
         "rows": ("one-byte rows (every base and query value a whole number in [0, 255]; integer distances equal "
                  "the FP32 chains bit for bit)" if u8 else "FP32 rows"),
         "f32_rows": f32_rows,
-        "launch": "HIP graph replay (counter-reset kernel + persistent router kernel)",
+        "launch": ("HIP graph replay (counter-reset kernel + persistent router kernel); timed steps alternate over "
+                   "%d streams, each with its own router and outputs; batch_ms = one batch alone" % len(lanes)),
         "distance_max_rel_err_vs_fp64": dist_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "router_persistent_kernel (batched router), one launch per step",
-                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_step * 1e3, 4)},
+                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_batch * 1e3, 4)},
         "cpu_baseline": cpu,
         "p99_ms": serving["loads"][2]["p99_ms"] if serving else None,
         "serving": serving,
