@@ -36,6 +36,17 @@ def queue_class(q: int) -> int:
     raise ValueError("queue size %d exceeds the device maximum" % q)
 
 
+_CLASSES = np.asarray(QUEUE_CLASSES, np.int64)
+
+
+def queue_classes(q: np.ndarray) -> np.ndarray:
+    """queue_class over an array of queue sizes."""
+    q = np.asarray(q, np.int64)
+    if q.size and int(q.max()) > QUEUE_CLASSES[-1]:
+        raise ValueError("queue size %d exceeds the device maximum" % int(q.max()))
+    return _CLASSES[np.searchsorted(_CLASSES, q, side="left")]
+
+
 class PinnedBuffer:
     def __init__(self, shape, dtype):
         self.dtype = np.dtype(dtype)
@@ -55,7 +66,8 @@ class PinnedBuffer:
 
 @dataclass
 class _Launch:
-    ids: np.ndarray   # request ids of one queue-size class segment
+    ids: object       # request ids of one queue-size class segment (index array, or a slice when consecutive)
+    count: int        # requests in the segment
     rows: int         # first row of the segment in the lane's buffers
     event: Event      # recorded after the segment's result copies
 
@@ -79,23 +91,32 @@ class _Lane:
         self.max_topk = max_topk
         self.inflight: List[_Launch] = []
 
-    def cls(self, q: int) -> int:
+    def classes(self, q: np.ndarray) -> np.ndarray:
         # smallest router class that holds the queue and the row stride
-        return queue_class(max(int(q), self.max_topk))
+        return queue_classes(np.maximum(q, self.max_topk))
 
-    def launch(self, ids, queries, topk, queue):
-        m = len(ids)
-        order = np.argsort(np.array([self.cls(q) for q in queue[ids]]), kind="stable")
-        ids = ids[order]
-        self.hq.array[:m] = queries[ids]
-        self.hk.array[:m] = topk[ids]
-        self.hl.array[:m] = queue[ids]
+    def launch(self, lo: int, hi: int, queries, topk, queue):
+        """Launches requests lo..hi-1, grouped by queue-size class."""
+        m = hi - lo
+        cl = self.classes(queue[lo:hi])
+        if m < 2 or bool(np.all(cl[1:] >= cl[:-1])):
+            # already grouped: every segment is a run of consecutive requests
+            ids, classes = None, cl
+            self.hq.array[:m] = queries[lo:hi]
+            self.hk.array[:m] = topk[lo:hi]
+            self.hl.array[:m] = queue[lo:hi]
+        else:
+            order = np.argsort(cl, kind="stable")
+            ids = lo + order
+            classes = cl[order]
+            self.hq.array[:m] = queries[ids]
+            self.hk.array[:m] = topk[ids]
+            self.hl.array[:m] = queue[ids]
         s = self.stream.h
         d = self.hq.array.shape[1]
         check(lib().artea_memcpy_htod(C.c_void_p(self.dq.ptr), C.c_void_p(self.hq.ptr), m * d * 4, C.c_void_p(s)))
         check(lib().artea_memcpy_htod(C.c_void_p(self.dk.ptr), C.c_void_p(self.hk.ptr), m * 4, C.c_void_p(s)))
         check(lib().artea_memcpy_htod(C.c_void_p(self.dl.ptr), C.c_void_p(self.hl.ptr), m * 4, C.c_void_p(s)))
-        classes = np.array([self.cls(q) for q in self.hl.array[:m]])
         # Smallest classes first, each with its own result copies and event,
         # so a request completes when its class does, not when the lane's
         # slowest class does.
@@ -113,7 +134,8 @@ class _Lane:
                                           C.c_void_p(s)))
             ev = Event()
             ev.record(self.stream)
-            self.inflight.append(_Launch(ids[start:end], start, ev))
+            seg = slice(lo + start, lo + end) if ids is None else ids[start:end]
+            self.inflight.append(_Launch(seg, end - start, start, ev))
             start = end
 
     def poll(self) -> List[_Launch]:
@@ -159,8 +181,7 @@ class Server:
             for ln in self.lanes:
                 for seg in ln.poll():
                     t = time.perf_counter() - t0
-                    ids, r0 = seg.ids, seg.rows
-                    m = len(ids)
+                    ids, r0, m = seg.ids, seg.rows, seg.count
                     out_v[ids] = ln.hv.array[r0:r0 + m]
                     out_d[ids] = ln.hd.array[r0:r0 + m]
                     lat[ids] = t - arrivals[ids]
@@ -168,7 +189,7 @@ class Server:
                     progressed = True
                 if not ln.inflight and nxt < n and arrivals[nxt] <= now:
                     end = min(n, int(np.searchsorted(arrivals, now, side="right")), nxt + ln.budget)
-                    ln.launch(np.arange(nxt, end), queries, topk, queue)
+                    ln.launch(nxt, end, queries, topk, queue)
                     nxt = end
                     progressed = True
             if not progressed:

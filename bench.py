@@ -8,7 +8,9 @@ l0_min_distance 1), queries of mixed topk {10, 100} (seeded 50/50 draw).  The
 queue size sweep {100, 200, 400, 800} picks the smallest queue with
 recall@100 >= 0.95 against an FP64 brute-force ground truth; the timed steps
 replay a HIP graph of the batched persistent router at that queue, with the
-kernel variant its autotuner picked on the same batch.  The plain router (one
+kernel variant its autotuner picked on the same batch, alternating over
+`--streams` streams so consecutive batches overlap (one batch alone is timed
+too, and is the roofline's launch duration).  The plain router (one
 launch per topk) is timed beside it, and the oracle (C/OpenMP restatement of
 the reference router) is timed on the host cores for the CPU baseline.
 """
@@ -182,7 +184,7 @@ def workload_build(args):
         stopk = rng.choice(np.array([10, 50, 100], np.uint32), ns)
         squeue = rng.choice(np.array([64, 128, 256, 512, 800], np.uint32), ns)
         squeue = np.maximum(squeue, stopk)
-        srv = Server(idx, max_topk=100, max_queue=800, lanes=4, budget=2048)
+        srv = Server(idx, max_topk=100, max_queue=800, lanes=args.serve_lanes, budget=args.serve_budget)
         # saturation: one closed batch of the same mix
         v, d, lat, wall = srv.run(queries[pick], stopk, squeue, np.zeros(ns))
         sat = ns / wall
@@ -224,8 +226,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--f32-rows", action="store_true",
                     help="search the FP32 rows only (drop the exact one-byte copy of whole-number data)")
-    ap.add_argument("--serve-requests", type=int, default=40000,
+    ap.add_argument("--serve-requests", type=int, default=100000,
                     help="requests per offered-load level of the serving leg (0 = skip)")
+    ap.add_argument("--serve-lanes", type=int, default=None,
+                    help="serving: overlapping launch lanes (streams); default 3 for search, 4 for build")
+    ap.add_argument("--serve-budget", type=int, default=None,
+                    help="serving: most requests one lane launches at once; default 8192 for search, 2048 for build")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one plain copy kernel of the base vectors (counter calibration)")
     ap.add_argument("--graph-cache", type=str, default="",
@@ -237,6 +243,10 @@ def main():
         sys.exit(2)
     if args.level_fraction is None:
         args.level_fraction = 1.0 if args.workload == "search" else 1.0 / 32
+    if args.serve_lanes is None:
+        args.serve_lanes = 3 if args.workload == "search" else 4
+    if args.serve_budget is None:
+        args.serve_budget = 8192 if args.workload == "search" else 2048
     if args.workload == "cosine":
         return workload_cosine(args)
     if args.workload == "build":
@@ -521,17 +531,21 @@ def main():
     serving = None
     if args.serve_requests > 0:
         # Open-loop Poisson arrivals of the same queries at fractions of the
-        # batched peak; continuous batching over 4 overlapping lanes.
+        # batched peak; continuous batching over overlapping lanes.
         from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
 
-        srv = Server(idx, max_topk=100, max_queue=chosen, lanes=4, budget=2048)
+        srv = Server(idx, max_topk=100, max_queue=chosen, lanes=args.serve_lanes, budget=args.serve_budget)
         ns = args.serve_requests
         pick = np.arange(ns) % nq
         sq, stopk, squeue = queries[pick], topk_q[pick], np.full(ns, chosen, np.uint32)
         # Untimed warm-up: every lane's routers launch once (first launches
         # load kernels and size the persistent grids).
-        nw = min(ns, 4 * 2048)
+        nw = min(ns, args.serve_lanes * args.serve_budget)
         srv.run(sq[:nw], stopk[:nw], squeue[:nw], np.zeros(nw))
+        # Closed-loop saturation: every request present at the start.
+        _, _, _, wall = srv.run(sq, stopk, squeue, np.zeros(ns))
+        sat = ns / wall
+        log("serving saturation (all %d requests at once): %.0f q/s" % (ns, sat))
         loads = []
         same = True
         for i, frac in enumerate((0.25, 0.5, 0.75, 0.9)):
@@ -543,8 +557,8 @@ def main():
                           "p99_ms": round(percentile_nearest_rank(lat, 99) * 1e3, 3)})
             log("serving at %.0f q/s offered: %.0f q/s achieved, p50 %.2f ms, p99 %.2f ms" % (
                 rate, ns / wall, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
-        serving = {"requests_per_load": ns, "lanes": 4, "lane_budget": 2048, "queue": chosen,
-                   "results_equal_batch": same, "loads": loads}
+        serving = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": args.serve_budget,
+                   "queue": chosen, "saturation_qps": round(sat), "results_equal_batch": same, "loads": loads}
 
     out = {
         "metric": "QPS @ recall@100>=0.95 (1M x 128 L2), batched vs plain GPU; p99 ms",

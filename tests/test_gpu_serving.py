@@ -28,3 +28,26 @@ def test_results_same_alone_and_in_the_mix(orc):
         assert np.array_equal(va[0, :k], v[i, :k]) and np.array_equal(da[0, :k], d[i, :k]), i
         pv, pd = aa.HierarchicalGraphRouter(idx, None, k, int(queue[i])).batch_query(qs[i:i + 1])
         assert np.array_equal(pv[0], v[i, :k]) and np.array_equal(pd[0], d[i, :k]), i
+
+
+def test_grouped_batches_take_consecutive_segments():
+    """Requests whose queue classes already arrive grouped (ascending here)
+    are launched from consecutive rows; results equal the plain router's."""
+    from artea_amd.serving import Server, poisson_arrivals
+
+    vecs = aa.generate(0, 43, 0, 5000, 128)
+    idx = aa.DeviceIndex(vecs, 0)
+    aa.build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=4)
+    n = 400
+    qs = aa.generate(0, 43, 1, n, 128)
+    topk = np.full(n, 100, np.uint32)
+    topk[::3] = 10
+    queue = np.sort(np.random.default_rng(2).choice([100, 200, 300, 600], n)).astype(np.uint32)
+    srv = Server(idx, max_topk=100, max_queue=800, lanes=2, budget=96)
+    v, d, lat, _ = srv.run(qs, topk, queue, poisson_arrivals(n, 50000.0, seed=2))
+    assert np.all(lat > 0)
+    for k in (10, 100):
+        for L in np.unique(queue):
+            sel = np.where((topk == k) & (queue == L))[0]
+            pv, pd = aa.HierarchicalGraphRouter(idx, None, k, int(L)).batch_query(qs[sel])
+            assert np.array_equal(pv, v[sel, :k]) and np.array_equal(pd, d[sel, :k]), (k, L)
