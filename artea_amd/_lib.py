@@ -55,6 +55,12 @@ SIGNATURES = {
     "artea_batched_router_get_variant": (i32, [P, C.POINTER(i32)]),
     "artea_launch_graph_launch": (i32, [P, P]),
     "artea_launch_graph_destroy": (i32, [P]),
+    "artea_server_create": (i32, [P, u32, u32, u32, u32, PP]),
+    "artea_server_destroy": (i32, [P]),
+    "artea_server_submit": (i32, [P, P, u32, P, P, C.POINTER(u64)]),
+    "artea_server_poll": (i32, [P, u32, P, P, P, C.POINTER(u32)]),
+    "artea_server_poll_into": (i32, [P, u64, u64, P, P, u32, P, C.POINTER(u32)]),
+    "artea_server_status": (i32, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     "artea_distance_matrix_device": (i32, [i32, P, u32, P, u32, u32, P, P]),
     "artea_generate": (i32, [i32, u64, u32, u64, u64, u32, u32, P, i32]),
     "artea_bruteforce_knn_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P]),

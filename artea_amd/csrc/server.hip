@@ -1,0 +1,422 @@
+// Continuous-batching request server over a resident index: requests of
+// mixed topk and queue size are submitted from the host as they arrive and
+// their results are collected by polling; no call blocks on the GPU.
+//
+// `lanes` streams each hold at most `budget` requests in flight.  A pump
+// step, run by every submit and poll call, (1) harvests the launches whose
+// completion event has fired, in launch order per lane, and (2) hands the
+// oldest queued requests (up to `budget`) to every lane with nothing in
+// flight.  A lane launch groups its requests by queue-size class (a stable
+// counting sort), stages them in pinned memory, and launches one batched
+// router per class, smallest class first, each followed by its own result
+// copies and event, so a request completes when its class does.  Every
+// request is routed by its own wavefront with no state shared with the rest
+// of its launch, so its result is what batch_query returns for it alone.
+//
+// Host copies per request: one into the pending queue (submit), one into the
+// lane's pinned staging (launch), and on the way out one from a pinned
+// result block to the caller (poll: its first topk slots, the rest of the
+// row filled with the invalid entry).  poll_into writes each result into the
+// caller's own per-request table, so a caller needs no scatter of its own.  Result blocks come from a pool of two
+// per lane; a completed segment stays in its block until polled, and a lane
+// waits for a free block before it launches again.
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "artea_hip.h"
+#include "common.hpp"
+
+using artea::set_error;
+
+namespace {
+
+// Candidate-list capacities of the router's register layouts (K blocks of 64
+// entries less 16 entries of tie headroom); a request runs on the smallest
+// class that holds max(queue, max_topk).
+constexpr uint32_t kClasses[] = {48, 112, 240, 496, 816, 1008};
+constexpr int kNumClasses = 6;
+constexpr int kBlocksPerLane = 2;
+
+int class_of(uint32_t q) {
+    for (int c = 0; c < kNumClasses; ++c)
+        if (q <= kClasses[c]) return c;
+    return -1;
+}
+
+// Requests of one submit call not yet launched (from `pos`).
+struct Chunk {
+    std::vector<float> q;
+    std::vector<uint32_t> k, l;
+    uint64_t first_ticket = 0;
+    size_t pos = 0;
+};
+
+// Pinned result rows of one lane launch, with the tickets of its rows.
+struct Block {
+    uint32_t* v = nullptr;
+    float* d = nullptr;
+    std::vector<uint64_t> t;
+    std::vector<uint32_t> k;   // topk of each row
+    uint32_t outstanding = 0;  // rows launched or completed but not yet polled
+};
+
+struct Segment {
+    Block* b;
+    uint32_t row0, count;
+    hipEvent_t ev;
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    artea_batched_router* routers[kNumClasses] = {};
+    hipEvent_t events[kNumClasses] = {};
+    float* hq = nullptr;  // pinned staging: queries, topk, queue
+    uint32_t* hk = nullptr;
+    uint32_t* hl = nullptr;
+    float* dq = nullptr;
+    uint32_t* dk = nullptr;
+    uint32_t* dl = nullptr;
+    uint32_t* dv = nullptr;
+    float* dd = nullptr;
+    std::deque<Segment> inflight;
+};
+
+}  // namespace
+
+struct artea_server {
+    artea_index* idx = nullptr;
+    uint32_t dim = 0, max_topk = 0, max_queue = 0, budget = 0;
+    int lo_class = 0, hi_class = 0;
+    std::vector<Lane> lanes;
+    std::vector<Block> blocks;
+    std::vector<Block*> free_blocks;
+    std::deque<Chunk> pending;
+    std::vector<Chunk> spare;   // consumed chunks kept for their storage (no fresh pages per submit)
+    std::deque<Segment> ready;  // completed, in harvest order; row0/count shrink as rows are polled
+    uint64_t next_ticket = 0, queued = 0, in_flight = 0, ready_rows = 0;
+};
+
+namespace {
+
+void lane_free(Lane& ln) {
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+    for (int c = 0; c < kNumClasses; ++c) {
+        if (ln.routers[c]) artea_batched_router_destroy(ln.routers[c]);
+        if (ln.events[c]) (void)hipEventDestroy(ln.events[c]);
+    }
+    for (void* p : {(void*)ln.hq, (void*)ln.hk, (void*)ln.hl})
+        if (p) (void)hipHostFree(p);
+    for (void* p : {(void*)ln.dq, (void*)ln.dk, (void*)ln.dl, (void*)ln.dv, (void*)ln.dd})
+        if (p) (void)hipFree(p);
+    if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    ln = Lane{};
+}
+
+void server_free(artea_server* s) {
+    for (Lane& ln : s->lanes) lane_free(ln);  // synchronises every stream first
+    for (Block& b : s->blocks) {
+        if (b.v) (void)hipHostFree(b.v);
+        if (b.d) (void)hipHostFree(b.d);
+    }
+    delete s;
+}
+
+template <class T>
+int host_alloc(T** p, size_t count) {
+    ARTEA_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+    return ARTEA_OK;
+}
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    ARTEA_HIP_TRY(hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)));
+    return ARTEA_OK;
+}
+
+int lane_init(artea_server* s, Lane& ln) {
+    ARTEA_HIP_TRY(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+    for (int c = s->lo_class; c <= s->hi_class; ++c) {
+        int rc = artea_batched_router_create(s->idx, s->max_topk, kClasses[c], &ln.routers[c]);
+        if (rc) return rc;
+        ARTEA_HIP_TRY(hipEventCreateWithFlags(&ln.events[c], hipEventDisableTiming));
+    }
+    const size_t B = s->budget, D = s->dim, K = s->max_topk;
+    int rc = 0;
+    if ((rc = host_alloc(&ln.hq, B * D)) || (rc = host_alloc(&ln.hk, B)) || (rc = host_alloc(&ln.hl, B)) ||
+        (rc = dev_alloc(&ln.dq, B * D)) || (rc = dev_alloc(&ln.dk, B)) || (rc = dev_alloc(&ln.dl, B)) ||
+        (rc = dev_alloc(&ln.dv, B * K)) || (rc = dev_alloc(&ln.dd, B * K)))
+        return rc;
+    return ARTEA_OK;
+}
+
+// Moves the completed launches of one lane to the ready queue.
+int harvest(artea_server* s, Lane& ln) {
+    while (!ln.inflight.empty()) {
+        const Segment seg = ln.inflight.front();
+        const hipError_t e = hipEventQuery(seg.ev);
+        if (e == hipErrorNotReady) break;
+        ARTEA_HIP_TRY(e);
+        s->ready.push_back(seg);
+        s->in_flight -= seg.count;
+        s->ready_rows += seg.count;
+        ln.inflight.pop_front();
+    }
+    return ARTEA_OK;
+}
+
+// Launches the oldest queued requests (up to the budget) on an idle lane.
+int launch(artea_server* s, Lane& ln) {
+    if (s->queued == 0 || s->free_blocks.empty()) return ARTEA_OK;
+    const uint32_t m = uint32_t(std::min<uint64_t>(s->queued, s->budget));
+    const size_t D = s->dim, K = s->max_topk;
+    Block* b = s->free_blocks.back();
+    s->free_blocks.pop_back();
+    // Source of each request, oldest first, and its class.
+    struct Src {
+        const float* q;
+        uint32_t k, l;
+        uint64_t t;
+        uint8_t c;
+    };
+    std::vector<Src> src(m);
+    uint32_t count[kNumClasses] = {};
+    // queued >= m, so the chunks from the front hold m requests; exhausted
+    // chunks are dropped only after their rows are copied below.
+    auto it = s->pending.begin();
+    for (uint32_t i = 0; i < m; ++it) {
+        Chunk& ch = *it;
+        const size_t take = std::min<size_t>(ch.k.size() - ch.pos, m - i);
+        for (size_t j = 0; j < take; ++j, ++i) {
+            const size_t p = ch.pos + j;
+            const uint8_t c = uint8_t(class_of(std::max(ch.l[p], s->max_topk)));
+            src[i] = Src{ch.q.data() + p * D, ch.k[p], ch.l[p], ch.first_ticket + p, c};
+            ++count[c];
+        }
+        ch.pos += take;
+    }
+    uint32_t first[kNumClasses], next[kNumClasses];
+    next[0] = 0;
+    for (int c = 1; c < kNumClasses; ++c) next[c] = next[c - 1] + count[c - 1];
+    std::memcpy(first, next, sizeof first);
+    for (uint32_t i = 0; i < m; ++i) {
+        const uint32_t row = next[src[i].c]++;
+        std::memcpy(ln.hq + row * D, src[i].q, D * sizeof(float));
+        ln.hk[row] = src[i].k;
+        ln.hl[row] = src[i].l;
+        b->t[row] = src[i].t;
+        b->k[row] = src[i].k;
+    }
+    while (!s->pending.empty() && s->pending.front().pos == s->pending.front().k.size()) {
+        if (s->spare.size() < 8) s->spare.push_back(std::move(s->pending.front()));
+        s->pending.pop_front();
+    }
+    s->queued -= m;
+    b->outstanding = m;
+    ARTEA_HIP_TRY(hipMemcpyAsync(ln.dq, ln.hq, m * D * sizeof(float), hipMemcpyHostToDevice, ln.stream));
+    ARTEA_HIP_TRY(hipMemcpyAsync(ln.dk, ln.hk, m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
+    ARTEA_HIP_TRY(hipMemcpyAsync(ln.dl, ln.hl, m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
+    for (int c = 0; c < kNumClasses; ++c) {
+        if (!count[c]) continue;
+        const size_t r0 = first[c], nc = count[c];
+        int rc = artea_batched_router_query_device(ln.routers[c], ln.dq + r0 * D, uint32_t(nc), ln.dk + r0,
+                                                   ln.dl + r0, 0, nullptr, ln.dv + r0 * K, ln.dd + r0 * K,
+                                                   ln.stream);
+        if (rc) return rc;
+        ARTEA_HIP_TRY(hipMemcpyAsync(b->v + r0 * K, ln.dv + r0 * K, nc * K * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, ln.stream));
+        ARTEA_HIP_TRY(hipMemcpyAsync(b->d + r0 * K, ln.dd + r0 * K, nc * K * sizeof(float), hipMemcpyDeviceToHost,
+                                     ln.stream));
+        ARTEA_HIP_TRY(hipEventRecord(ln.events[c], ln.stream));
+        ln.inflight.push_back(Segment{b, uint32_t(r0), uint32_t(nc), ln.events[c]});
+    }
+    s->in_flight += m;
+    return ARTEA_OK;
+}
+
+int pump(artea_server* s) {
+    for (Lane& ln : s->lanes) {
+        int rc = harvest(s, ln);
+        if (rc) return rc;
+        if (ln.inflight.empty()) {
+            rc = launch(s, ln);
+            if (rc) return rc;
+        }
+    }
+    return ARTEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
+                        artea_server** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(idx && out, "artea_server_create: null argument");
+    ARTEA_CHECK_ARG(max_topk >= 1 && max_topk <= max_queue, "artea_server_create: need 1 <= max_topk <= max_queue");
+    ARTEA_CHECK_ARG(class_of(max_queue) >= 0, "artea_server_create: max_queue exceeds the router's largest queue");
+    ARTEA_CHECK_ARG(lanes >= 1 && lanes <= 16, "artea_server_create: lanes must be in [1, 16]");
+    ARTEA_CHECK_ARG(budget >= 1 && budget <= (1u << 20), "artea_server_create: budget must be in [1, 2^20]");
+    artea_server* s = new artea_server();
+    s->idx = idx;
+    s->dim = idx->dim_pad;
+    s->max_topk = max_topk;
+    s->max_queue = max_queue;
+    s->budget = budget;
+    s->lo_class = class_of(max_topk);
+    s->hi_class = class_of(max_queue);
+    s->lanes.resize(lanes);
+    s->blocks.resize(size_t(lanes) * kBlocksPerLane);
+    int rc = 0;
+    for (Lane& ln : s->lanes)
+        if (!rc) rc = lane_init(s, ln);
+    for (Block& b : s->blocks) {
+        if (!rc) rc = host_alloc(&b.v, size_t(budget) * max_topk);
+        if (!rc) rc = host_alloc(&b.d, size_t(budget) * max_topk);
+        b.t.assign(budget, 0);
+        b.k.assign(budget, 0);
+        s->free_blocks.push_back(&b);
+    }
+    if (rc) {
+        server_free(s);
+        return rc;
+    }
+    *out = s;
+    return ARTEA_OK;
+}
+
+int artea_server_destroy(artea_server* s) {
+    if (s) server_free(s);
+    return ARTEA_OK;
+}
+
+int artea_server_submit(artea_server* s, const float* queries, uint32_t n, const uint32_t* topk,
+                        const uint32_t* queue, uint64_t* first_ticket) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s, "artea_server_submit: null server");
+    ARTEA_CHECK_ARG(n == 0 || (queries && topk && queue), "artea_server_submit: null buffer");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (topk[i] < 1 || topk[i] > s->max_topk || queue[i] > s->max_queue) {
+            set_error("artea_server_submit: request " + std::to_string(i) + " has topk " + std::to_string(topk[i]) +
+                      ", queue " + std::to_string(queue[i]) + " (server: topk 1.." + std::to_string(s->max_topk) +
+                      ", queue <= " + std::to_string(s->max_queue) + ")");
+            return ARTEA_EINVAL;
+        }
+        if (queue[i] < topk[i]) {
+            set_error("candidate_queue_size (" + std::to_string(queue[i]) + ") must be >= topk (" +
+                      std::to_string(topk[i]) + ")");
+            return ARTEA_EINVAL;
+        }
+    }
+    if (first_ticket) *first_ticket = s->next_ticket;
+    if (n) {
+        Chunk ch;
+        if (!s->spare.empty()) {
+            ch = std::move(s->spare.back());
+            s->spare.pop_back();
+        }
+        ch.pos = 0;
+        ch.q.assign(queries, queries + size_t(n) * s->dim);
+        ch.k.assign(topk, topk + n);
+        ch.l.assign(queue, queue + n);
+        ch.first_ticket = s->next_ticket;
+        s->pending.push_back(std::move(ch));
+        s->next_ticket += n;
+        s->queued += n;
+    }
+    return pump(s);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Copies row r of a completed block to (v, d): the router's first topk
+// slots, then the invalid entry up to max_topk.
+inline void copy_row(const Block* b, uint32_t r, size_t K, uint32_t* v, float* d) {
+    const size_t k = std::min<size_t>(b->k[r], K);
+    std::memcpy(v, b->v + r * K, k * sizeof(uint32_t));
+    std::memcpy(d, b->d + r * K, k * sizeof(float));
+    std::fill(v + k, v + K, ARTEA_INVALID_VID);
+    std::fill(d + k, d + K, FLT_MAX);
+}
+
+// Common body of both polls.  table_rows == 0: compact output (row i of the
+// outputs = i-th returned request); else row (ticket - base) of the tables.
+int poll_common(artea_server* s, uint32_t max_results, uint64_t* tickets, uint32_t* out_vids, float* out_dists,
+                uint32_t* n_out, uint64_t base, uint64_t table_rows) {
+    *n_out = 0;
+    int rc = pump(s);
+    if (rc) return rc;
+    const size_t K = s->max_topk;
+    uint32_t got = 0;
+    while (got < max_results && !s->ready.empty()) {
+        Segment& seg = s->ready.front();
+        const uint32_t m = std::min(seg.count, max_results - got);
+        if (table_rows) {
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t t = seg.b->t[seg.row0 + i];
+                if (t < base || t - base >= table_rows) {
+                    set_error("artea_server_poll_into: ticket " + std::to_string(t) + " is outside the table");
+                    *n_out = got;
+                    return ARTEA_EINVAL;
+                }
+            }
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t r = seg.row0 + i;
+            const uint64_t t = seg.b->t[r];
+            const size_t row = table_rows ? size_t(t - base) : size_t(got + i);
+            copy_row(seg.b, r, K, out_vids + row * K, out_dists + row * K);
+            tickets[got + i] = t;
+        }
+        got += m;
+        seg.row0 += m;
+        seg.count -= m;
+        s->ready_rows -= m;
+        seg.b->outstanding -= m;
+        if (seg.b->outstanding == 0) s->free_blocks.push_back(seg.b);
+        if (seg.count == 0) s->ready.pop_front();
+    }
+    *n_out = got;
+    // a freed block may let an idle lane launch
+    if (got) rc = pump(s);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int artea_server_poll(artea_server* s, uint32_t max_results, uint64_t* tickets, uint32_t* out_vids,
+                      float* out_dists, uint32_t* n_out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s && n_out, "artea_server_poll: null argument");
+    ARTEA_CHECK_ARG(max_results == 0 || (tickets && out_vids && out_dists), "artea_server_poll: null buffer");
+    return poll_common(s, max_results, tickets, out_vids, out_dists, n_out, 0, 0);
+}
+
+int artea_server_poll_into(artea_server* s, uint64_t base_ticket, uint64_t table_rows, uint32_t* table_vids,
+                           float* table_dists, uint32_t max_results, uint64_t* tickets, uint32_t* n_out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s && n_out, "artea_server_poll_into: null argument");
+    ARTEA_CHECK_ARG(table_rows > 0 && table_vids && table_dists, "artea_server_poll_into: empty table");
+    ARTEA_CHECK_ARG(max_results == 0 || tickets, "artea_server_poll_into: null buffer");
+    return poll_common(s, max_results, tickets, table_vids, table_dists, n_out, base_ticket, table_rows);
+}
+
+int artea_server_status(artea_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* ready) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s, "artea_server_status: null server");
+    if (queued) *queued = s->queued;
+    if (in_flight) *in_flight = s->in_flight;
+    if (ready) *ready = s->ready_rows;
+    return ARTEA_OK;
+}
+
+}  // extern "C"

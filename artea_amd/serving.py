@@ -1,28 +1,26 @@
-"""Continuous-batching request loop over a resident index.
+"""Continuous-batching request server over a resident index.
 
-Requests (a query row, topk, queue size) arrive over time.  The server keeps
-the index and every request's device state resident, and runs `lanes`
-independent streams; whenever a lane is free it admits the requests that have
-arrived since, up to the lane's budget, and launches them.  Work of different
-lanes overlaps on the device, so requests that arrive while others are on the
-GPU start as soon as a lane frees up.  Each admitted batch is split by queue
-size class so each class runs the register layout sized for it.
+`Server` wraps the library's native server (artea_server_* in
+include/artea_hip.h): requests (a query row, topk, queue size) are submitted
+as they arrive and completed ones are collected by polling.  The native side
+runs `lanes` streams, each holding at most `budget` requests in flight;
+whenever a lane is idle it launches the oldest queued requests, grouped by
+queue-size class (one batched router per class, smallest first, each with its
+own result copies and event), so a request completes when its class does.
 
 Per-request results do not depend on what they are batched with: every query
 is routed by its own wavefront with no shared state besides the work counter.
+`Server.run` replays an arrival schedule against the server and measures each
+request's latency from its arrival to the poll that returned it.
 """
 from __future__ import annotations
 
 import ctypes as C
 import time
-from dataclasses import dataclass
-from typing import List, Optional
 
 import numpy as np
 
-from . import BatchedRouter
 from ._lib import check, lib
-from .device import DeviceBuffer, Event, Stream
 
 # Queue-size classes of the router's register layouts (candidate list of K
 # blocks of 64 entries, 16 entries of tie headroom).
@@ -47,156 +45,108 @@ def queue_classes(q: np.ndarray) -> np.ndarray:
     return _CLASSES[np.searchsorted(_CLASSES, q, side="left")]
 
 
-class PinnedBuffer:
-    def __init__(self, shape, dtype):
-        self.dtype = np.dtype(dtype)
-        self.shape = tuple(shape)
-        n = int(np.prod(shape)) * self.dtype.itemsize
-        p = C.c_void_p()
-        check(lib().artea_host_alloc(max(n, 1), C.byref(p)))
-        self.ptr = p.value
-        self.array = np.ctypeslib.as_array((C.c_uint8 * max(n, 1)).from_address(self.ptr))[:n].view(self.dtype)
-        self.array = self.array.reshape(self.shape)
-
-    def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().artea_host_free(C.c_void_p(self.ptr))
-            self.ptr = None
-
-
-@dataclass
-class _Launch:
-    ids: object       # request ids of one queue-size class segment (index array, or a slice when consecutive)
-    count: int        # requests in the segment
-    rows: int         # first row of the segment in the lane's buffers
-    event: Event      # recorded after the segment's result copies
-
-
-class _Lane:
-    def __init__(self, index, max_topk: int, budget: int, classes):
-        self.stream = Stream()
-        self.routers = {c: BatchedRouter(index, max_topk, c) for c in classes}
-        d = index.dim_pad
-        self.budget = budget
-        self.hq = PinnedBuffer((budget, d), np.float32)
-        self.hk = PinnedBuffer((budget,), np.uint32)
-        self.hl = PinnedBuffer((budget,), np.uint32)
-        self.hv = PinnedBuffer((budget, max_topk), np.uint32)
-        self.hd = PinnedBuffer((budget, max_topk), np.float32)
-        self.dq = DeviceBuffer.empty((budget, d), np.float32)
-        self.dk = DeviceBuffer.empty((budget,), np.uint32)
-        self.dl = DeviceBuffer.empty((budget,), np.uint32)
-        self.dv = DeviceBuffer.empty((budget, max_topk), np.uint32)
-        self.dd = DeviceBuffer.empty((budget, max_topk), np.float32)
-        self.max_topk = max_topk
-        self.inflight: List[_Launch] = []
-
-    def classes(self, q: np.ndarray) -> np.ndarray:
-        # smallest router class that holds the queue and the row stride
-        return queue_classes(np.maximum(q, self.max_topk))
-
-    def launch(self, lo: int, hi: int, queries, topk, queue):
-        """Launches requests lo..hi-1, grouped by queue-size class."""
-        m = hi - lo
-        cl = self.classes(queue[lo:hi])
-        if m < 2 or bool(np.all(cl[1:] >= cl[:-1])):
-            # already grouped: every segment is a run of consecutive requests
-            ids, classes = None, cl
-            self.hq.array[:m] = queries[lo:hi]
-            self.hk.array[:m] = topk[lo:hi]
-            self.hl.array[:m] = queue[lo:hi]
-        else:
-            order = np.argsort(cl, kind="stable")
-            ids = lo + order
-            classes = cl[order]
-            self.hq.array[:m] = queries[ids]
-            self.hk.array[:m] = topk[ids]
-            self.hl.array[:m] = queue[ids]
-        s = self.stream.h
-        d = self.hq.array.shape[1]
-        check(lib().artea_memcpy_htod(C.c_void_p(self.dq.ptr), C.c_void_p(self.hq.ptr), m * d * 4, C.c_void_p(s)))
-        check(lib().artea_memcpy_htod(C.c_void_p(self.dk.ptr), C.c_void_p(self.hk.ptr), m * 4, C.c_void_p(s)))
-        check(lib().artea_memcpy_htod(C.c_void_p(self.dl.ptr), C.c_void_p(self.hl.ptr), m * 4, C.c_void_p(s)))
-        # Smallest classes first, each with its own result copies and event,
-        # so a request completes when its class does, not when the lane's
-        # slowest class does.
-        start = 0
-        while start < m:
-            c = classes[start]
-            end = start + int(np.searchsorted(classes[start:], c, side="right"))
-            self.routers[c].query_device(self.dq.ptr + start * d * 4, end - start, self.dk.ptr + start * 4,
-                                         self.dl.ptr + start * 4, self.dv.ptr + start * self.max_topk * 4,
-                                         self.dd.ptr + start * self.max_topk * 4, stream=s)
-            off, nb = start * self.max_topk * 4, (end - start) * self.max_topk * 4
-            check(lib().artea_memcpy_dtoh(C.c_void_p(self.hv.ptr + off), C.c_void_p(self.dv.ptr + off), nb,
-                                          C.c_void_p(s)))
-            check(lib().artea_memcpy_dtoh(C.c_void_p(self.hd.ptr + off), C.c_void_p(self.dd.ptr + off), nb,
-                                          C.c_void_p(s)))
-            ev = Event()
-            ev.record(self.stream)
-            seg = slice(lo + start, lo + end) if ids is None else ids[start:end]
-            self.inflight.append(_Launch(seg, end - start, start, ev))
-            start = end
-
-    def poll(self) -> List[_Launch]:
-        """Segments completed since the last poll, in launch order."""
-        out = []
-        while self.inflight:
-            done = C.c_int()
-            check(lib().artea_event_query(C.c_void_p(self.inflight[0].event.h), C.byref(done)))
-            if not done.value:
-                break
-            out.append(self.inflight.pop(0))
-        return out
-
-
 class Server:
-    """Open-loop request server.  ``budget`` bounds the queries one lane
-    holds in flight (its device buffers are sized for it); ``lanes`` bounds
-    how many launches overlap."""
+    """Open-loop request server.  ``budget`` bounds the requests one lane
+    holds in flight; ``lanes`` bounds how many launches overlap."""
 
     def __init__(self, index, max_topk: int = 100, max_queue: int = 800, lanes: int = 4, budget: int = 2048):
-        top = queue_class(max(max_queue, max_topk))
-        classes = [c for c in QUEUE_CLASSES if queue_class(max_topk) <= c <= top]
-        self.lanes = [_Lane(index, max_topk, budget, classes) for _ in range(lanes)]
+        h = C.c_void_p()
+        check(lib().artea_server_create(index.h, max_topk, max_queue, lanes, budget, C.byref(h)))
+        self.h = h
+        self.index = index  # the server reads the index's device rows and graph
+        self.dim_pad = index.dim_pad
         self.max_topk = max_topk
+        self.lanes = lanes
         self.budget = budget
+        self._cap = max(lanes * budget, 1)
+        self._t = np.empty(self._cap, np.uint64)
+        self._v = np.empty((self._cap, max_topk), np.uint32)
+        self._d = np.empty((self._cap, max_topk), np.float32)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().artea_server_destroy(self.h)
+            self.h = None
+
+    def submit(self, queries: np.ndarray, topk: np.ndarray, queue: np.ndarray) -> int:
+        """Queues requests (rows of dim_pad floats); returns the first ticket
+        (the others follow consecutively)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        k = np.ascontiguousarray(topk, np.uint32)
+        L = np.ascontiguousarray(queue, np.uint32)
+        n = q.shape[0]
+        if q.ndim != 2 or q.shape[1] != self.dim_pad or k.shape != (n,) or L.shape != (n,):
+            raise ValueError("submit: queries must be n x %d with n topk and n queue values" % self.dim_pad)
+        first = C.c_uint64()
+        check(lib().artea_server_submit(self.h, C.c_void_p(q.ctypes.data), n, C.c_void_p(k.ctypes.data),
+                                        C.c_void_p(L.ctypes.data), C.byref(first)))
+        return first.value
+
+    def poll(self):
+        """Completed requests since the last poll: (tickets, vids, dists) views
+        valid until the next poll."""
+        got = C.c_uint32()
+        check(lib().artea_server_poll(self.h, self._cap, C.c_void_p(self._t.ctypes.data),
+                                      C.c_void_p(self._v.ctypes.data), C.c_void_p(self._d.ctypes.data),
+                                      C.byref(got)))
+        m = got.value
+        return self._t[:m], self._v[:m], self._d[:m]
+
+    def _poll_into(self, base: int, table_v: np.ndarray, table_d: np.ndarray) -> int:
+        """Completed requests written to rows (ticket - base) of the tables;
+        their tickets land in self._t[:count]."""
+        got = C.c_uint32()
+        check(lib().artea_server_poll_into(self.h, base, table_v.shape[0], C.c_void_p(table_v.ctypes.data),
+                                           C.c_void_p(table_d.ctypes.data), self._cap,
+                                           C.c_void_p(self._t.ctypes.data), C.byref(got)))
+        return got.value
+
+    def status(self):
+        """(queued, in flight, completed but not yet polled) request counts."""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().artea_server_status(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def run(self, queries: np.ndarray, topk: np.ndarray, queue: np.ndarray, arrivals: np.ndarray):
         """Serve requests arriving at `arrivals` (seconds from now, sorted).
+        Arrived requests are submitted, at most one lane budget per call,
+        while fewer than two launches' worth per lane are inside the server
+        (queued or in flight); the rest wait on the host, and their wait
+        counts in their latency.
         Returns (vids, dists, latency_seconds, wall_seconds)."""
         n = len(arrivals)
         queries = np.ascontiguousarray(queries, np.float32)
-        topk = np.asarray(topk, np.uint32)
-        queue = np.asarray(queue, np.uint32)
+        topk = np.ascontiguousarray(topk, np.uint32)
+        queue = np.ascontiguousarray(queue, np.uint32)
         out_v = np.full((n, self.max_topk), 0x7FFFFFFF, np.uint32)
         out_d = np.full((n, self.max_topk), np.finfo(np.float32).max, np.float32)
         lat = np.zeros(n, np.float64)
+        window = 2 * self.lanes * self.budget
+        base = 0
         nxt = 0
         done = 0
         t0 = time.perf_counter()
         while done < n:
             now = time.perf_counter() - t0
-            progressed = False
-            for ln in self.lanes:
-                for seg in ln.poll():
-                    t = time.perf_counter() - t0
-                    ids, r0, m = seg.ids, seg.rows, seg.count
-                    out_v[ids] = ln.hv.array[r0:r0 + m]
-                    out_d[ids] = ln.hd.array[r0:r0 + m]
-                    lat[ids] = t - arrivals[ids]
-                    done += m
-                    progressed = True
-                if not ln.inflight and nxt < n and arrivals[nxt] <= now:
-                    end = min(n, int(np.searchsorted(arrivals, now, side="right")), nxt + ln.budget)
-                    ln.launch(nxt, end, queries, topk, queue)
-                    nxt = end
-                    progressed = True
-            if not progressed:
-                if nxt < n and all(not ln.inflight for ln in self.lanes):
-                    wait = arrivals[nxt] - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 1e-3))
+            if nxt < n and arrivals[nxt] <= now and nxt - done < window:
+                # at most one lane budget per submit, so a backlog reaches
+                # the lanes in launch-sized pieces
+                end = min(int(np.searchsorted(arrivals, now, side="right")), done + window, nxt + self.budget)
+                first = self.submit(queries[nxt:end], topk[nxt:end], queue[nxt:end])
+                if nxt == 0:
+                    base = first  # request i has ticket base + i
+                nxt = end
+            m = self._poll_into(base, out_v, out_d) if nxt else 0
+            if m:
+                t = time.perf_counter() - t0
+                ids = (self._t[:m] - np.uint64(base)).astype(np.int64)
+                lat[ids] = t - arrivals[ids]
+                done += m
+            elif nxt < n and done == nxt:
+                # nothing in flight: sleep towards the next arrival
+                wait = arrivals[nxt] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 1e-3))
         return out_v, out_d, lat, time.perf_counter() - t0
 
 

@@ -320,6 +320,38 @@ int artea_bruteforce_knn_mfma_device(int metric, const float* d_base, uint32_t n
                                      uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* d_out_vids,
                                      double* d_out_dists, void* stream, uint64_t* stats);
 
+/* ---------------------------------------------------------------- serving */
+/* Continuous-batching request server (no reference counterpart: the
+ * reference's batch_query, router/hierarchical_graph_router.hpp:293-385, is
+ * one synchronous call per batch; this queues requests across calls).  Each
+ * request carries its own topk (1..max_topk) and queue size (topk..max_queue;
+ * a queue below topk fails as the router ctor does, :104-108) and gets
+ * exactly batch_query's result for it: ids ascending by (distance, vid),
+ * padded to max_topk slots with (ARTEA_INVALID_VID, FLT_MAX).  `lanes`
+ * streams (1..16) each run at most `budget` requests at once; requests are
+ * launched oldest first whenever a lane is idle, grouped by queue-size class.
+ * submit copies the requests (queries n x dim_pad floats) and returns at
+ * once; tickets are consecutive from *first_ticket.  poll never blocks: it
+ * launches what idle lanes can take and returns up to max_results completed
+ * requests (tickets[i], row i of out_vids / out_dists, max_topk entries per
+ * row) in completion order.  status counts queued, in-flight and completed-
+ * but-unreturned requests.  Not thread-safe per handle. */
+typedef struct artea_server artea_server;
+int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
+                        artea_server** out);
+int artea_server_destroy(artea_server* s);
+int artea_server_submit(artea_server* s, const float* queries, uint32_t n, const uint32_t* topk,
+                        const uint32_t* queue, uint64_t* first_ticket);
+int artea_server_poll(artea_server* s, uint32_t max_results, uint64_t* tickets, uint32_t* out_vids,
+                      float* out_dists, uint32_t* n_out);
+/* Same, writing the result of ticket t into row t - base_ticket of the
+ * caller's tables (table_rows rows of max_topk entries each); tickets[i]
+ * lists the requests returned.  A ticket outside the table fails with
+ * ARTEA_EINVAL (the requests returned before it are counted in *n_out). */
+int artea_server_poll_into(artea_server* s, uint64_t base_ticket, uint64_t table_rows, uint32_t* table_vids,
+                           float* table_dists, uint32_t max_results, uint64_t* tickets, uint32_t* n_out);
+int artea_server_status(artea_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* ready);
+
 /* Device plumbing for callers without a GPU framework of their own: plain
  * allocations, copies, streams (default priority) and timing events of the
  * library's HIP runtime. */

@@ -51,3 +51,92 @@ def test_grouped_batches_take_consecutive_segments():
             sel = np.where((topk == k) & (queue == L))[0]
             pv, pd = aa.HierarchicalGraphRouter(idx, None, k, int(L)).batch_query(qs[sel])
             assert np.array_equal(pv, v[sel, :k]) and np.array_equal(pd, d[sel, :k]), (k, L)
+
+
+def test_native_server_submit_poll_through_the_c_abi():
+    """artea_server_*: consecutive tickets, each result equal to the plain
+    router's for its (topk, queue) with the slots beyond topk invalid,
+    partial polls, status counts, and argument errors."""
+    import ctypes as C
+
+    from artea_amd._lib import lib
+    from artea_amd.serving import Server
+
+    vecs = aa.generate(0, 47, 0, 4000, 128)
+    idx = aa.DeviceIndex(vecs, 0)
+    aa.build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=6)
+    n = 300
+    qs = aa.generate(0, 47, 1, n, 128)
+    rng = np.random.default_rng(5)
+    topk = rng.choice([1, 10, 64], n).astype(np.uint32)
+    queue = np.maximum(topk, rng.choice([40, 100, 230, 500], n)).astype(np.uint32)
+    srv = Server(idx, max_topk=64, max_queue=500, lanes=2, budget=50)
+    t0 = srv.submit(qs[:200], topk[:200], queue[:200])
+    t1 = srv.submit(qs[200:], topk[200:], queue[200:])
+    assert t1 == t0 + 200
+    queued, inflight, ready = srv.status()
+    assert queued + inflight + ready == n and inflight <= 2 * 50
+    tickets = np.empty(7, np.uint64)
+    vids = np.empty((7, 64), np.uint32)
+    dists = np.empty((7, 64), np.float32)
+    got = C.c_uint32()
+    seen_v = np.zeros((n, 64), np.uint32)
+    seen_d = np.zeros((n, 64), np.float32)
+    seen = np.zeros(n, bool)
+    import time
+
+    deadline = time.time() + 60
+    while time.time() < deadline:
+        assert lib().artea_server_poll(srv.h, 7, C.c_void_p(tickets.ctypes.data), C.c_void_p(vids.ctypes.data),
+                                       C.c_void_p(dists.ctypes.data), C.byref(got)) == 0
+        m = got.value
+        assert m <= 7
+        ids = (tickets[:m] - np.uint64(t0)).astype(np.int64)
+        assert not seen[ids].any()
+        seen[ids] = True
+        seen_v[ids], seen_d[ids] = vids[:m], dists[:m]
+        if seen.all():
+            break
+    assert seen.all()
+    assert srv.status() == (0, 0, 0)
+    for k in np.unique(topk):
+        for L in np.unique(queue[topk == k]):
+            sel = np.where((topk == k) & (queue == L))[0]
+            pv, pd = aa.HierarchicalGraphRouter(idx, None, int(k), int(L)).batch_query(qs[sel])
+            assert np.array_equal(seen_v[sel, :k], pv) and np.array_equal(seen_d[sel, :k], pd), (k, L)
+            assert np.all(seen_v[sel, k:] == 0x7FFFFFFF) and np.all(seen_d[sel, k:] == np.finfo(np.float32).max)
+    # poll_into: the same requests again, each result written to its row of
+    # a per-request table; equal to the compact poll's rows.
+    t2 = srv.submit(qs, topk, queue)
+    tv = np.zeros((n, 64), np.uint32)
+    td = np.zeros((n, 64), np.float32)
+    tix = np.empty(n, np.uint64)
+    left = n
+    deadline = time.time() + 60
+    while left and time.time() < deadline:
+        assert lib().artea_server_poll_into(srv.h, t2, n, C.c_void_p(tv.ctypes.data), C.c_void_p(td.ctypes.data),
+                                            n, C.c_void_p(tix.ctypes.data), C.byref(got)) == 0
+        left -= got.value
+    assert left == 0 and np.array_equal(tv, seen_v) and np.array_equal(td.view(np.uint32), seen_d.view(np.uint32))
+    # a ticket outside the table is refused and stays ready for a later poll
+    t3 = srv.submit(qs[:1], topk[:1], queue[:1])
+    while srv.status()[2] == 0 and time.time() < deadline:
+        # max_results 0: only advances the server (harvest, launch)
+        assert lib().artea_server_poll(srv.h, 0, None, None, None, C.byref(got)) == 0
+    assert srv.status()[2] == 1
+    assert lib().artea_server_poll_into(srv.h, t3 + 1, 4, C.c_void_p(tv.ctypes.data), C.c_void_p(td.ctypes.data), 4,
+                                        C.c_void_p(tix.ctypes.data), C.byref(got)) != 0
+    assert srv.status()[2] == 1
+    assert lib().artea_server_poll_into(srv.h, t3, 1, C.c_void_p(tv.ctypes.data), C.c_void_p(td.ctypes.data), 4,
+                                        C.c_void_p(tix.ctypes.data), C.byref(got)) == 0
+    assert got.value == 1 and int(tix[0]) == t3 and np.array_equal(tv[0], seen_v[0])
+    assert srv.status() == (0, 0, 0)
+    one = qs[:1]
+    for k, L in [(0, 10), (65, 100), (10, 501), (20, 10)]:
+        with pytest.raises(aa.ArteaError):
+            srv.submit(one, np.array([k], np.uint32), np.array([L], np.uint32))
+    assert srv.status() == (0, 0, 0)
+    with pytest.raises(aa.ArteaError):
+        Server(idx, max_topk=64, max_queue=2000)
+    with pytest.raises(aa.ArteaError):
+        Server(idx, max_topk=64, max_queue=500, lanes=0)
