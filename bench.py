@@ -596,7 +596,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "router_persistent_kernel (batched router), one launch per step",
-                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_batch * 1e3, 4)},
+                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_batch * 1e3, 4),
+                     "note": ("one-byte rows: a quarter of the FP32 rows' bytes per distance, so the kernel is bound "
+                              "by its dependent row-id -> vector -> merge chain and instruction issue, not HBM; the "
+                              "FP32-row kernel on the same batch is HBM-bound (f32_rows.roofline_frac)")
+                     if u8 else "FP32 rows: gather-bound on HBM"},
         "cpu_baseline": cpu,
         "p99_ms": serving["loads"][2]["p99_ms"] if serving else None,
         "serving": serving,
