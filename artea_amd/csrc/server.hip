@@ -2,14 +2,13 @@
 // mixed topk and queue size are submitted from the host as they arrive and
 // their results are collected by polling; no call blocks on the GPU.
 //
-// `lanes` streams each hold at most `budget` requests in flight.  A pump
-// step, run by every submit and poll call, (1) harvests the launches whose
-// completion event has fired, in launch order per lane, and (2) hands the
-// oldest queued requests (up to `budget`) to every lane with nothing in
-// flight.  A lane launch groups its requests by queue-size class (a stable
-// counting sort), stages them in pinned memory, and launches one batched
-// router per class, smallest class first, each followed by its own result
-// copies and event, so a request completes when its class does.  Every
+// Requests wait in one FIFO per queue-size class (the router's register
+// layouts).  `lanes` streams each hold at most `budget` requests in flight.
+// A pump step, run by every submit and poll call, (1) harvests the launches
+// whose completion event has fired, and (2) gives every lane with nothing in
+// flight the class of the oldest queued request: up to `budget` of that
+// class's requests, oldest first, staged in pinned memory and run by that
+// class's batched router, followed by the result copies and an event.  Every
 // request is routed by its own wavefront with no state shared with the rest
 // of its launch, so its result is what batch_query returns for it alone.
 //
@@ -48,11 +47,12 @@ int class_of(uint32_t q) {
     return -1;
 }
 
-// Requests of one submit call not yet launched (from `pos`).
+// Requests of one queue-size class from one submit call, not yet launched
+// (from `pos`).
 struct Chunk {
     std::vector<float> q;
     std::vector<uint32_t> k, l;
-    uint64_t first_ticket = 0;
+    std::vector<uint64_t> t;
     size_t pos = 0;
 };
 
@@ -95,7 +95,8 @@ struct artea_server {
     std::vector<Lane> lanes;
     std::vector<Block> blocks;
     std::vector<Block*> free_blocks;
-    std::deque<Chunk> pending;
+    std::deque<Chunk> pending[kNumClasses];  // per queue-size class, oldest first
+    uint64_t queued_class[kNumClasses] = {};
     std::vector<Chunk> spare;   // consumed chunks kept for their storage (no fresh pages per submit)
     std::deque<Segment> ready;  // completed, in harvest order; row0/count shrink as rows are polled
     uint64_t next_ticket = 0, queued = 0, in_flight = 0, ready_rows = 0;
@@ -169,71 +170,57 @@ int harvest(artea_server* s, Lane& ln) {
     return ARTEA_OK;
 }
 
-// Launches the oldest queued requests (up to the budget) on an idle lane.
+// Launches queued requests of one queue-size class on an idle lane: the class
+// of the oldest queued request, up to the budget of its requests, oldest
+// first.  A launch of short searches then never waits behind long ones, and
+// no request is passed over by a later one of its own class.
 int launch(artea_server* s, Lane& ln) {
     if (s->queued == 0 || s->free_blocks.empty()) return ARTEA_OK;
-    const uint32_t m = uint32_t(std::min<uint64_t>(s->queued, s->budget));
+    int c = -1;
+    uint64_t oldest = ~0ull;
+    for (int cc = 0; cc < kNumClasses; ++cc) {
+        if (!s->queued_class[cc]) continue;
+        const Chunk& f = s->pending[cc].front();
+        if (f.t[f.pos] < oldest) {
+            oldest = f.t[f.pos];
+            c = cc;
+        }
+    }
+    const uint32_t m = uint32_t(std::min<uint64_t>(s->queued_class[c], s->budget));
     const size_t D = s->dim, K = s->max_topk;
     Block* b = s->free_blocks.back();
     s->free_blocks.pop_back();
-    // Source of each request, oldest first, and its class.
-    struct Src {
-        const float* q;
-        uint32_t k, l;
-        uint64_t t;
-        uint8_t c;
-    };
-    std::vector<Src> src(m);
-    uint32_t count[kNumClasses] = {};
-    // queued >= m, so the chunks from the front hold m requests; exhausted
-    // chunks are dropped only after their rows are copied below.
-    auto it = s->pending.begin();
-    for (uint32_t i = 0; i < m; ++it) {
-        Chunk& ch = *it;
+    // The class's chunks from the front hold at least m requests; each
+    // chunk's rows are consecutive, so they are staged in block copies.
+    std::deque<Chunk>& q = s->pending[c];
+    for (uint32_t i = 0; i < m;) {
+        Chunk& ch = q.front();
         const size_t take = std::min<size_t>(ch.k.size() - ch.pos, m - i);
-        for (size_t j = 0; j < take; ++j, ++i) {
-            const size_t p = ch.pos + j;
-            const uint8_t c = uint8_t(class_of(std::max(ch.l[p], s->max_topk)));
-            src[i] = Src{ch.q.data() + p * D, ch.k[p], ch.l[p], ch.first_ticket + p, c};
-            ++count[c];
-        }
+        std::memcpy(ln.hq + size_t(i) * D, ch.q.data() + ch.pos * D, take * D * sizeof(float));
+        std::memcpy(ln.hk + i, ch.k.data() + ch.pos, take * sizeof(uint32_t));
+        std::memcpy(ln.hl + i, ch.l.data() + ch.pos, take * sizeof(uint32_t));
+        std::memcpy(b->t.data() + i, ch.t.data() + ch.pos, take * sizeof(uint64_t));
+        std::memcpy(b->k.data() + i, ch.k.data() + ch.pos, take * sizeof(uint32_t));
         ch.pos += take;
+        i += uint32_t(take);
+        if (ch.pos == ch.k.size()) {
+            if (s->spare.size() < 8) s->spare.push_back(std::move(ch));
+            q.pop_front();
+        }
     }
-    uint32_t first[kNumClasses], next[kNumClasses];
-    next[0] = 0;
-    for (int c = 1; c < kNumClasses; ++c) next[c] = next[c - 1] + count[c - 1];
-    std::memcpy(first, next, sizeof first);
-    for (uint32_t i = 0; i < m; ++i) {
-        const uint32_t row = next[src[i].c]++;
-        std::memcpy(ln.hq + row * D, src[i].q, D * sizeof(float));
-        ln.hk[row] = src[i].k;
-        ln.hl[row] = src[i].l;
-        b->t[row] = src[i].t;
-        b->k[row] = src[i].k;
-    }
-    while (!s->pending.empty() && s->pending.front().pos == s->pending.front().k.size()) {
-        if (s->spare.size() < 8) s->spare.push_back(std::move(s->pending.front()));
-        s->pending.pop_front();
-    }
+    s->queued_class[c] -= m;
     s->queued -= m;
     b->outstanding = m;
     ARTEA_HIP_TRY(hipMemcpyAsync(ln.dq, ln.hq, m * D * sizeof(float), hipMemcpyHostToDevice, ln.stream));
     ARTEA_HIP_TRY(hipMemcpyAsync(ln.dk, ln.hk, m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
     ARTEA_HIP_TRY(hipMemcpyAsync(ln.dl, ln.hl, m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
-    for (int c = 0; c < kNumClasses; ++c) {
-        if (!count[c]) continue;
-        const size_t r0 = first[c], nc = count[c];
-        int rc = artea_batched_router_query_device(ln.routers[c], ln.dq + r0 * D, uint32_t(nc), ln.dk + r0,
-                                                   ln.dl + r0, 0, nullptr, ln.dv + r0 * K, ln.dd + r0 * K,
-                                                   ln.stream);
-        if (rc) return rc;
-        ARTEA_HIP_TRY(hipMemcpyAsync(b->v + r0 * K, ln.dv + r0 * K, nc * K * sizeof(uint32_t),
-                                     hipMemcpyDeviceToHost, ln.stream));
-        ARTEA_HIP_TRY(hipMemcpyAsync(b->d + r0 * K, ln.dd + r0 * K, nc * K * sizeof(float), hipMemcpyDeviceToHost,
-                                     ln.stream));
-        ARTEA_HIP_TRY(hipEventRecord(ln.events[c], ln.stream));
-        ln.inflight.push_back(Segment{b, uint32_t(r0), uint32_t(nc), ln.events[c]});
-    }
+    int rc = artea_batched_router_query_device(ln.routers[c], ln.dq, m, ln.dk, ln.dl, 0, nullptr, ln.dv, ln.dd,
+                                               ln.stream);
+    if (rc) return rc;
+    ARTEA_HIP_TRY(hipMemcpyAsync(b->v, ln.dv, m * K * sizeof(uint32_t), hipMemcpyDeviceToHost, ln.stream));
+    ARTEA_HIP_TRY(hipMemcpyAsync(b->d, ln.dd, m * K * sizeof(float), hipMemcpyDeviceToHost, ln.stream));
+    ARTEA_HIP_TRY(hipEventRecord(ln.events[c], ln.stream));
+    ln.inflight.push_back(Segment{b, 0, m, ln.events[c]});
     s->in_flight += m;
     return ARTEA_OK;
 }
@@ -315,17 +302,42 @@ int artea_server_submit(artea_server* s, const float* queries, uint32_t n, const
     }
     if (first_ticket) *first_ticket = s->next_ticket;
     if (n) {
-        Chunk ch;
-        if (!s->spare.empty()) {
-            ch = std::move(s->spare.back());
-            s->spare.pop_back();
+        const size_t D = s->dim;
+        std::vector<uint8_t> cls(n);
+        uint32_t cnt[kNumClasses] = {};
+        for (uint32_t i = 0; i < n; ++i) {
+            cls[i] = uint8_t(class_of(std::max(queue[i], s->max_topk)));
+            ++cnt[cls[i]];
         }
-        ch.pos = 0;
-        ch.q.assign(queries, queries + size_t(n) * s->dim);
-        ch.k.assign(topk, topk + n);
-        ch.l.assign(queue, queue + n);
-        ch.first_ticket = s->next_ticket;
-        s->pending.push_back(std::move(ch));
+        Chunk* dst[kNumClasses] = {};
+        for (int c = 0; c < kNumClasses; ++c) {
+            if (!cnt[c]) continue;
+            Chunk ch;
+            if (!s->spare.empty()) {
+                ch = std::move(s->spare.back());
+                s->spare.pop_back();
+            }
+            ch.pos = 0;
+            ch.q.clear();
+            ch.k.clear();
+            ch.l.clear();
+            ch.t.clear();
+            ch.q.reserve(size_t(cnt[c]) * D);
+            s->pending[c].push_back(std::move(ch));
+            dst[c] = &s->pending[c].back();
+            s->queued_class[c] += cnt[c];
+        }
+        for (uint32_t i = 0; i < n;) {
+            // runs of consecutive requests of one class are appended at once
+            uint32_t j = i + 1;
+            while (j < n && cls[j] == cls[i]) ++j;
+            Chunk& ch = *dst[cls[i]];
+            ch.q.insert(ch.q.end(), queries + size_t(i) * D, queries + size_t(j) * D);
+            ch.k.insert(ch.k.end(), topk + i, topk + j);
+            ch.l.insert(ch.l.end(), queue + i, queue + j);
+            for (uint32_t r = i; r < j; ++r) ch.t.push_back(s->next_ticket + r);
+            i = j;
+        }
         s->next_ticket += n;
         s->queued += n;
     }

@@ -3,10 +3,11 @@
 `Server` wraps the library's native server (artea_server_* in
 include/artea_hip.h): requests (a query row, topk, queue size) are submitted
 as they arrive and completed ones are collected by polling.  The native side
-runs `lanes` streams, each holding at most `budget` requests in flight;
-whenever a lane is idle it launches the oldest queued requests, grouped by
-queue-size class (one batched router per class, smallest first, each with its
-own result copies and event), so a request completes when its class does.
+keeps one queue per queue-size class and runs `lanes` streams, each holding
+at most `budget` requests in flight; an idle lane takes the class of the
+oldest waiting request and launches up to `budget` of that class's requests
+with its batched router, so short searches never wait behind long ones in
+the same launch.
 
 Per-request results do not depend on what they are batched with: every query
 is routed by its own wavefront with no shared state besides the work counter.

@@ -328,14 +328,16 @@ int artea_bruteforce_knn_mfma_device(int metric, const float* d_base, uint32_t n
  * a queue below topk fails as the router ctor does, :104-108) and gets
  * exactly batch_query's result for it: ids ascending by (distance, vid),
  * padded to max_topk slots with (ARTEA_INVALID_VID, FLT_MAX).  `lanes`
- * streams (1..16) each run at most `budget` requests at once; requests are
- * launched oldest first whenever a lane is idle, grouped by queue-size class.
+ * streams (1..16) each run at most `budget` requests at once; requests wait
+ * in one queue per queue-size class, and an idle lane takes the class of the
+ * oldest waiting request, up to `budget` of its requests, oldest first.
  * submit copies the requests (queries n x dim_pad floats) and returns at
  * once; tickets are consecutive from *first_ticket.  poll never blocks: it
  * launches what idle lanes can take and returns up to max_results completed
  * requests (tickets[i], row i of out_vids / out_dists, max_topk entries per
  * row) in completion order.  status counts queued, in-flight and completed-
- * but-unreturned requests.  Not thread-safe per handle. */
+ * but-unreturned requests.  Not thread-safe per handle; after a failed call
+ * other than an argument error, destroy the server. */
 typedef struct artea_server artea_server;
 int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
                         artea_server** out);
