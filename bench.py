@@ -118,6 +118,98 @@ def workload_cosine(args):
     print(json.dumps(out))
 
 
+def workload_c0(args):
+    """100,000 x 128 L2 (the integer-valued 256-component mixture, seed
+    20261203), 1,000 queries, k = 100 at queue 200: the reference's own
+    CPU-runnable configuration.  The hierarchy is built on the GPU and by the
+    oracle on the host cores (the two HGRA files compared byte for byte);
+    then the GPU routers (plain, and batched as a HIP graph) and the oracle
+    router search the same graph (results compared slot for slot), with
+    recall@100 against the FP64 ground truth."""
+    import artea_amd as aa
+    from artea_amd.device import DeviceBuffer, Event, Stream, bruteforce_knn
+
+    n, nq, D, seed, k, L = 100_000, 1000, 128, 20261203, 100, 200
+    base = aa.generate(0, seed, 0, n, D)
+    queries = aa.generate(0, seed, 1, nq, D)
+    params = dict(ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
+                  l0_min_distance=1.0, seed=seed)
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    t0 = time.time()
+    aa.build_full(idx, **params)
+    gpu_build = time.time() - t0
+    log("C0 GPU build %.2fs" % gpu_build)
+    stream = Stream()
+    q_d = DeviceBuffer.from_numpy(queries)
+    view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
+    gt_i, _ = bruteforce_knn(aa.DistanceMetrics.EUCLIDEAN_SQR, view, n, q_d, nq, D, k, stream)
+    # plain router (one launch) and batched router (autotuned, HIP graph)
+    ov = DeviceBuffer.empty((nq, k), np.uint32)
+    od = DeviceBuffer.empty((nq, k), np.float32)
+    plain = aa.HierarchicalGraphRouter(idx, None, k, L)
+    tk = DeviceBuffer.from_numpy(np.full(nq, k, np.uint32))
+    tl = DeviceBuffer.from_numpy(np.full(nq, L, np.uint32))
+    br = aa.BatchedRouter(idx, k, L)
+    br.autotune(q_d.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr)
+    graph = br.capture(q_d.ptr, nq, tk.ptr, tl.ptr, ov.ptr, od.ptr)
+    pv_d, pd_d = DeviceBuffer.empty((nq, k), np.uint32), DeviceBuffer.empty((nq, k), np.float32)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        e0, e1 = Event(), Event()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1) / args.steps / 1e3
+
+    t_plain = timed(lambda: plain.batch_query_device(q_d.ptr, nq, pv_d.ptr, pd_d.ptr, stream=stream.h))
+    t_batched = timed(lambda: graph.launch(stream.h))
+    gv, gd = ov.numpy(stream), od.numpy(stream)
+    same_plain = bool(np.array_equal(gv, pv_d.numpy(stream)) and np.array_equal(gd, pd_d.numpy(stream)))
+    rec = recall(gv, gt_i, k)
+    log("C0 GPU search: plain %.0f q/s, batched %.0f q/s, recall@100 %.4f, batched = plain %s" % (
+        nq / t_plain, nq / t_batched, rec, same_plain))
+    out = {"metric": "C0 build seconds and QPS at k=100, queue 200 (100K x 128 L2), GPU vs CPU oracle",
+           "value": round(nq / t_batched, 1), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "u8" if idx.u8_rows else "f32",
+           "data": "synthetic (seeded 256-component integer Gaussian mixture)",
+           "config": {"workload": "C0: 100,000 x 128 L2, 1,000 queries, k 100, queue 200", "n": n, "nq": nq},
+           "gpu_build_s": round(gpu_build, 3), "recall@100": round(rec, 4), "plain_qps": round(nq / t_plain, 1),
+           "batched_qps": round(nq / t_batched, 1), "batched_equals_plain": same_plain}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        t0 = time.time()
+        g, _ = orc.build_full(base, metric=orc.L2, nthreads=cores, **params)
+        cpu_build = time.time() - t0
+        with tempfile.TemporaryDirectory() as d:
+            a, b = os.path.join(d, "gpu.graph"), os.path.join(d, "cpu.graph")
+            idx.snapshot_hgra(a)
+            g.snapshot(b)
+            hgra_same = open(a, "rb").read() == open(b, "rb").read()
+        cv, cd, _ = orc.batch_query(g, base, orc.L2SQR, queries, k, L, nthreads=cores)
+        reps, t0 = 0, time.time()
+        while time.time() - t0 < min(args.cpu_seconds, 5.0) or reps == 0:
+            orc.batch_query(g, base, orc.L2SQR, queries, k, L, nthreads=cores)
+            reps += 1
+        cpu_qps = nq * reps / (time.time() - t0)
+        same_cpu = bool(np.array_equal(cv, gv) and np.array_equal(cd, gd))
+        log("C0 CPU oracle on %d cores: build %.1fs (HGRA identical to the GPU's: %s), search %.0f q/s, results equal "
+            "%s" % (cores, cpu_build, hgra_same, cpu_qps, same_cpu))
+        out["cpu_baseline"] = {"build_s": round(cpu_build, 2), "value": round(cpu_qps, 1), "unit": "queries/s",
+                               "cores": cores, "kind": "port", "hgra_identical_to_gpu": hgra_same,
+                               "results_equal_gpu": same_cpu,
+                               "sample": "the whole C0 workload: oracle build_full and batch_query, OpenMP"}
+    print(json.dumps(out))
+
+
 def workload_build(args):
     """10,000,000 x 96 unit-norm 1024-component mixture, L2: the hierarchy
     build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) with sparse
@@ -217,9 +309,10 @@ def main():
                     help="triangle-join pair distances of the search workload's build (the graph is the same)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
-    ap.add_argument("--workload", choices=("search", "cosine", "build"), default="search",
+    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0"), default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
-                         "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index")
+                         "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index; "
+                         "c0: 100K x 128 build and search, GPU vs the CPU oracle")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -255,6 +348,8 @@ def main():
         args.serve_budget = 8192 if args.workload == "search" else 2048
     if args.workload == "cosine":
         return workload_cosine(args)
+    if args.workload == "c0":
+        return workload_c0(args)
     if args.workload == "build":
         if args.n == 1_000_000:
             args.n = 10_000_000
