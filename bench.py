@@ -115,6 +115,33 @@ def workload_cosine(args):
     cos64 = 1.0 - (xb * qq[:, None, :]).sum(-1) / np.sqrt((xb * xb).sum(-1) * (qq * qq).sum(-1)[:, None])
     out["distance_max_abs_err_vs_fp64"] = float(np.max(np.abs(pd.astype(np.float64) - cos64)))
     out["value"] = out["topk100"]["chosen"]["qps"]
+    if not args.no_cpu:
+        # The oracle router on the same graph (HGRA file), host cores: top-10
+        # at the queue chosen for recall@10, on a sample of the queries.
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        L10 = out["topk10"]["chosen"]["queue"]
+        with tempfile.TemporaryDirectory() as d:
+            p = os.path.join(d, "c2.graph")
+            idx.snapshot_hgra(p)
+            g = orc.Graph.restore(p)
+        sample = queries[:1000]
+        gv, gd = aa.HierarchicalGraphRouter(idx, None, 10, L10).batch_query(sample)
+        reps, t0 = 0, time.time()
+        while time.time() - t0 < args.cpu_seconds or reps == 0:
+            cv, cd, _ = orc.batch_query(g, base, orc.COSINE, sample, 10, L10, nthreads=cores)
+            reps += 1
+        cpu_qps = sample.shape[0] * reps / (time.time() - t0)
+        same = bool(np.array_equal(cv, gv) and np.array_equal(cd, gd))
+        out["cpu_baseline"] = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": cores, "kind": "port",
+                               "sample": "1000 queries, top-10 at queue %d, %d repetitions, oracle restatement of "
+                                         "batch_query on the GPU-built graph" % (L10, reps),
+                               "results_equal_gpu": same}
+        log("cosine CPU oracle: %.0f q/s on %d cores (top-10, queue %d), results equal %s" % (cpu_qps, cores, L10,
+                                                                                           same))
     print(json.dumps(out))
 
 
