@@ -326,6 +326,38 @@ def workload_build(args):
                            loads[-1]["p99_ms_queue800"]))
         out["serving"] = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": args.serve_budget,
                           "saturation_qps": round(sat), "recall_at_topk": round(rec, 4), "loads": loads}
+    if not args.no_cpu:
+        # CPU baseline on a bounded sample: the whole build (same parameters,
+        # same level draw) of the first 500,000 vectors, by the GPU (FP32
+        # join) and by the oracle on the host cores; HGRA files compared.
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        m = min(n, 500_000)
+        sub = np.ascontiguousarray(base[:m])
+        sidx = aa.DeviceIndex(sub, aa.DistanceMetrics.EUCLIDEAN_SQR)
+        t0 = time.time()
+        build_full(sidx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
+                   l0_min_distance=1.0, seed=seed, join=JOIN_FP32, level_fraction=args.level_fraction)
+        gpu_s = time.time() - t0
+        t0 = time.time()
+        g, _ = orc.build_full(sub, metric=orc.L2, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4,
+                              scale=1.10, shift=1.5, l0_min_distance=1.0, seed=seed,
+                              level_fraction=args.level_fraction, nthreads=cores)
+        cpu_s = time.time() - t0
+        with tempfile.TemporaryDirectory() as d:
+            a, b = os.path.join(d, "gpu.graph"), os.path.join(d, "cpu.graph")
+            sidx.snapshot_hgra(a)
+            g.snapshot(b)
+            hgra_same = open(a, "rb").read() == open(b, "rb").read()
+        out["cpu_baseline"] = {"value": round(cpu_s, 2), "unit": "s", "cores": cores, "kind": "port",
+                               "sample": "whole build of the first %d vectors (FP32 join), oracle build_full with "
+                                         "OpenMP" % m,
+                               "gpu_seconds_same_sample": round(gpu_s, 2), "hgra_identical_to_gpu": hgra_same}
+        log("CPU oracle build of the first %d vectors: %.1fs on %d cores (GPU %.2fs), HGRA identical %s" % (
+            m, cpu_s, cores, gpu_s, hgra_same))
     print(json.dumps(out))
 
 
