@@ -326,6 +326,40 @@ def workload_build(args):
                            loads[-1]["p99_ms_queue800"]))
         out["serving"] = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": args.serve_budget,
                           "saturation_qps": round(sat), "recall_at_topk": round(rec, 4), "loads": loads}
+        if not args.no_cpu:
+            # CPU closed loop on a sample of the same mix: the oracle router on
+            # this index's HGRA file, one batch per (topk, queue) group, every
+            # request present at the start; a request's latency is the time
+            # its group's batch completes.  Its lists are compared with the
+            # server's for the same requests.
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as orc
+
+            orc.build()
+            cores = len(os.sched_getaffinity(0))
+            with tempfile.TemporaryDirectory() as tmpd:
+                gp = os.path.join(tmpd, "c3.graph")
+                idx.snapshot_hgra(gp)
+                g = orc.Graph.restore(gp)
+            smp = np.arange(min(ns, 2000))
+            clat = np.zeros(len(smp))
+            same_cpu = True
+            t0 = time.time()
+            for kk in np.unique(stopk[smp]):
+                for LL in np.unique(squeue[smp][stopk[smp] == kk]):
+                    sel = smp[(stopk[smp] == kk) & (squeue[smp] == LL)]
+                    cv, cd, _ = orc.batch_query(g, base, orc.L2SQR, queries[pick[sel]], int(kk), int(LL),
+                                                nthreads=cores)
+                    clat[sel] = time.time() - t0
+                    same_cpu &= bool(np.array_equal(cv, v[sel, :kk]) and np.array_equal(cd, d[sel, :kk]))
+            cwall = time.time() - t0
+            out["serving"]["cpu_closed_loop"] = {
+                "requests": len(smp), "qps": round(len(smp) / cwall, 1), "cores": cores,
+                "p50_ms": round(percentile_nearest_rank(clat, 50) * 1e3, 1),
+                "p99_ms": round(percentile_nearest_rank(clat, 99) * 1e3, 1), "results_equal_gpu": same_cpu}
+            log("CPU oracle closed loop on %d requests of the mix: %.0f q/s on %d cores, p50 %.0f ms, p99 %.0f ms, "
+                "results equal %s" % (len(smp), len(smp) / cwall, cores, out["serving"]["cpu_closed_loop"]["p50_ms"],
+                                       out["serving"]["cpu_closed_loop"]["p99_ms"], same_cpu))
     if not args.no_cpu:
         # CPU baseline on a bounded sample: the whole build (same parameters,
         # same level draw) of the first 500,000 vectors, by the GPU (FP32
