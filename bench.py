@@ -406,7 +406,7 @@ def main():
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index; "
                          "c0: 100K x 128 build and search, GPU vs the CPU oracle")
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
