@@ -685,12 +685,18 @@ def main():
         f32_rows["roofline_frac"] = round(fb / (f32_rows["batch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (calibrated on a copy kernel), committed under profiles/.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "router_c1_traffic.json")
-    if os.path.exists(tpath) and args.n == 1_000_000 and nq == 10_000 and chosen == 200:
-        tj = json.load(open(tpath))
-        if tj.get("rows", "f32") == ("u8" if u8 else "f32"):
-            traffic = tj["traffic_bytes_per_launch"]
+    def measured_traffic(rows):
+        name = "router_c1_traffic.json" if rows == "u8" else "router_c1_traffic_f32.json"
+        tpath = os.path.join(ROOT, "profiles", name)
+        if os.path.exists(tpath) and args.n == 1_000_000 and nq == 10_000 and chosen == 200:
+            tj = json.load(open(tpath))
+            if tj.get("rows") == rows:
+                return tj["traffic_bytes_per_launch"]
+        return None
+
+    traffic = measured_traffic("u8" if u8 else "f32")
+    if f32_rows is not None:
+        f32_rows["traffic"] = measured_traffic("f32")
     log("roofline: %.1f GB algorithmic per launch, %.1f GB/s (%.3f of peak)" % (alg_bytes / 1e9, achieved,
                                                                           achieved / HBM_PEAK_GBS))
 
