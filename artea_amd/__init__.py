@@ -294,6 +294,18 @@ class HierarchicalGraphRouter:
         check(lib().artea_router_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ndc": a.value, "expansions": b.value, "tail_overflows": c.value}
 
+    def last_overflows(self):
+        """(count, batch indices) of the last launch's queries whose candidate
+        list overflowed its tie tail (at most 255 indices listed)."""
+        return _overflows(lib().artea_router_last_overflows, self.h)
+
+
+def _overflows(fn, h):
+    idx = np.zeros(255, np.uint32)
+    cnt = C.c_uint32()
+    check(fn(h, 255, C.c_void_p(idx.ctypes.data), C.byref(cnt)))
+    return cnt.value, idx[:min(cnt.value, 255)].copy()
+
 
 class BatchedRouter:
     """Persistent fused router with per-query topk / queue size."""
@@ -324,6 +336,9 @@ class BatchedRouter:
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().artea_batched_router_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"ndc": a.value, "expansions": b.value, "tail_overflows": c.value}
+
+    def last_overflows(self):
+        return _overflows(lib().artea_batched_router_last_overflows, self.h)
 
     # Kernel variant names by slot: FP32 rows (0..3), and one-byte rows
     # (4..7, while the index keeps them: every value a whole number in [0, 255]).

@@ -45,6 +45,8 @@ SIGNATURES = {
     "artea_router_batch_query": (i32, [P, P, u32, u32, P, P, P]),
     "artea_router_batch_query_device": (i32, [P, P, u32, u32, P, P, P, P]),
     "artea_router_last_stats": (i32, [P, P, P, P]),
+    "artea_router_last_overflows": (i32, [P, u32, P, C.POINTER(u32)]),
+    "artea_batched_router_last_overflows": (i32, [P, u32, P, C.POINTER(u32)]),
     "artea_batched_router_create": (i32, [P, u32, u32, PP]),
     "artea_batched_router_destroy": (i32, [P]),
     "artea_batched_router_query_device": (i32, [P, P, u32, P, P, u32, P, P, P, P]),
@@ -61,6 +63,7 @@ SIGNATURES = {
     "artea_server_poll": (i32, [P, u32, P, P, P, C.POINTER(u32)]),
     "artea_server_poll_into": (i32, [P, u64, u64, P, P, u32, P, C.POINTER(u32)]),
     "artea_server_status": (i32, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+    "artea_server_overflows": (i32, [P, C.POINTER(u64), C.POINTER(u64)]),
     "artea_distance_matrix_device": (i32, [i32, P, u32, P, u32, u32, P, P]),
     "artea_generate": (i32, [i32, u64, u32, u64, u64, u32, u32, P, i32]),
     "artea_bruteforce_knn_device": (i32, [i32, P, u32, P, u32, u32, u32, P, P, P]),

@@ -39,3 +39,11 @@ struct artea_index {
     artea::HostCsr csr;  // host copy for snapshot / info
     hipStream_t stream = nullptr;
 };
+
+namespace artea {
+// Router internals used by the server: the length of a launch's overflow
+// list, and an asynchronous copy of that list (count, then batch indices)
+// into host memory, ordered after the router's latest launch on `stream`.
+uint32_t rk_overflow_list();
+int router_overflow_list_async(artea_batched_router* r, uint32_t* host_dst, hipStream_t stream);
+}  // namespace artea

@@ -2,8 +2,10 @@
 // (persistence/hierarchical_graph_file_manager.hpp:20-40 wire format,
 // :88-145 snapshot, :152-324 restore).  Same validation order and message
 // texts as the reference's ARTEA_ERROR calls.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 
 #include "host.hpp"
@@ -24,7 +26,9 @@ struct File {
 bool rd32(FILE* f, uint32_t* x) { return fread(x, 4, 1, f) == 1; }
 }  // namespace
 
-int hgra_read(const char* path, HostCsr* out) {
+namespace {
+
+int hgra_read_impl(const char* path, HostCsr* out) {
     const std::string p = path ? path : "";
     File fh;
     fh.f = fopen(p.c_str(), "rb");
@@ -33,6 +37,18 @@ int hgra_read(const char* path, HostCsr* out) {
         return ARTEA_EIO;
     }
     FILE* f = fh.f;
+    // Bytes in the file: per-vertex tables are sized from the header only
+    // once the file is long enough to hold them (every vertex takes at least
+    // 4 bytes), so a corrupt count cannot request a huge allocation.
+    uint64_t file_bytes = 0;
+    if (fseek(f, 0, SEEK_END) == 0) {
+        const long e = ftell(f);
+        file_bytes = e > 0 ? uint64_t(e) : 0;
+    }
+    if (fseek(f, 0, SEEK_SET) != 0) {
+        set_error("HierarchicalGraphFileManager::restore: failed to open " + p);
+        return ARTEA_EIO;
+    }
     uint32_t hdr[7];
     if (fread(hdr, 4, 7, f) != 7) {
         set_error("HierarchicalGraphFileManager::restore: header read failed: " + p);
@@ -59,15 +75,26 @@ int hgra_read(const char* path, HostCsr* out) {
         set_error("HierarchicalGraphFileManager::restore: invalid header in " + p);
         return ARTEA_EIO;
     }
+    // The device layout addresses at most 64 levels (the reference has no
+    // such bound; a graph of ln(N / 1000) levels needs 64 only past 2^92
+    // vertices).
+    if (g.top >= 64) {
+        snprintf(buf, sizeof buf,
+                 "HierarchicalGraphFileManager::restore: top_level_id %u exceeds the supported maximum of 63 in ",
+                 g.top);
+        set_error(buf + p);
+        return ARTEA_EIO;
+    }
     const uint32_t groups = g.top + 1;
-    g.highest.assign(g.n, kInvalidLevel);
-    g.local.assign(g.n, kNoVertex);
+    const uint64_t body_words = file_bytes > 28 ? (file_bytes - 28) / 4 : 0;
+    const size_t table = size_t(std::min<uint64_t>(g.n, body_words + 1));
+    g.highest.assign(table, kInvalidLevel);
     g.offsets.assign(groups, {});
     // Per vid, levels h..0: counts then ids, kept in file order first.
     std::vector<uint32_t> lens;
     std::vector<uint32_t> ids;
     std::vector<uint64_t> hist(groups, 0);
-    lens.reserve(size_t(g.n) * 2);
+    lens.reserve(table * 2);
     for (uint32_t v = 0; v < g.n; ++v) {
         uint32_t h;
         if (!rd32(f, &h)) {
@@ -75,7 +102,7 @@ int hgra_read(const char* path, HostCsr* out) {
             set_error(buf + p);
             return ARTEA_EIO;
         }
-        g.highest[v] = h;
+        g.highest[v] = h;  // v < table: vertices 0..v took at least 4(v + 1) bytes of the body
         if (h == kInvalidLevel) continue;
         if (h > g.top) {
             snprintf(buf, sizeof buf,
@@ -103,6 +130,12 @@ int hgra_read(const char* path, HostCsr* out) {
             }
             lens.push_back(c);
             const size_t at = ids.size();
+            if (uint64_t(at) + c > body_words) {  // more ids than the file holds
+                snprintf(buf, sizeof buf,
+                         "HierarchicalGraphFileManager::restore: truncated reading nbrs of vid %u level %u in ", v, l);
+                set_error(buf + p);
+                return ARTEA_EIO;
+            }
             ids.resize(at + c);
             if (c && fread(ids.data() + at, 4, c, f) != c) {
                 snprintf(buf, sizeof buf,
@@ -121,6 +154,9 @@ int hgra_read(const char* path, HostCsr* out) {
             if (l == 0) break;
         }
     }
+    // Every vertex was read: the per-vertex tables can take their full size.
+    g.highest.resize(g.n, kInvalidLevel);
+    g.local.assign(g.n, kNoVertex);
     // Buckets in ascending vid order; row counts, prefix sum, then ids.
     std::vector<uint64_t> fill(groups, 0);
     for (uint32_t h = 0; h < groups; ++h) g.offsets[h].assign(hist[h] * (h + 1) + 1, 0);
@@ -160,6 +196,18 @@ int hgra_read(const char* path, HostCsr* out) {
     }
     *out = std::move(g);
     return ARTEA_OK;
+}
+
+}  // namespace
+
+int hgra_read(const char* path, HostCsr* out) {
+    try {
+        return hgra_read_impl(path, out);
+    } catch (const std::bad_alloc&) {
+        set_error(std::string("HierarchicalGraphFileManager::restore: out of host memory reading ") +
+                  (path ? path : ""));
+        return ARTEA_ENOMEM;
+    }
 }
 
 int hgra_write(const HostCsr& g, const char* path) {

@@ -1289,11 +1289,26 @@ int artea_refine_upload_rows(artea_refine* r, const uint32_t* counts, const uint
     clear_error();
     ARTEA_CHECK_ARG(r && counts, "artea_refine_upload_rows: null argument");
     Ctx& c = r->c;
+    // Every entry is checked on the host before anything reaches the device:
+    // the join and the reverse pass index the vectors and the layer map by
+    // these ids.
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < c.n; ++i) {
+        ARTEA_CHECK_ARG(counts[i] <= c.cap, "artea_refine_upload_rows: row longer than cap");
+        total += counts[i];
+    }
+    ARTEA_CHECK_ARG(total == 0 || entries, "artea_refine_upload_rows: null entries");
+    for (uint64_t pos = 0; pos < total; ++pos) {
+        if ((entries[2 * pos] & kMask) >= c.n_global) {
+            set_error("artea_refine_upload_rows: neighbour id " + std::to_string(entries[2 * pos] & kMask) +
+                      " is not below the vector count " + std::to_string(c.n_global));
+            return ARTEA_EINVAL;
+        }
+    }
     std::vector<uint2> dense(std::max<size_t>(size_t(c.n) * c.cap, 1), make_uint2(0xFFFFFFFFu, 0));
     std::vector<uint32_t> cnt(c.n);
     size_t pos = 0;
     for (uint32_t i = 0; i < c.n; ++i) {
-        ARTEA_CHECK_ARG(counts[i] <= c.cap, "artea_refine_upload_rows: row longer than cap");
         cnt[i] = counts[i];
         for (uint32_t k = 0; k < counts[i]; ++k, ++pos)
             dense[size_t(i) * c.cap + k] = make_uint2(entries[2 * pos], entries[2 * pos + 1]);

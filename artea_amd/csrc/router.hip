@@ -17,7 +17,7 @@
 #include <memory>
 #include <string>
 
-#include "common.hpp"
+#include "router_host.hpp"
 #include "router_kernels.hpp"
 
 using namespace artea;
@@ -29,6 +29,7 @@ namespace {
 // captured launch sequence replays correctly.
 __global__ void reset_kernel(unsigned long long* stats, uint32_t* counter) {
     if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+    if (threadIdx.x == 4) *reinterpret_cast<uint32_t*>(stats + 4) = 0;  // overflow list count
     if (threadIdx.x == 0 && counter) *counter = 0;
 }
 
@@ -86,28 +87,19 @@ size_t lds_bytes(uint32_t D, uint32_t K, bool u8) { return kWavesPerBlock * wave
 
 }  // namespace
 
-struct artea_router {
-    artea_index* idx = nullptr;
-    uint32_t topk = 0, queue = 0;
-    hipStream_t stream = nullptr;
-    unsigned long long* d_stats = nullptr;
-    float* d_thr = nullptr;
-    uint32_t* d_counter = nullptr;
-    bool persistent = false;
-    const void* occ_fn = nullptr;  // cached persistent grid size for (kernel, LDS bytes)
-    size_t occ_lds = 0;
-    uint32_t occ_blocks = 0;
-    int variant = -1;  // persistent kernel variant (-1: default_variant)
-};
-
 struct artea_batched_router {
     artea_router base;
     uint32_t max_topk = 0, max_queue = 0;
 };
 
+// A captured launch sequence keeps its own copy of the early-stop
+// thresholds; it shares the router's work counter and counters, so its
+// replays are ordered with the router's other launches.
 struct artea_launch_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    artea_router* r = nullptr;
+    float* d_thr = nullptr;
 };
 
 namespace {
@@ -117,24 +109,63 @@ int router_init(artea_router* r, artea_index* idx, uint32_t topk, uint32_t queue
     r->topk = topk;
     r->queue = queue;
     ARTEA_HIP_TRY(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-    ARTEA_HIP_TRY(hipMalloc(&r->d_stats, 4 * sizeof(unsigned long long)));
+    ARTEA_HIP_TRY(hipMalloc(&r->d_stats, kStatsBytes));
+    ARTEA_HIP_TRY(hipMemset(r->d_stats, 0, kStatsBytes));  // counters read before any launch are zero
     ARTEA_HIP_TRY(hipMalloc(&r->d_thr, 64 * sizeof(float)));
     ARTEA_HIP_TRY(hipMalloc(&r->d_counter, 64));
+    ARTEA_HIP_TRY(hipEventCreateWithFlags(&r->last, hipEventDisableTiming));
     return ARTEA_OK;
 }
 
 void router_free(artea_router* r) {
+    if (r->last_stream) (void)hipEventSynchronize(r->last);
+    if (r->last) (void)hipEventDestroy(r->last);
     if (r->d_stats) (void)hipFree(r->d_stats);
     if (r->d_thr) (void)hipFree(r->d_thr);
     if (r->d_counter) (void)hipFree(r->d_counter);
     if (r->stream) (void)hipStreamDestroy(r->stream);
 }
 
+// Orders a launch on stream s after the router's previous one.
+int order_after_last(artea_router* r, hipStream_t s) {
+    if (r->last_stream && r->last_stream != s) ARTEA_HIP_TRY(hipStreamWaitEvent(s, r->last, 0));
+    return ARTEA_OK;
+}
+
+int record_last(artea_router* r, hipStream_t s) {
+    ARTEA_HIP_TRY(hipEventRecord(r->last, s));
+    r->last_stream = s;
+    return ARTEA_OK;
+}
+
 // Common launch path of both routers.  The graph must be attached.
+// thr_mode: 0 = enqueue the thresholds' upload on s, 1 = upload them
+// synchronously to d_thr (before a capture), 2 = they are already in d_thr
+// (inside a capture: no ordering event either, the capture is preceded by a
+// synchronisation).  d_thr = nullptr: the router's own buffer.
+int launch_body(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue, const uint32_t* d_topk,
+                const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds,
+                uint32_t* d_ov, float* d_od, hipStream_t s, bool persistent, int thr_mode, float* d_thr);
+
 int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue, const uint32_t* d_topk,
            const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds, uint32_t* d_ov,
-           float* d_od, hipStream_t s, bool persistent, int thr_mode = 0) {
+           float* d_od, hipStream_t s, bool persistent, int thr_mode = 0, float* d_thr = nullptr) {
+    if (thr_mode == 2)
+        return launch_body(r, d_q, nq, topk, queue, d_topk, d_queue, out_stride, flags, thresholds, d_ov, d_od, s,
+                           persistent, thr_mode, d_thr);
+    int rc = order_after_last(r, s);
+    if (rc) return rc;
+    rc = launch_body(r, d_q, nq, topk, queue, d_topk, d_queue, out_stride, flags, thresholds, d_ov, d_od, s,
+                     persistent, thr_mode, d_thr);
+    const int rc2 = record_last(r, s);  // also after a failed launch: whatever was enqueued stays ordered
+    return rc ? rc : rc2;
+}
+
+int launch_body(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue, const uint32_t* d_topk,
+                const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds,
+                uint32_t* d_ov, float* d_od, hipStream_t s, bool persistent, int thr_mode, float* d_thr) {
     artea_index* idx = r->idx;
+    if (!d_thr) d_thr = r->d_thr;
     ARTEA_CHECK_ARG(idx->dg.attached, "router: no graph attached to the index");
     ARTEA_CHECK_ARG((flags & ~uint32_t(ARTEA_EARLY_STOP | ARTEA_L0_ONLY)) == 0, "router: unknown query flags");
     hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, s, r->d_stats, persistent ? r->d_counter : nullptr);
@@ -158,12 +189,11 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
                 return ARTEA_EOVERFLOW;
             }
         }
-        // thr_mode: 0 = enqueue the upload on s, 1 = upload synchronously (before a
-        // capture), 2 = already on the device (inside a capture).
         if (thr_mode == 1) {
-            ARTEA_HIP_TRY(hipMemcpy(r->d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice));
+            ARTEA_HIP_TRY(hipStreamSynchronize(s));
+            ARTEA_HIP_TRY(hipMemcpy(d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice));
         } else if (thr_mode == 0) {
-            ARTEA_HIP_TRY(hipMemcpyAsync(r->d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice, s));
+            ARTEA_HIP_TRY(hipMemcpyAsync(d_thr, thresholds, sizeof(float) * (g.top + 1), hipMemcpyHostToDevice, s));
         }
     }
     RouterArgs a{};
@@ -188,7 +218,7 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
     a.queue_arr = d_queue;
     a.out_stride = out_stride;
     a.flags = flags;
-    a.thresholds = r->d_thr;
+    a.thresholds = d_thr;
     a.out_vids = d_ov;
     a.out_dists = d_od;
     a.stats = r->d_stats;
@@ -241,6 +271,7 @@ int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32
 }
 
 int read_stats(artea_router* r, uint64_t* ndc, uint64_t* exps, uint64_t* ovf) {
+    if (r->last_stream) ARTEA_HIP_TRY(hipEventSynchronize(r->last));
     unsigned long long h[4];
     ARTEA_HIP_TRY(hipMemcpy(h, r->d_stats, sizeof h, hipMemcpyDeviceToHost));
     if (ndc) *ndc = h[0];
@@ -249,7 +280,45 @@ int read_stats(artea_router* r, uint64_t* ndc, uint64_t* exps, uint64_t* ovf) {
     return ARTEA_OK;
 }
 
+int read_overflows(artea_router* r, uint32_t max, uint32_t* qidx, uint32_t* count) {
+    ARTEA_CHECK_ARG(count && (max == 0 || qidx), "last_overflows: null argument");
+    if (r->last_stream) ARTEA_HIP_TRY(hipEventSynchronize(r->last));
+    uint32_t list[1 + kOverflowList];
+    ARTEA_HIP_TRY(hipMemcpy(list, r->d_stats + 4, sizeof list, hipMemcpyDeviceToHost));
+    *count = list[0];
+    const uint32_t listed = std::min(std::min(list[0], kOverflowList), max);
+    std::copy(list + 1, list + 1 + listed, qidx);
+    std::sort(qidx, qidx + listed);
+    return ARTEA_OK;
+}
+
 }  // namespace
+
+namespace artea {
+
+int router_launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue,
+                  const uint32_t* d_topk, const uint32_t* d_queue, uint32_t out_stride, uint32_t flags,
+                  const float* thresholds, uint32_t* d_ov, float* d_od, hipStream_t s, bool persistent, int thr_mode,
+                  float* d_thr) {
+    return launch_body(r, d_q, nq, topk, queue, d_topk, d_queue, out_stride, flags, thresholds, d_ov, d_od, s,
+                       persistent, thr_mode, d_thr);
+}
+
+int router_check_queue(uint32_t queue, uint32_t dim_pad, int metric) {
+    return pick_kernels(queue, dim_pad / 16, metric, nullptr, nullptr);
+}
+
+uint32_t rk_overflow_list() { return kOverflowList; }
+
+int router_overflow_list_async(artea_batched_router* r, uint32_t* host_dst, hipStream_t stream) {
+    int rc = order_after_last(&r->base, stream);
+    if (rc) return rc;
+    ARTEA_HIP_TRY(hipMemcpyAsync(host_dst, r->base.d_stats + 4, (1 + kOverflowList) * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, stream));
+    return ARTEA_OK;
+}
+
+}  // namespace artea
 
 extern "C" {
 
@@ -457,18 +526,21 @@ int artea_batched_router_capture(artea_batched_router* r, const float* d_q, uint
     artea::clear_error();
     ARTEA_CHECK_ARG(r && out, "artea_batched_router_capture: null argument");
     ARTEA_CHECK_ARG(nq == 0 || (d_q && d_ov && d_od && d_topk && d_queue), "artea_batched_router_capture: null buffer");
-    auto g = std::make_unique<artea_launch_graph>();
+    auto g = std::unique_ptr<artea_launch_graph, int (*)(artea_launch_graph*)>(new artea_launch_graph{},
+                                                                                 artea_launch_graph_destroy);
+    g->r = &r->base;
+    ARTEA_HIP_TRY(hipMalloc(&g->d_thr, 64 * sizeof(float)));
     hipStream_t s = r->base.stream;
     // One eager launch sizes the persistent grid (occupancy query) and puts
-    // the early-stop thresholds on the device, so the captured sequence holds
-    // kernel nodes only: the counter reset and the router.
+    // the early-stop thresholds into the graph's own buffer, so the captured
+    // sequence holds kernel nodes only: the counter reset and the router.
     int rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
-                    d_od, s, true, 1);
+                    d_od, s, true, 1, g->d_thr);
     if (rc) return rc;
     ARTEA_HIP_TRY(hipStreamSynchronize(s));
     ARTEA_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     rc = launch(&r->base, d_q, nq, r->max_topk, r->max_queue, d_topk, d_queue, r->max_topk, flags, thresholds, d_ov,
-                d_od, s, true, 2);
+                d_od, s, true, 2, g->d_thr);
     hipGraph_t graph = nullptr;
     const hipError_t ec = hipStreamEndCapture(s, &graph);
     if (rc) {
@@ -482,7 +554,6 @@ int artea_batched_router_capture(artea_batched_router* r, const float* d_q, uint
     g->graph = graph;
     const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
-        (void)hipGraphDestroy(graph);
         set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
         return ARTEA_EHIP;
     }
@@ -493,14 +564,19 @@ int artea_batched_router_capture(artea_batched_router* r, const float* d_q, uint
 int artea_launch_graph_launch(artea_launch_graph* g, void* stream) {
     artea::clear_error();
     ARTEA_CHECK_ARG(g && g->exec, "artea_launch_graph_launch: null graph");
-    ARTEA_HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
-    return ARTEA_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int rc = order_after_last(g->r, s);
+    if (rc) return rc;
+    ARTEA_HIP_TRY(hipGraphLaunch(g->exec, s));
+    return record_last(g->r, s);
 }
 
 int artea_launch_graph_destroy(artea_launch_graph* g) {
     if (!g) return ARTEA_OK;
+    if (g->r && g->r->last_stream) (void)hipEventSynchronize(g->r->last);  // replays may still read d_thr
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_thr) (void)hipFree(g->d_thr);
     delete g;
     return ARTEA_OK;
 }
@@ -510,5 +586,18 @@ int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint
     ARTEA_CHECK_ARG(r, "artea_batched_router_last_stats: null router");
     return read_stats(&r->base, ndc, exps, ovf);
 }
+
+int artea_router_last_overflows(artea_router* r, uint32_t max, uint32_t* qidx, uint32_t* count) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r, "artea_router_last_overflows: null router");
+    return read_overflows(r, max, qidx, count);
+}
+
+int artea_batched_router_last_overflows(artea_batched_router* r, uint32_t max, uint32_t* qidx, uint32_t* count) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(r, "artea_batched_router_last_overflows: null router");
+    return read_overflows(&r->base, max, qidx, count);
+}
+
 
 }  // extern "C"

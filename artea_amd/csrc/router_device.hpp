@@ -764,12 +764,17 @@ __device__ __forceinline__ WaveLds wave_lds(const RouterArgs& a) {
     return w;
 }
 
-__device__ __forceinline__ void add_stats(const RouterArgs& a, unsigned long long ndc, unsigned long long exps,
-                                          unsigned long long ovf) {
+__device__ __forceinline__ void add_stats(const RouterArgs& a, uint32_t qi, unsigned long long ndc,
+                                          unsigned long long exps, unsigned long long ovf) {
     if (lane_id() == 0 && a.stats) {
         atomicAdd(a.stats + 0, ndc);
         atomicAdd(a.stats + 1, exps);
-        if (ovf) atomicAdd(a.stats + 2, ovf);
+        if (ovf) {
+            atomicAdd(a.stats + 2, ovf);
+            uint32_t* list = reinterpret_cast<uint32_t*>(a.stats + 4);
+            const uint32_t at = atomicAdd(list, 1u);
+            if (at < kOverflowList) list[1 + at] = qi;
+        }
     }
 }
 
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void router_kernel(RouterArgs 
     const uint32_t qi = blockIdx.x * kWavesPerBlock + threadIdx.x / 64;
     if (qi >= a.nq) return;
     const QueryStats st = route_one<K, CT, MF, (U8R == 0), U8R>(a, w, qi);
-    add_stats(a, st.ndc, st.expansions, st.overflow);
+    add_stats(a, qi, st.ndc, st.expansions, st.overflow);
 }
 
 // Batched router: a persistent grid sized to the resident wave slots pulls
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64, WAVES) void router_persistent_
         qi = readfirst_u(qi);
         if (qi >= a.nq) break;
         const QueryStats st = route_one<K, CT, MF, WIDE, U8R>(a, w, qi);
-        add_stats(a, st.ndc, st.expansions, st.overflow);
+        add_stats(a, qi, st.ndc, st.expansions, st.overflow);
     }
 }
 

@@ -58,11 +58,18 @@ struct RouterArgs {
     const float* thresholds;                      // device, top+1 entries
     uint32_t* out_vids;
     float* out_dists;
-    unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow
+    unsigned long long* stats;                    // [0] ndc, [1] expansions, [2] tail overflow, [3] spare,
+                                                  // then the overflow list (kStatsBytes in all)
     uint32_t hmask;                               // 2^B - 1, B = max(13, ceil(log2 n))
     uint32_t vis_mode;                            // visited cache codes: 0 8-bit (n <= 2^20), 1 16-bit (n <= 2^27), 2 as 1 with one choice
     uint32_t* work_counter;                       // persistent mode when non-null
 };
+
+// Queries whose candidate list dropped an entry tied with its bound (a
+// result that may differ from the reference's) are listed after the four
+// counters: uint32 count, then up to kOverflowList query indices.
+constexpr uint32_t kOverflowList = 255;
+constexpr size_t kStatsBytes = 4 * sizeof(unsigned long long) + (1 + kOverflowList) * sizeof(uint32_t);
 
 using KernelFn = void (*)(RouterArgs);
 

@@ -60,6 +60,7 @@ struct Chunk {
 struct Block {
     uint32_t* v = nullptr;
     float* d = nullptr;
+    uint32_t* ovf = nullptr;  // the launch's overflow list (count, then rows)
     std::vector<uint64_t> t;
     std::vector<uint32_t> k;   // topk of each row
     uint32_t outstanding = 0;  // rows launched or completed but not yet polled
@@ -69,6 +70,7 @@ struct Segment {
     Block* b;
     uint32_t row0, count;
     hipEvent_t ev;
+    int cls;  // queue-size class it ran in
 };
 
 struct Lane {
@@ -100,6 +102,7 @@ struct artea_server {
     std::vector<Chunk> spare;   // consumed chunks kept for their storage (no fresh pages per submit)
     std::deque<Segment> ready;  // completed, in harvest order; row0/count shrink as rows are polled
     uint64_t next_ticket = 0, queued = 0, in_flight = 0, ready_rows = 0;
+    uint64_t rerun = 0, unresolved = 0;  // tie-tail overflows re-run on a larger list / left as they were
 };
 
 namespace {
@@ -123,6 +126,7 @@ void server_free(artea_server* s) {
     for (Block& b : s->blocks) {
         if (b.v) (void)hipHostFree(b.v);
         if (b.d) (void)hipHostFree(b.d);
+        if (b.ovf) (void)hipHostFree(b.ovf);
     }
     delete s;
 }
@@ -155,6 +159,73 @@ int lane_init(artea_server* s, Lane& ln) {
     return ARTEA_OK;
 }
 
+// Requests of a completed segment whose candidate list overflowed its tie
+// tail (artea_batched_router_last_overflows) run again, synchronously, on
+// the next larger list class until none overflows, and their rows of the
+// block are replaced; one still overflowing in the largest class is counted
+// as unresolved.  The lane's staging still holds the segment's requests: a
+// lane launches again only after its segment was harvested.
+int rerun_overflows(artea_server* s, Lane& ln, const Segment& seg) {
+    const uint32_t listed = seg.b->ovf[0];
+    std::vector<uint32_t> rows;
+    if (listed <= artea::rk_overflow_list()) {
+        rows.assign(seg.b->ovf + 1, seg.b->ovf + 1 + listed);
+    } else {
+        for (uint32_t r = 0; r < seg.count; ++r) rows.push_back(seg.row0 + r);
+    }
+    std::sort(rows.begin(), rows.end());
+    s->rerun += rows.size();
+    const size_t D = s->dim, K = s->max_topk;
+    std::vector<float> q;
+    std::vector<uint32_t> k, l, ov, list(artea::rk_overflow_list());
+    std::vector<float> od;
+    for (int c = seg.cls + 1; !rows.empty(); ++c) {
+        if (c >= kNumClasses) {
+            s->unresolved += rows.size();
+            break;
+        }
+        if (!ln.routers[c]) {
+            int rc = artea_batched_router_create(s->idx, s->max_topk, kClasses[c], &ln.routers[c]);
+            if (rc) return rc;
+        }
+        const size_t m = rows.size();
+        q.resize(m * D);
+        k.resize(m);
+        l.resize(m);
+        for (size_t i = 0; i < m; ++i) {
+            std::memcpy(q.data() + i * D, ln.hq + size_t(rows[i]) * D, D * sizeof(float));
+            k[i] = ln.hk[rows[i]];
+            l[i] = ln.hl[rows[i]];
+        }
+        ARTEA_HIP_TRY(hipMemcpyAsync(ln.dq, q.data(), m * D * sizeof(float), hipMemcpyHostToDevice, ln.stream));
+        ARTEA_HIP_TRY(hipMemcpyAsync(ln.dk, k.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
+        ARTEA_HIP_TRY(hipMemcpyAsync(ln.dl, l.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, ln.stream));
+        int rc = artea_batched_router_query_device(ln.routers[c], ln.dq, uint32_t(m), ln.dk, ln.dl, 0, nullptr, ln.dv,
+                                                   ln.dd, ln.stream);
+        if (rc) return rc;
+        ov.resize(m * K);
+        od.resize(m * K);
+        ARTEA_HIP_TRY(hipMemcpyAsync(ov.data(), ln.dv, m * K * sizeof(uint32_t), hipMemcpyDeviceToHost, ln.stream));
+        ARTEA_HIP_TRY(hipMemcpyAsync(od.data(), ln.dd, m * K * sizeof(float), hipMemcpyDeviceToHost, ln.stream));
+        ARTEA_HIP_TRY(hipStreamSynchronize(ln.stream));
+        for (size_t i = 0; i < m; ++i) {
+            std::memcpy(seg.b->v + size_t(rows[i]) * K, ov.data() + i * K, K * sizeof(uint32_t));
+            std::memcpy(seg.b->d + size_t(rows[i]) * K, od.data() + i * K, K * sizeof(float));
+        }
+        uint32_t again = 0;
+        rc = artea_batched_router_last_overflows(ln.routers[c], uint32_t(list.size()), list.data(), &again);
+        if (rc) return rc;
+        std::vector<uint32_t> next;
+        if (again > list.size()) {
+            next = rows;
+        } else {
+            for (uint32_t i = 0; i < again; ++i) next.push_back(rows[list[i]]);
+        }
+        rows.swap(next);
+    }
+    return ARTEA_OK;
+}
+
 // Moves the completed launches of one lane to the ready queue.
 int harvest(artea_server* s, Lane& ln) {
     while (!ln.inflight.empty()) {
@@ -162,6 +233,10 @@ int harvest(artea_server* s, Lane& ln) {
         const hipError_t e = hipEventQuery(seg.ev);
         if (e == hipErrorNotReady) break;
         ARTEA_HIP_TRY(e);
+        if (seg.b->ovf[0]) {
+            int rc = rerun_overflows(s, ln, seg);
+            if (rc) return rc;
+        }
         s->ready.push_back(seg);
         s->in_flight -= seg.count;
         s->ready_rows += seg.count;
@@ -219,8 +294,10 @@ int launch(artea_server* s, Lane& ln) {
     if (rc) return rc;
     ARTEA_HIP_TRY(hipMemcpyAsync(b->v, ln.dv, m * K * sizeof(uint32_t), hipMemcpyDeviceToHost, ln.stream));
     ARTEA_HIP_TRY(hipMemcpyAsync(b->d, ln.dd, m * K * sizeof(float), hipMemcpyDeviceToHost, ln.stream));
+    rc = artea::router_overflow_list_async(ln.routers[c], b->ovf, ln.stream);
+    if (rc) return rc;
     ARTEA_HIP_TRY(hipEventRecord(ln.events[c], ln.stream));
-    ln.inflight.push_back(Segment{b, 0, m, ln.events[c]});
+    ln.inflight.push_back(Segment{b, 0, m, ln.events[c], c});
     s->in_flight += m;
     return ARTEA_OK;
 }
@@ -265,6 +342,7 @@ int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue,
     for (Block& b : s->blocks) {
         if (!rc) rc = host_alloc(&b.v, size_t(budget) * max_topk);
         if (!rc) rc = host_alloc(&b.d, size_t(budget) * max_topk);
+        if (!rc) rc = host_alloc(&b.ovf, size_t(1) + artea::rk_overflow_list());
         b.t.assign(budget, 0);
         b.k.assign(budget, 0);
         s->free_blocks.push_back(&b);
@@ -420,6 +498,14 @@ int artea_server_poll_into(artea_server* s, uint64_t base_ticket, uint64_t table
     ARTEA_CHECK_ARG(table_rows > 0 && table_vids && table_dists, "artea_server_poll_into: empty table");
     ARTEA_CHECK_ARG(max_results == 0 || tickets, "artea_server_poll_into: null buffer");
     return poll_common(s, max_results, tickets, table_vids, table_dists, n_out, base_ticket, table_rows);
+}
+
+int artea_server_overflows(artea_server* s, uint64_t* rerun, uint64_t* unresolved) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s, "artea_server_overflows: null server");
+    if (rerun) *rerun = s->rerun;
+    if (unresolved) *unresolved = s->unresolved;
+    return ARTEA_OK;
 }
 
 int artea_server_status(artea_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* ready) {

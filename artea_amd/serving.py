@@ -108,6 +108,13 @@ class Server:
         check(lib().artea_server_status(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
 
+    def overflows(self):
+        """(re-run, unresolved) counts of requests whose candidate list
+        overflowed its tie tail (re-run on a larger list before returning)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        check(lib().artea_server_overflows(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def run(self, queries: np.ndarray, topk: np.ndarray, queue: np.ndarray, arrivals: np.ndarray):
         """Serve requests arriving at `arrivals` (seconds from now, sorted).
         Arrived requests are submitted, at most one lane budget per call,

@@ -146,7 +146,12 @@ int artea_router_batch_query(artea_router* r, const float* queries, uint32_t nq,
                              const float* thresholds, uint32_t* out_vids, float* out_dists);
 
 /* Same, with device buffers, enqueued on `stream` (a hipStream_t; NULL =
- * the router's own stream) and returning without a synchronisation. */
+ * the router's own stream) and returning without a synchronisation.
+ * Launches of one router (plain or batched, eager or a launch-graph replay)
+ * share its counters, so they run one after another whichever streams they
+ * are enqueued on: a launch on another stream than the router's previous
+ * launch waits for that launch first.  Launches that should overlap need a
+ * router each. */
 int artea_router_batch_query_device(artea_router* r, const float* d_queries, uint32_t nq, uint32_t flags,
                                     const float* thresholds, uint32_t* d_out_vids, float* d_out_dists,
                                     void* stream);
@@ -154,8 +159,20 @@ int artea_router_batch_query_device(artea_router* r, const float* d_queries, uin
 /* Counters of the last launch on this router (device-counted with the
  * reference profiler's definitions, router/profiler/hg_router_profiler.hpp
  * :153-197): distance evaluations (NDC, entry included), beam expansions,
- * and tie-tail overflows (0 = the candidate list held every tied evictee). */
+ * and tie-tail overflows (0 = the candidate list held every tied evictee).
+ * Waits for that launch to complete. */
 int artea_router_last_stats(artea_router* r, uint64_t* ndc, uint64_t* expansions, uint64_t* tail_overflows);
+
+/* Queries of the last launch whose candidate list overflowed its tie tail
+ * (no reference counterpart: the reference's heaps are unbounded).  Entries
+ * evicted from the top-L set stay poppable in the reference only while their
+ * distance equals the bound; the device list keeps L plus at least 16 such
+ * entries, and a query that had to drop one more may return a result that
+ * differs from the reference's.  *count = the number of such queries (0 in
+ * every workload measured); qidx receives up to min(max, 255) of their batch
+ * indices in ascending order.  artea_server_* re-runs such requests on a
+ * larger list (artea_server_overflows).  Waits for the launch. */
+int artea_router_last_overflows(artea_router* r, uint32_t max, uint32_t* qidx, uint32_t* count);
 
 /* --------------------------------------------------------- batched router */
 /* Persistent fused router: queries of mixed topk / queue size are pulled
@@ -173,6 +190,7 @@ int artea_batched_router_query_device(artea_batched_router* r, const float* d_qu
                                       void* stream);
 int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint64_t* expansions,
                                     uint64_t* tail_overflows);
+int artea_batched_router_last_overflows(artea_batched_router* r, uint32_t max, uint32_t* qidx, uint32_t* count);
 
 /* Kernel variants of the batched router (no reference counterpart).  Slots
  * 0..3 read the FP32 rows: the distance gather width (16 or 32 candidate
@@ -205,7 +223,9 @@ int artea_batched_router_get_variant(artea_batched_router* r, int* variant);
  * (counter-reset kernel + persistent router kernel, kernel nodes only) on
  * the router's stream; every replay re-runs the query on the buffers'
  * current contents and refreshes the router's last_stats.  Thresholds are
- * copied at capture time. */
+ * copied at capture time into a buffer the graph owns.  Replays are ordered
+ * with the router's other launches (see artea_router_batch_query_device);
+ * destroy the graph before its router. */
 typedef struct artea_launch_graph artea_launch_graph;
 int artea_batched_router_capture(artea_batched_router* r, const float* d_queries, uint32_t nq,
                                  const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
@@ -353,6 +373,12 @@ int artea_server_poll(artea_server* s, uint32_t max_results, uint64_t* tickets, 
 int artea_server_poll_into(artea_server* s, uint64_t base_ticket, uint64_t table_rows, uint32_t* table_vids,
                            float* table_dists, uint32_t max_results, uint64_t* tickets, uint32_t* n_out);
 int artea_server_status(artea_server* s, uint64_t* queued, uint64_t* in_flight, uint64_t* ready);
+/* Requests whose candidate list overflowed its tie tail
+ * (artea_router_last_overflows) are run again, before they are returned, on
+ * the next larger list class until none overflows: *rerun counts them;
+ * *unresolved counts those that still overflowed in the largest class
+ * (1024 entries) and were returned as computed. */
+int artea_server_overflows(artea_server* s, uint64_t* rerun, uint64_t* unresolved);
 
 /* Device plumbing for callers without a GPU framework of their own: plain
  * allocations, copies, streams (default priority) and timing events of the
