@@ -47,6 +47,7 @@ SIGNATURES = {
     "artea_router_last_stats": (i32, [P, P, P, P]),
     "artea_router_last_overflows": (i32, [P, u32, P, C.POINTER(u32)]),
     "artea_batched_router_last_overflows": (i32, [P, u32, P, C.POINTER(u32)]),
+    "artea_batched_query_noalloc": (i32, [P, u32, u32, P, u32, P, P, u32, P, P, P, P]),
     "artea_batched_router_create": (i32, [P, u32, u32, PP]),
     "artea_batched_router_destroy": (i32, [P]),
     "artea_batched_router_query_device": (i32, [P, P, u32, P, P, u32, P, P, P, P]),

@@ -192,6 +192,23 @@ int artea_batched_router_last_stats(artea_batched_router* r, uint64_t* ndc, uint
                                     uint64_t* tail_overflows);
 int artea_batched_router_last_overflows(artea_batched_router* r, uint32_t max, uint32_t* qidx, uint32_t* count);
 
+/* Allocation-free form of artea_batched_router_query_device.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It takes the index and the launch's
+ * maxima instead of a router handle (max_topk <= max_queue <= 1008, as
+ * artea_batched_router_create), runs the built-in default kernel variant,
+ * and keeps the launch's work counter, counters and early-stop thresholds
+ * (one per level, at most 64 levels) in a fixed 2 KiB device array of the
+ * library, one per GPU, held until the process ends: it calls no hipMalloc,
+ * hipFree or other allocating routine.  The caller passes the inputs and
+ * outputs as for artea_batched_router_query_device; the results are the
+ * same.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU race: enqueue them on one stream, or order them. */
+int artea_batched_query_noalloc(artea_index* idx, uint32_t max_topk, uint32_t max_queue, const float* d_queries,
+                                uint32_t nq, const uint32_t* d_topk, const uint32_t* d_queue, uint32_t flags,
+                                const float* thresholds, uint32_t* d_out_vids, float* d_out_dists, void* stream);
+
 /* Kernel variants of the batched router (no reference counterpart).  Slots
  * 0..3 read the FP32 rows: the distance gather width (16 or 32 candidate
  * rows per pass) and the register budget the kernel is compiled for (no
