@@ -31,6 +31,25 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_numa_policy():
+    import glob
+    import shutil
+
+    nodes = len(glob.glob("/sys/devices/system/node/node[0-9]*"))
+    tool = "numactl present but not used" if shutil.which("numactl") else "numactl not installed"
+    return "%d NUMA node(s); default local (first-touch) allocation, no interleave (%s)" % (max(nodes, 1), tool)
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -137,6 +156,7 @@ def workload_cosine(args):
         cpu_qps = sample.shape[0] * reps / (time.time() - t0)
         same = bool(np.array_equal(cv, gv) and np.array_equal(cd, gd))
         out["cpu_baseline"] = {"value": round(cpu_qps, 1), "unit": "queries/s", "cores": cores, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
                                "sample": "1000 queries, top-10 at queue %d, %d repetitions, oracle restatement of "
                                          "batch_query on the GPU-built graph" % (L10, reps),
                                "results_equal_gpu": same}
@@ -232,6 +252,7 @@ def workload_c0(args):
             "%s" % (cores, cpu_build, hgra_same, cpu_qps, same_cpu))
         out["cpu_baseline"] = {"build_s": round(cpu_build, 2), "value": round(cpu_qps, 1), "unit": "queries/s",
                                "cores": cores, "kind": "port", "hgra_identical_to_gpu": hgra_same,
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
                                "results_equal_gpu": same_cpu,
                                "sample": "the whole C0 workload: oracle build_full and batch_query, OpenMP"}
     print(json.dumps(out))
@@ -286,11 +307,11 @@ def workload_build(args):
     gt_s = time.time() - t0
     log("ground truth (BF16 MFMA filter + FP64 recheck) in %.2fs" % gt_s)
     rows, best = sweep_queues(aa, idx, q_d, nq, gt_i, 100, [100, 200, 400, 800], stream, args.target_recall)
-    out = {"metric": "10M x 96 propagate build seconds (layers 0-1), BF16 vs FP32 join; serving p50/p99 ms",
+    out = {"metric": "10M x 96 propagate build seconds (every level), BF16 vs FP32 join; serving p50/p99 ms",
            "value": builds["bf16"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
            "dtype": "bf16 MFMA join tiles (FP32 accumulate, FP32 rechecks) / f32",
            "data": "synthetic (1024-component mixture, unit-norm)",
-           "config": {"workload": "layers 0-1 build + serving, %d x 96 L2" % n, "n": n, "nq": nq},
+           "config": {"workload": "hierarchy build (every level) + serving, %d x 96 L2" % n, "n": n, "nq": nq},
            "builds": builds, "bf16_speedup": round(builds["fp32"]["seconds"] / builds["bf16"]["seconds"], 3),
            "edge_sets_identical": same, "top": idx.info()["top"], "level_fraction": args.level_fraction,
            "ground_truth_s": round(gt_s, 2), "search_sweep": rows, "search_chosen": best}
@@ -387,6 +408,7 @@ def workload_build(args):
             g.snapshot(b)
             hgra_same = open(a, "rb").read() == open(b, "rb").read()
         out["cpu_baseline"] = {"value": round(cpu_s, 2), "unit": "s", "cores": cores, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
                                "sample": "whole build of the first %d vectors (FP32 join), oracle build_full with "
                                          "OpenMP" % m,
                                "gpu_seconds_same_sample": round(gpu_s, 2), "hgra_identical_to_gpu": hgra_same}
@@ -404,7 +426,7 @@ def main():
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
     ap.add_argument("--workload", choices=("search", "cosine", "build", "c0"), default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
-                         "build: 10M x 96 layers 0-1 build FP32 vs BF16 join, then serving on that index; "
+                         "build: 10M x 96 hierarchy build (every level) FP32 vs BF16 join, then serving on that index; "
                          "c0: 100K x 128 build and search, GPU vs the CPU oracle")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
@@ -605,24 +627,39 @@ def main():
 
         lanes.append((s_j, run_b, run_p, ov_j, od_j, br_j))
 
-    def time_lanes(fns, reps):
+    def time_lanes(fns, reps, launch_ms=None):
+        """Seconds per step over `reps` steps going round the lanes.  With
+        launch_ms, every launch of the timed region is bracketed by events on
+        its own stream and their durations are appended (the kernel's
+        average launch duration for the roofline, overlap included)."""
         for fn in fns:
             for _ in range(args.warmup):
                 fn()
         for ln in lanes:
             ln[0].synchronize()
+        evs = [(Event(), Event()) for _ in range(reps)] if launch_ms is not None else None
         t0 = time.perf_counter()
         for i in range(reps):
-            fns[i % len(fns)]()
+            j = i % len(fns)
+            if evs:
+                evs[i][0].record(lanes[j][0])
+            fns[j]()
+            if evs:
+                evs[i][1].record(lanes[j][0])
         for ln in lanes:
             ln[0].synchronize()
-        return (time.perf_counter() - t0) / reps
+        dt = (time.perf_counter() - t0) / reps
+        if evs:
+            launch_ms.extend(e0.elapsed_ms(e1) for e0, e1 in evs)
+        return dt
 
     def written_equal(a, b):
         # A top-k query writes the first k slots of its row; the rest are untouched.
         return bool(np.array_equal(a[sel100, :100], b[sel100, :100]) and np.array_equal(a[sel10, :10], b[sel10, :10]))
 
-    t_step = time_lanes([ln[1] for ln in lanes], args.steps)
+    launch_ms = []
+    t_step = time_lanes([ln[1] for ln in lanes], args.steps, launch_ms)
+    t_launch = float(np.mean(launch_ms)) / 1e3
     t_plain = time_lanes([ln[2] for ln in lanes], args.steps)
     lanes_equal = all(written_equal(ln[3].numpy(ln[0]), pred) and
                       written_equal(ln[4].numpy(ln[0]).view(np.uint32), pdist.view(np.uint32)) for ln in lanes)
@@ -644,13 +681,17 @@ def main():
         for _ in range(args.warmup):
             f32_fns[0]()
         t_f32_batch = time_it(f32_fns[0], args.steps)
-        t_f32 = time_lanes(f32_fns, args.steps)
+        st_f32 = br.last_stats()  # the FP32 kernel's own counters (its visited cache differs)
+        f32_launch_ms = []
+        t_f32 = time_lanes(f32_fns, args.steps, f32_launch_ms)
         same_f32 = all(written_equal(ln[3].numpy(ln[0]), pred) and
                        written_equal(ln[4].numpy(ln[0]).view(np.uint32), pdist.view(np.uint32)) for ln in lanes)
         for ln in lanes:
             ln[5].set_variant(slot)
         f32_rows = {"qps": round(nq / t_f32, 1), "ms_per_step": round(t_f32 * 1e3, 4),
                     "batch_ms": round(t_f32_batch * 1e3, 4),
+                    "launch_ms": round(float(np.mean(f32_launch_ms)), 4),
+                    "ndc_per_query": round(st_f32["ndc"] / nq, 1),
                     "variant": br.variant_names[f32_slot], "results_equal": same_f32}
         log("FP32 rows: %.0f q/s (%.3f ms/step; one batch alone %.3f ms, %s), results equal %s" % (
             nq / t_f32, t_f32 * 1e3, t_f32_batch * 1e3, br.variant_names[f32_slot], same_f32))
@@ -679,10 +720,14 @@ def main():
     row_bytes = 4 * 64
     vec_bytes = D if u8 else 4 * D
     alg_bytes = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * vec_bytes + st["expansions"] * row_bytes
-    achieved = alg_bytes / t_batch / 1e9
+    # the roofline's time: the kernel's average launch duration over the
+    # timed steps (two launches overlap at a time); alone = one batch alone
+    achieved = alg_bytes / t_launch / 1e9
     if f32_rows is not None:
-        fb = nq * 4 * D + 8 * int(topk_q.sum()) + st["ndc"] * 4 * D + st["expansions"] * row_bytes
-        f32_rows["roofline_frac"] = round(fb / (f32_rows["batch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        fb = nq * 4 * D + 8 * int(topk_q.sum()) + st_f32["ndc"] * 4 * D + st_f32["expansions"] * row_bytes
+        f32_rows["algorithmic_bytes_per_launch"] = int(fb)
+        f32_rows["roofline_frac"] = round(fb / (f32_rows["launch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+        f32_rows["roofline_frac_alone"] = round(fb / (f32_rows["batch_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (calibrated on a copy kernel), committed under profiles/.
     def measured_traffic(rows):
@@ -697,6 +742,11 @@ def main():
     traffic = measured_traffic("u8" if u8 else "f32")
     if f32_rows is not None:
         f32_rows["traffic"] = measured_traffic("f32")
+        if f32_rows["traffic"]:
+            # counter-based: bytes the kernel fetched from and wrote to memory
+            # beyond L2 (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)
+            f32_rows["traffic_frac"] = round(f32_rows["traffic"] / (f32_rows["launch_ms"] / 1e3) / 1e9 /
+                                             HBM_PEAK_GBS, 4)
     log("roofline: %.1f GB algorithmic per launch, %.1f GB/s (%.3f of peak)" % (alg_bytes / 1e9, achieved,
                                                                           achieved / HBM_PEAK_GBS))
 
@@ -711,22 +761,33 @@ def main():
             idx.snapshot_hgra(p)
             g = orc.Graph.restore(p)
         cores = len(os.sched_getaffinity(0))
-        sample = queries[sel100[:1000]]
+        # the first 1000 queries of the batch with their own topk (the same
+        # mix of 10 and 100), one oracle batch_query per topk
+        smp = np.arange(min(1000, nq))
+        s10, s100 = smp[topk_q[smp] == 10], smp[topk_q[smp] == 100]
         t0 = time.time()
         reps = 0
-        cv = None
+        cv10 = cv100 = None
+        ndcs = []
         while time.time() - t0 < args.cpu_seconds or reps == 0:
-            cv, cd, cndc = orc.batch_query(g, base, orc.L2SQR, sample, 100, chosen, nthreads=cores)
+            cv10, _, n10 = orc.batch_query(g, base, orc.L2SQR, queries[s10], 10, chosen, nthreads=cores)
+            cv100, _, n100 = orc.batch_query(g, base, orc.L2SQR, queries[s100], 100, chosen, nthreads=cores)
+            ndcs = [n10, n100]
             reps += 1
         cpu_t = (time.time() - t0) / reps
-        # parity of the sample against the batched GPU router (top-100 queries)
-        match = float(np.mean(cv == pred[sel100[:1000], :100]))
-        cpu = {"value": round(sample.shape[0] / cpu_t, 1), "unit": "queries/s", "cores": cores, "kind": "port",
-               "sample": "%d top-100 queries of the same batch at queue %d, %d repetitions (%.1fs), oracle "
-                         "restatement of HierarchicalGraphRouter::batch_query with OpenMP over queries" % (
-                             sample.shape[0], chosen, reps, time.time() - t0),
-               "slot_match_vs_gpu": round(match, 6), "ndc_per_query": round(float(cndc.mean()), 1)}
-        log("cpu baseline %.0f q/s on %d cores, GPU slot match %.6f" % (cpu["value"], cores, match))
+        # parity of the sample against the batched GPU router, slot for slot
+        match = float((np.sum(cv10 == pred[s10, :10]) + np.sum(cv100 == pred[s100, :100])) /
+                      (cv10.size + cv100.size))
+        cpu = {"value": round(smp.size / cpu_t, 1), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": "the first %d queries of the same batch with their own topk (%d top-10, %d top-100) at "
+                         "queue %d, %d repetitions (%.1fs), oracle restatement of "
+                         "HierarchicalGraphRouter::batch_query with OpenMP over queries" % (
+                             smp.size, s10.size, s100.size, chosen, reps, time.time() - t0),
+               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
+               "slot_match_vs_gpu": round(match, 6),
+               "ndc_per_query": round(float(np.concatenate(ndcs).mean()), 1)}
+        log("cpu baseline %.0f q/s on %d cores (%s), GPU slot match %.6f" % (cpu["value"], cores, cpu["cpu_model"],
+                                                                           match))
 
     serving = None
     if args.serve_requests > 0:
@@ -795,8 +856,13 @@ def main():
         "distance_max_rel_err_vs_fp64": dist_err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_frac": round(traffic / t_launch / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                      "kernel": "router_persistent_kernel (batched router), one launch per step",
-                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_batch * 1e3, 4),
+                     "algorithmic_bytes_per_launch": int(alg_bytes), "launch_ms": round(t_launch * 1e3, 4),
+                     "launch_ms_source": "HIP events around each launch of the timed steps on its own stream "
+                                         "(%d launches, %d streams)" % (len(launch_ms), len(lanes)),
+                     "frac_alone": round(alg_bytes / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+                     "frac_per_step": round(alg_bytes / t_step / 1e9 / HBM_PEAK_GBS, 4),
                      "note": ("one-byte rows: a quarter of the FP32 rows' bytes per distance, so the kernel is bound "
                               "by its dependent row-id -> vector -> merge chain and instruction issue, not HBM; the "
                               "FP32-row kernel on the same batch is HBM-bound (f32_rows.roofline_frac)")
