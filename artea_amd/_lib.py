@@ -92,6 +92,8 @@ SIGNATURES = {
     "artea_refine_download_rows": (i32, [P, P, P]),
     "artea_refine_prefill": (i32, [P, u32, u32, u32, u32, u32, u64]),
     "artea_refine_triangle": (i32, [P, f32, f32]),
+    "artea_refine_triangle_range": (i32, [P, f32, f32, u32, u32]),
+    "artea_refine_reverse_range": (i32, [P, u32, u32]),
     "artea_refine_truncate": (i32, [P]),
     "artea_refine_reverse": (i32, [P]),
     "artea_refine_schedule": (i32, [P, u32, u32, f32, f32]),

@@ -55,6 +55,7 @@ struct Ctx {
     uint2* seg;           // n * cap
     unsigned long long* stats;  // [0] logs, [1] FP32 pair distances, [2] BF16 pair distances
     uint32_t list_cap;          // triangle recheck queue entries per row (<= kListCap)
+    uint32_t rb, re;            // rows the triangle / reverse producers run on (next_range); [0, n) otherwise
 };
 
 __device__ __forceinline__ uint32_t storage_of(const Ctx& c, uint32_t local) { return c.l2g ? c.l2g[local] : local; }
@@ -139,7 +140,7 @@ __global__ void reverse_kernel(Ctx c) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(c.n) * c.cap) return;
     const uint32_t row = uint32_t(i / c.cap), k = uint32_t(i % c.cap);
-    const uint32_t nr = c.cnt[row];
+    const uint32_t nr = (row >= c.rb && row < c.re) ? c.cnt[row] : 0u;  // outside the window: no logs
     if (k == 0) c.e_cnt[row] = nr;
     if (k >= nr) return;
     const uint2 e = c.rows[i];
@@ -419,13 +420,13 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
     uint32_t n_pf = 0;
     uint2 e_pf = make_uint2(0u, 0u);
     auto prefetch = [&](uint32_t r) {
-        if (r < c.n) {
+        if (r < c.re) {
             n_pf = c.cnt[r];
             if (tid < c.cap) e_pf = c.rows[size_t(r) * c.cap + tid];
         }
     };
-    prefetch(blockIdx.x);
-    for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
+    prefetch(c.rb + blockIdx.x);
+    for (uint32_t row = c.rb + blockIdx.x; row < c.re; row += gridDim.x) {
         const uint32_t n = n_pf;
         const uint2 e_row = e_pf;
         prefetch(row + gridDim.x);
@@ -1041,8 +1042,8 @@ void refine_free(artea_refine* r) {
     if (r->s) (void)hipStreamDestroy(r->s);
 }
 
-uint32_t grid_for(const artea_refine* r, uint32_t rows_per_block, uint32_t blocks_per_cu) {
-    const uint32_t want = (r->c.n + rows_per_block - 1) / rows_per_block;
+uint32_t grid_for(const artea_refine* r, uint32_t rows_per_block, uint32_t blocks_per_cu, uint32_t rows = ~0u) {
+    const uint32_t want = (std::min(rows, r->c.n) + rows_per_block - 1) / rows_per_block;
     return std::max(1u, std::min(want, uint32_t(r->dev_cus) * blocks_per_cu));
 }
 
@@ -1092,8 +1093,11 @@ JoinBounds join_bounds(uint32_t D) {
     return b;
 }
 
-int do_triangle(artea_refine* r, float scale, float shift) {
-    Ctx& c = r->c;
+int do_triangle(artea_refine* r, float scale, float shift, uint32_t rb = 0, uint32_t re = ~0u) {
+    Ctx c = r->c;
+    c.re = std::min(re, c.n);
+    c.rb = std::min(rb, c.re);
+    if (c.rb > 0 || c.re < c.n) ARTEA_HIP_TRY(hipMemsetAsync(c.e_cnt, 0, sizeof(uint32_t) * c.n, r->s));
     const float inv = 1.0f / scale;
     const float shift_term = (1.0f + inv) * shift;
     int dev = 0, max_lds = 65536;
@@ -1109,7 +1113,7 @@ int do_triangle(artea_refine* r, float scale, float shift) {
     ARTEA_CHECK_ARG(lds <= size_t(max_lds), "triangle: row capacity too large for LDS");
     const JoinBounds kb = join_bounds(c.D);
     const uint32_t per_cu = std::max<uint32_t>(1u, uint32_t(160u * 1024u / std::max<size_t>(lds, 1)));
-    const dim3 grid(grid_for(r, 1, std::min<uint32_t>(per_cu, 4u)));
+    const dim3 grid(grid_for(r, 1, std::min<uint32_t>(per_cu, 4u), std::max(c.re - c.rb, 1u)));
     auto launch = [&](auto kernel) -> int {
         const void* fn = reinterpret_cast<const void*>(kernel);
         if (lds > 65536) ARTEA_HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
@@ -1132,8 +1136,10 @@ int do_triangle(artea_refine* r, float scale, float shift) {
     return merge_logs(r);
 }
 
-int do_reverse(artea_refine* r) {
-    Ctx& c = r->c;
+int do_reverse(artea_refine* r, uint32_t rb = 0, uint32_t re = ~0u) {
+    Ctx c = r->c;
+    c.re = std::min(re, c.n);
+    c.rb = std::min(rb, c.re);
     const size_t slots = size_t(c.n) * c.cap;
     hipLaunchKernelGGL(reverse_kernel, dim3(uint32_t((slots + 255) / 256)), dim3(256), 0, r->s, c);
     ARTEA_HIP_TRY(hipGetLastError());
@@ -1171,6 +1177,8 @@ int refine_make(const float* d_vecs, uint32_t n_global, uint32_t dim_pad, int me
     c.metric = metric;
     c.cap = cap;
     c.n = l2g ? n_local : n_global;
+    c.rb = 0;
+    c.re = c.n;
     c.list_cap = kListCap;
     r->seed = seed;
     const size_t slots = std::max<size_t>(size_t(c.n) * cap, 1);
@@ -1351,6 +1359,24 @@ int artea_refine_triangle(artea_refine* r, float scale, float shift) {
     clear_error();
     ARTEA_CHECK_ARG(r, "artea_refine_triangle: null handle");
     int rc = do_triangle(r, scale, shift);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_triangle_range(artea_refine* r, float scale, float shift, uint32_t row_begin, uint32_t row_end) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_triangle_range: null handle");
+    ARTEA_CHECK_ARG(row_begin <= row_end, "artea_refine_triangle_range: row_begin > row_end");
+    int rc = do_triangle(r, scale, shift, row_begin, row_end);
+    if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return rc;
+}
+
+int artea_refine_reverse_range(artea_refine* r, uint32_t row_begin, uint32_t row_end) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_reverse_range: null handle");
+    ARTEA_CHECK_ARG(row_begin <= row_end, "artea_refine_reverse_range: row_begin > row_end");
+    int rc = do_reverse(r, row_begin, row_end);
     if (rc == ARTEA_OK) ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
     return rc;
 }

@@ -74,6 +74,12 @@ class PropagateEngine:
     def next_triangle(self, scale_coeffs, shifted_coeffs):
         check(lib().artea_refine_triangle(self.h, scale_coeffs, shifted_coeffs))
 
+    def next_triangle_range(self, scale_coeffs, shifted_coeffs, row_begin, row_end):
+        check(lib().artea_refine_triangle_range(self.h, scale_coeffs, shifted_coeffs, row_begin, row_end))
+
+    def next_reverse_range(self, row_begin, row_end):
+        check(lib().artea_refine_reverse_range(self.h, row_begin, row_end))
+
     def next_truncate(self):
         check(lib().artea_refine_truncate(self.h))
 

@@ -281,6 +281,13 @@ int artea_refine_prefill(artea_refine* r, uint32_t gen_size, uint32_t start, uin
  * next() keeps at most cap entries per row, which equals the reference's
  * merge followed by the arc updater that refine_layer always runs next. */
 int artea_refine_triangle(artea_refine* r, float scale_coeffs, float shifted_coeffs);
+/* next_range(TriangleUpdater / ReverseUpdater, row_begin, row_end)
+ * (refiner/propagate_engine.hpp:217-225): the updater runs on the rows
+ * [row_begin, row_end) only (clamped to the row count); the logs it emits
+ * are merged into every row. */
+int artea_refine_triangle_range(artea_refine* r, float scale_coeffs, float shifted_coeffs, uint32_t row_begin,
+                                uint32_t row_end);
+int artea_refine_reverse_range(artea_refine* r, uint32_t row_begin, uint32_t row_end);
 /* next(arc_updater_t) (refiner/updaters/truncate_updater.hpp:58-76). */
 int artea_refine_truncate(artea_refine* r);
 /* next(ReverseUpdater) (refiner/updaters/reverse_updater.hpp:72-100). */

@@ -1150,12 +1150,17 @@ void orc_next_prefill(orc_layer* L, uint32_t gen_size, uint32_t start, uint32_t 
 
 /* TriangleUpdater::update_impl / _internal_check
  * (refiner/updaters/triangle_updater.hpp:114-157, 200-237). */
-void orc_next_triangle(orc_layer* L, float scale_coeffs, float shifted_coeffs, int nthreads) {
+/* next_range(TriangleUpdater, row_begin, row_end)
+ * (refiner/propagate_engine.hpp:217-225): the updater runs on the rows of
+ * the window, the logs are merged into every row. */
+void orc_next_triangle_range(orc_layer* L, float scale_coeffs, float shifted_coeffs, uint32_t row_begin,
+                             uint32_t row_end, int nthreads) {
     const float inv = 1.0f / scale_coeffs;
     const float shift_term = (1.0f + inv) * shifted_coeffs;
     uint64_t pairs = 0;
+    const int64_t re = row_end < L->n_local ? (int64_t)row_end : (int64_t)L->n_local;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : pairs) num_threads(pick_threads(nthreads))
-    for (int64_t ri = 0; ri < (int64_t)L->n_local; ++ri) {
+    for (int64_t ri = (int64_t)row_begin; ri < re; ++ri) {
         nvec* row = &L->rows[ri];
         if (row->n == 0) continue;
         orc_nbr* kept = (orc_nbr*)malloc(sizeof(orc_nbr) * row->n);
@@ -1198,6 +1203,10 @@ void orc_next_triangle(orc_layer* L, float scale_coeffs, float shifted_coeffs, i
     merge_logs(L, nthreads);
 }
 
+void orc_next_triangle(orc_layer* L, float scale_coeffs, float shifted_coeffs, int nthreads) {
+    orc_next_triangle_range(L, scale_coeffs, shifted_coeffs, 0, L->n_local, nthreads);
+}
+
 /* TruncateUpdater (refiner/updaters/truncate_updater.hpp:58-76) with the
  * default max_distance cutoff, i.e. resize(min(size, cap)). */
 void orc_next_truncate(orc_layer* L, int nthreads) {
@@ -1208,9 +1217,13 @@ void orc_next_truncate(orc_layer* L, int nthreads) {
 }
 
 /* ReverseUpdater::update_impl (refiner/updaters/reverse_updater.hpp:72-100). */
-void orc_next_reverse(orc_layer* L, int nthreads) {
+void orc_next_reverse(orc_layer* L, int nthreads) { orc_next_reverse_range(L, 0, L->n_local, nthreads); }
+
+/* next_range(ReverseUpdater, row_begin, row_end). */
+void orc_next_reverse_range(orc_layer* L, uint32_t row_begin, uint32_t row_end, int nthreads) {
+    const int64_t re = row_end < L->n_local ? (int64_t)row_end : (int64_t)L->n_local;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(pick_threads(nthreads))
-    for (int64_t ri = 0; ri < (int64_t)L->n_local; ++ri) {
+    for (int64_t ri = (int64_t)row_begin; ri < re; ++ri) {
         const nvec* row = &L->rows[ri];
         const uint32_t sv = storage_vid(L, (uint32_t)ri);
         for (uint32_t i = 0; i < row->n; ++i) {

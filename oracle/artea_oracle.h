@@ -167,6 +167,11 @@ void orc_next_prefill(orc_layer* L, uint32_t gen_size, uint32_t start_vid, uint3
 void orc_next_triangle(orc_layer* L, float scale_coeffs, float shifted_coeffs, int nthreads);
 void orc_next_truncate(orc_layer* L, int nthreads);
 void orc_next_reverse(orc_layer* L, int nthreads);
+/* next_range(...) (refiner/propagate_engine.hpp:217-225): the updater on
+ * rows [row_begin, row_end) only, the logs merged into every row. */
+void orc_next_triangle_range(orc_layer* L, float scale_coeffs, float shifted_coeffs, uint32_t row_begin,
+                             uint32_t row_end, int nthreads);
+void orc_next_reverse_range(orc_layer* L, uint32_t row_begin, uint32_t row_end, int nthreads);
 /* The artea_graph::IndexFactory::refine_layer schedule
  * (index/artea_graph/index_factory.hpp:344-393): optional L0 prefill, then
  * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */

@@ -93,6 +93,8 @@ def _declare(l):
     d("orc_next_triangle", None, P, f32, f32, i32)
     d("orc_next_truncate", None, P, i32)
     d("orc_next_reverse", None, P, i32)
+    d("orc_next_triangle_range", None, P, f32, f32, u32, u32, i32)
+    d("orc_next_reverse_range", None, P, u32, u32, i32)
     d("orc_refine_schedule", None, P, u32, u32, f32, f32, i32)
     d("orc_layer_last_logs", u64, P)
     d("orc_layer_pair_distances", u64, P)
@@ -387,6 +389,12 @@ class Layer:
 
     def reverse(self, nthreads=0):
         lib().orc_next_reverse(self.h, nthreads)
+
+    def triangle_range(self, scale, shift, row_begin, row_end, nthreads=0):
+        lib().orc_next_triangle_range(self.h, scale, shift, row_begin, row_end, nthreads)
+
+    def reverse_range(self, row_begin, row_end, nthreads=0):
+        lib().orc_next_reverse_range(self.h, row_begin, row_end, nthreads)
 
     def schedule(self, build_loops, triu_iters, scale, shift, nthreads=0):
         lib().orc_refine_schedule(self.h, build_loops, triu_iters, scale, shift, nthreads)

@@ -169,3 +169,34 @@ def test_full_build_parity(orc, join, fraction):
         idx.snapshot_hgra(a)
         g.snapshot(b)
         assert open(a, "rb").read() == open(b, "rb").read()
+
+
+@pytest.mark.parametrize("join", [JOIN_FP32, JOIN_BF16])
+@pytest.mark.parametrize("sparse", [False, True])
+def test_row_window_passes_equal_oracle(orc, join, sparse):
+    """next_range with the triangle and reverse updaters
+    (refiner/propagate_engine.hpp:217-225): the updater runs on a window of
+    rows, the logs merge into every row; identical to the oracle's followed
+    by the arc updater (the device merge keeps at most cap entries per row,
+    which refine_layer's arc pass after every merge does in the reference)."""
+    n = 2400
+    vecs = aa.generate(2, 19, 0, n, 96)
+    idx = aa.DeviceIndex(vecs, 3)
+    l2g = np.arange(0, n, 3, dtype=np.uint32) if sparse else None
+    nl = n if l2g is None else len(l2g)
+    eng = PropagateEngine(idx, cap=96, seed=5, join=join, local_to_global=l2g)
+    L = orc.Layer(vecs, 3, 96, local_to_global=l2g)
+    eng.next_prefill(38, 0, nl, stream=0)
+    L.prefill(38, 0, nl, seed=5, stream=0, nthreads=8)
+    assert rows_of(eng) == orc_rows(L)
+    for rb, re in ((nl // 5, nl // 2), (0, nl // 3), (nl - 7, nl + 50)):
+        eng.next_triangle_range(1.10, 1.5, rb, re)
+        L.triangle_range(1.10, 1.5, rb, re, nthreads=8)
+        L.truncate(nthreads=8)
+        assert rows_of(eng) == orc_rows(L), ("triangle", rb, re)
+        eng.next_reverse_range(rb, re)
+        L.reverse_range(rb, re, nthreads=8)
+        L.truncate(nthreads=8)
+        assert rows_of(eng) == orc_rows(L), ("reverse", rb, re)
+    eng.next_triangle_range(1.10, 1.5, 5, 5)  # empty window: only a merge of nothing
+    assert rows_of(eng) == orc_rows(L)

@@ -538,3 +538,42 @@ def test_geometric_levels_and_compaction_trim(orc):
             else:
                 assert ids.size > 0 and np.all(ids < 3000) and v not in ids
                 assert np.all(hl[ids] >= lvl)  # neighbours live on the same level
+
+
+def test_oracle_row_window_passes(orc):
+    """next_range (refiner/propagate_engine.hpp:217-225) in the oracle: the
+    full window equals next(); a window that leaves out the only populated
+    row changes nothing; a window holding it equals next()."""
+    import artea_amd as aa
+
+    vecs = aa.generate(0, 3, 0, 400, 128)
+
+    def layer_after(fn):
+        L = orc.Layer(vecs, orc.L2, 16)
+        L.prefill(6, 0, 400, seed=1, stream=0, nthreads=2)
+        fn(L)
+        return [L.row(i).tolist() for i in range(400)]
+
+    full = layer_after(lambda L: L.triangle(1.1, 0.5, nthreads=2))
+    assert layer_after(lambda L: L.triangle_range(1.1, 0.5, 0, 400, nthreads=2)) == full
+    assert layer_after(lambda L: L.triangle_range(1.1, 0.5, 0, 10 ** 6, nthreads=2)) == full
+    base = layer_after(lambda L: None)
+    assert layer_after(lambda L: L.triangle_range(1.1, 0.5, 7, 7, nthreads=2)) == base
+    assert layer_after(lambda L: L.reverse_range(0, 0, nthreads=2)) == base
+    # reverse of rows [0, 200) only: every row >= 200 keeps its prefill
+    # entries, and an edge u -> v with u >= 200 is reversed only if some row
+    # below 200 also lists it
+    rev = layer_after(lambda L: L.reverse_range(0, 200, nthreads=2))
+    allrev = layer_after(lambda L: L.reverse(nthreads=2))
+    assert rev != allrev and rev != base
+    for u in range(200):
+        for e in base[u]:
+            v = e[0] & 0x7FFFFFFF
+            assert any((x[0] & 0x7FFFFFFF) == u for x in rev[v]) or len(rev[v]) == 16
+
+    # one populated row: a window without it leaves the layer as it was
+    L1 = orc.Layer(vecs, orc.L2, 16)
+    L1.set_row_raw(5, orc.nbrs([(u, orc.distance(orc.L2, vecs[5], vecs[u])) for u in (9, 40, 77, 123)], new=True))
+    before = [L1.row(i).tolist() for i in range(400)]
+    L1.triangle_range(1.0, 0.0, 6, 400, nthreads=2)
+    assert [L1.row(i).tolist() for i in range(400)] == before
