@@ -169,6 +169,32 @@ class DeviceIndex:
     def snapshot_hgra(self, path: str):
         check(lib().artea_index_snapshot_hgra(self.h, path.encode()))
 
+    def compact_graph(self, highest, levels, ul_max: int, bl_max: int, order=None, metric: Optional[int] = None):
+        """HierarchicalGraphCompactor::compact_graph on the device
+        (compactor/hierarchical_graph_compactor.hpp:118-327): ``highest``
+        per vid (INVALID_LEVEL = unassigned), ``levels[l][v]`` the rows of
+        level l, ``order`` the order vertices were assigned to their buckets
+        (None = ascending vid).  Attaches the compact graph to the index."""
+        n = self.n
+        hl = np.ascontiguousarray(highest, np.uint32)
+        offs, ids = [], []
+        for rows in levels:
+            cnt = np.array([len(rows[v]) if v < len(rows) and rows[v] is not None else 0 for v in range(n)], np.uint64)
+            off = np.zeros(n + 1, np.uint64)
+            np.cumsum(cnt, out=off[1:])
+            flat = np.zeros(max(int(off[-1]), 1), np.uint32)
+            for v in range(n):
+                if cnt[v]:
+                    flat[off[v]:off[v + 1]] = rows[v]
+            offs.append(off)
+            ids.append(flat)
+        op = (C.c_void_p * max(len(offs), 1))(*[o.ctypes.data for o in offs])
+        ip = (C.c_void_p * max(len(ids), 1))(*[i.ctypes.data for i in ids])
+        od = None if order is None else np.ascontiguousarray(order, np.uint32)
+        check(lib().artea_index_compact_graph(self.h, len(levels) - 1, ul_max, bl_max, _ptr(hl), op, ip,
+                                              _ptr(od) if od is not None else None,
+                                              self.metric if metric is None else metric))
+
     def info(self):
         vals = [C.c_uint32() for _ in range(6)]
         check(lib().artea_index_info(self.h, *[C.byref(x) for x in vals]))

@@ -101,6 +101,7 @@ SIGNATURES = {
     "artea_refine_set_recheck_queue": (i32, [P, u32]),
     "artea_refine_last_stats": (i32, [P, P, P, P]),
     "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, P]),
+    "artea_index_compact_graph": (i32, [P, u32, u32, u32, P, P, P, P, i32]),
 }
 
 STATUS = {0: "ARTEA_OK", 1: "ARTEA_EINVAL", 2: "ARTEA_EIO", 3: "ARTEA_EHIP", 4: "ARTEA_ENOMEM", 5: "ARTEA_EOVERFLOW"}

@@ -1276,6 +1276,140 @@ __global__ void centroid_final_kernel(const double* part, uint32_t blocks, uint3
     }
 }
 
+
+// HierarchicalGraphCompactor::compact_graph
+// (compactor/hierarchical_graph_compactor.hpp:118-327) of a dynamic graph
+// given per vertex its source level (kInvalidLevel = unassigned) and per
+// level l its rows (row_off[l]: n + 1 offsets into row_ids[l]), attached to
+// idx.  Step 1: the first bucket from the top with >= 32 vertices
+// (min_layer_cap, framework/type_traits/index_traits.hpp:79) is the new top.
+// Step 2: buckets keep the order vertices were assigned (`order`, null =
+// ascending vid); the top bucket holds its own vertices, then the demoted
+// ones by ascending source level, each keeping its rows at levels <= top.
+// Groups are laid out h = 0..top in the neighbour array, a vertex's rows from
+// its highest level down.  Entry: the top-bucket vertex closest, by
+// `metric`, to the FP64 centroid of the top bucket, computed on the GPU;
+// ties keep the earliest bucket position.
+int compact_and_attach(artea_index* idx, const uint32_t* src_h, const uint32_t* order,
+                       const uint64_t* const* row_off, const uint32_t* const* row_ids, uint32_t ul_max,
+                       uint32_t bl_max, int metric) {
+    const uint32_t n = idx->n;
+    uint32_t src_top = kInvalidLevel;
+    for (uint32_t v = 0; v < n; ++v)
+        if (src_h[v] != kInvalidLevel && (src_top == kInvalidLevel || src_h[v] > src_top)) src_top = src_h[v];
+    HostCsr g;
+    g.n = n;
+    g.ul_max = ul_max;
+    g.bl_max = bl_max;
+    g.highest.assign(n, kInvalidLevel);
+    g.local.assign(n, kNoVertex);
+    if (src_top == kInvalidLevel) {  // no assigned vertex: an empty graph, no entry
+        g.top = 0;
+        g.offsets.assign(1, std::vector<uint64_t>(1, 0));
+        std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2), kInvalidLevel);
+        const uint64_t* p0 = g.offsets[0].data();
+        return artea_index_set_graph_csr(idx, 0, ul_max, bl_max, kNoVertex, vi.data(), &p0, nullptr, 0);
+    }
+    std::vector<uint64_t> bucket_n(src_top + 1, 0);
+    for (uint32_t v = 0; v < n; ++v)
+        if (src_h[v] != kInvalidLevel) bucket_n[src_h[v]]++;
+    uint32_t top = 0;
+    for (uint32_t h = src_top;; --h) {
+        if (bucket_n[h] >= 32) {
+            top = h;
+            break;
+        }
+        if (h == 0) break;
+    }
+    g.top = top;
+    std::vector<std::vector<uint32_t>> bucket(top + 1);
+    for (uint32_t sh = 0; sh <= src_top; ++sh) {
+        const uint32_t grp = std::min(sh, top);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t v = order ? order[i] : i;
+            if (src_h[v] == sh) {
+                g.highest[v] = grp;
+                g.local[v] = uint32_t(bucket[grp].size());
+                bucket[grp].push_back(v);
+            }
+        }
+    }
+    g.offsets.resize(top + 1);
+    uint64_t total = 0;
+    for (uint32_t h = 0; h <= top; ++h) {
+        auto& o = g.offsets[h];
+        o.assign(bucket[h].size() * (h + 1) + 1, 0);
+        for (size_t loc = 0; loc < bucket[h].size(); ++loc) {
+            const uint32_t v = bucket[h][loc];
+            for (uint32_t l = h;; --l) {
+                o[loc * (h + 1) + (h - l)] = total;
+                const uint64_t a = row_off[l][v], b = row_off[l][v + 1];
+                g.nbrs.insert(g.nbrs.end(), row_ids[l] + a, row_ids[l] + b);
+                total += b - a;
+                if (l == 0) break;
+            }
+        }
+        o.back() = total;
+    }
+    {
+        const std::vector<uint32_t>& tb = bucket[top];
+        const uint32_t per = 4096, blocks = std::max(1u, uint32_t((tb.size() + per - 1) / per));
+        double* part = nullptr;
+        float* cen = nullptr;
+        float* dists = nullptr;
+        uint32_t* d_ids = nullptr;
+        auto release = [&]() {
+            for (void* q : {(void*)part, (void*)cen, (void*)dists, (void*)d_ids})
+                if (q) (void)hipFree(q);
+        };
+        hipError_t e = hipMalloc(&part, size_t(blocks) * idx->dim_pad * 8);
+        if (e == hipSuccess) e = hipMalloc(&cen, size_t(idx->dim_pad) * 4);
+        if (e == hipSuccess) e = hipMalloc(&dists, std::max<size_t>(n, 1) * 4);
+        if (e == hipSuccess) e = hipMalloc(&d_ids, std::max<size_t>(tb.size(), 1) * 4);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_ids, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, idx->stream);
+        if (e != hipSuccess) {
+            release();
+            set_error(std::string("compact_graph: device buffers failed: ") + hipGetErrorString(e));
+            return ARTEA_EHIP;
+        }
+        hipLaunchKernelGGL(centroid_partial_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->d_vecs, d_ids,
+                           uint32_t(tb.size()), idx->dim_pad, per, part);
+        hipLaunchKernelGGL(centroid_final_kernel, dim3(1), dim3(256), 0, idx->stream, part, blocks,
+                           uint32_t(tb.size()), idx->dim_pad, cen);
+        e = hipStreamSynchronize(idx->stream);
+        int rc = e == hipSuccess ? artea_distance_matrix_device(metric, cen, 1, idx->d_vecs, n, idx->dim_pad, dists,
+                                                                idx->stream)
+                                 : ARTEA_EHIP;
+        std::vector<float> h(n);
+        if (rc == ARTEA_OK) {
+            e = hipMemcpyAsync(h.data(), dists, size_t(n) * 4, hipMemcpyDeviceToHost, idx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(idx->stream);
+            if (e != hipSuccess) rc = ARTEA_EHIP;
+        }
+        release();
+        if (rc) {
+            if (e != hipSuccess) set_error(std::string("compact_graph: centroid failed: ") + hipGetErrorString(e));
+            return rc;
+        }
+        float best = FLT_MAX;
+        uint32_t bv = kNoVertex;
+        for (uint32_t v : tb)
+            if (h[v] < best) {
+                best = h[v];
+                bv = v;
+            }
+        g.entry = bv;
+    }
+    std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2));
+    for (uint32_t v = 0; v < n; ++v) {
+        vi[2 * size_t(v)] = g.highest[v];
+        vi[2 * size_t(v) + 1] = g.local[v];
+    }
+    std::vector<const uint64_t*> ptrs;
+    for (auto& x : g.offsets) ptrs.push_back(x.data());
+    return artea_index_set_graph_csr(idx, g.top, g.ul_max, g.bl_max, g.entry, vi.data(), ptrs.data(), g.nbrs.data(),
+                                     g.nbrs.size());
+}
 }  // namespace
 
 extern "C" {
@@ -1284,6 +1418,41 @@ int artea_refine_create(const float* d_vecs, uint32_t n_global, uint32_t dim_pad
                         const uint32_t* local_to_global, uint32_t n_local, uint64_t seed, artea_refine** out) {
     clear_error();
     return refine_make(d_vecs, n_global, dim_pad, metric, cap, local_to_global, n_local, seed, out);
+}
+
+int artea_index_compact_graph(artea_index* idx, uint32_t max_level, uint32_t ul_max, uint32_t bl_max,
+                              const uint32_t* highest, const uint64_t* const* row_off, const uint32_t* const* row_ids,
+                              const uint32_t* order, int metric) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx, "artea_index_compact_graph: null index");
+    ARTEA_CHECK_ARG(max_level < 64, "artea_index_compact_graph: max_level must be < 64");
+    ARTEA_CHECK_ARG(metric >= 0 && metric <= 3, "artea_index_compact_graph: metric must be 0..3");
+    const uint32_t n = idx->n;
+    ARTEA_CHECK_ARG(n == 0 || (highest && row_off && row_ids), "artea_index_compact_graph: null arrays");
+    for (uint32_t v = 0; v < n; ++v)
+        if (highest[v] != kInvalidLevel && highest[v] > max_level) {
+            set_error("artea_index_compact_graph: vid " + std::to_string(v) + " highest level " +
+                      std::to_string(highest[v]) + " > max level " + std::to_string(max_level));
+            return ARTEA_EINVAL;
+        }
+    if (order) {
+        std::vector<uint8_t> seen(n, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            ARTEA_CHECK_ARG(order[i] < n && !seen[order[i]], "artea_index_compact_graph: order is not a permutation");
+            seen[order[i]] = 1;
+        }
+    }
+    for (uint32_t l = 0; l <= max_level && n; ++l) {
+        ARTEA_CHECK_ARG(row_off[l] && (row_off[l][n] == 0 || row_ids[l]), "artea_index_compact_graph: null level rows");
+        for (uint32_t v = 0; v < n; ++v)
+            ARTEA_CHECK_ARG(row_off[l][v + 1] >= row_off[l][v], "artea_index_compact_graph: row offsets decrease");
+    }
+    if (n == 0) {
+        const uint64_t zero = 0;
+        const uint64_t* p0 = &zero;
+        return artea_index_set_graph_csr(idx, 0, ul_max, bl_max, kNoVertex, nullptr, &p0, nullptr, 0);
+    }
+    return compact_and_attach(idx, highest, order, row_off, row_ids, ul_max, bl_max, metric);
 }
 
 int artea_refine_destroy(artea_refine* r) {
@@ -1513,90 +1682,13 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
         if (seconds_per_level)
             seconds_per_level[l] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
-    // Grouped CSR (compactor step 2): buckets below the top keep ascending
-    // vid order; the top bucket holds its own vertices, then the demoted
-    // ones by ascending source level.  Groups are laid out h = 0..top in the
-    // neighbour array, rows of a vertex from its highest level down.
-    HostCsr g;
-    g.n = n;
-    g.top = top;
-    g.ul_max = ul_max;
-    g.bl_max = bl_max;
-    g.highest.assign(n, 0);
-    g.local.assign(n, 0);
-    std::vector<std::vector<uint32_t>> bucket(top + 1);
-    for (uint32_t sh = 0; sh <= src_top; ++sh) {
-        const uint32_t grp = std::min(sh, top);
-        for (uint32_t v = 0; v < n; ++v)
-            if (src_h[v] == sh) {
-                g.highest[v] = grp;
-                g.local[v] = uint32_t(bucket[grp].size());
-                bucket[grp].push_back(v);
-            }
+    std::vector<const uint64_t*> offp(top + 1);
+    std::vector<const uint32_t*> idp(top + 1);
+    for (uint32_t l = 0; l <= top; ++l) {
+        offp[l] = roff[l].data();
+        idp[l] = rids[l].data();
     }
-    g.offsets.resize(top + 1);
-    uint64_t total = 0;
-    for (uint32_t h = 0; h <= top; ++h) {
-        auto& o = g.offsets[h];
-        o.assign(bucket[h].size() * (h + 1) + 1, 0);
-        for (size_t loc = 0; loc < bucket[h].size(); ++loc) {
-            const uint32_t v = bucket[h][loc];
-            for (uint32_t l = h;; --l) {
-                o[loc * (h + 1) + (h - l)] = total;
-                const uint64_t a = roff[l][v], b = roff[l][v + 1];
-                g.nbrs.insert(g.nbrs.end(), rids[l].begin() + a, rids[l].begin() + b);
-                total += b - a;
-                if (l == 0) break;
-            }
-        }
-        o.back() = total;
-    }
-    // Entry point: the top-bucket vertex closest to the FP64 centroid of the
-    // top bucket (build metric); ties keep the earliest bucket position.
-    if (n > 0) {
-        const std::vector<uint32_t>& tb = bucket[top];
-        const uint32_t per = 4096, blocks = std::max(1u, uint32_t((tb.size() + per - 1) / per));
-        double* part = nullptr;
-        float* cen = nullptr;
-        float* dists = nullptr;
-        uint32_t* d_ids = nullptr;
-        ARTEA_HIP_TRY(hipMalloc(&part, size_t(blocks) * idx->dim_pad * 8));
-        ARTEA_HIP_TRY(hipMalloc(&cen, size_t(idx->dim_pad) * 4));
-        ARTEA_HIP_TRY(hipMalloc(&dists, std::max<size_t>(n, 1) * 4));
-        ARTEA_HIP_TRY(hipMalloc(&d_ids, std::max<size_t>(tb.size(), 1) * 4));
-        ARTEA_HIP_TRY(hipMemcpyAsync(d_ids, tb.data(), tb.size() * 4, hipMemcpyHostToDevice, idx->stream));
-        hipLaunchKernelGGL(centroid_partial_kernel, dim3(blocks), dim3(256), 0, idx->stream, idx->d_vecs, d_ids,
-                           uint32_t(tb.size()), idx->dim_pad, per, part);
-        hipLaunchKernelGGL(centroid_final_kernel, dim3(1), dim3(256), 0, idx->stream, part, blocks,
-                           uint32_t(tb.size()), idx->dim_pad, cen);
-        ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
-        int rc = artea_distance_matrix_device(bmetric, cen, 1, idx->d_vecs, n, idx->dim_pad, dists, idx->stream);
-        if (rc) return rc;
-        std::vector<float> h(n);
-        ARTEA_HIP_TRY(hipMemcpyAsync(h.data(), dists, size_t(n) * 4, hipMemcpyDeviceToHost, idx->stream));
-        ARTEA_HIP_TRY(hipStreamSynchronize(idx->stream));
-        float best = FLT_MAX;
-        uint32_t bv = kNoVertex;
-        for (uint32_t v : tb)
-            if (h[v] < best) {
-                best = h[v];
-                bv = v;
-            }
-        g.entry = bv;
-        (void)hipFree(part);
-        (void)hipFree(cen);
-        (void)hipFree(dists);
-        (void)hipFree(d_ids);
-    }
-    std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2));
-    for (uint32_t v = 0; v < n; ++v) {
-        vi[2 * size_t(v)] = g.highest[v];
-        vi[2 * size_t(v) + 1] = g.local[v];
-    }
-    std::vector<const uint64_t*> ptrs;
-    for (auto& x : g.offsets) ptrs.push_back(x.data());
-    return artea_index_set_graph_csr(idx, g.top, g.ul_max, g.bl_max, g.entry, vi.data(), ptrs.data(), g.nbrs.data(),
-                                     g.nbrs.size());
+    return compact_and_attach(idx, src_h.data(), nullptr, offp.data(), idp.data(), ul_max, bl_max, bmetric);
 }
 
 }  // extern "C"

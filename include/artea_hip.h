@@ -315,6 +315,22 @@ int artea_refine_set_recheck_queue(artea_refine* r, uint32_t entries);
  * FP32 chain, pair distances evaluated on the BF16 tiles. */
 int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pairs, uint64_t* approx_pairs);
 
+/* HierarchicalGraphCompactor::compact_graph (compactor/
+ * hierarchical_graph_compactor.hpp:118-327) of a dynamic hierarchical graph,
+ * attached to idx: highest[v] = source level (ARTEA_INVALID_LEVEL =
+ * unassigned, at most max_level), row_off[l] (n + 1 offsets) and row_ids[l]
+ * = the rows of level l (l = 0..max_level, global ids), order = the order
+ * the vertices were assigned to their buckets (a permutation; NULL =
+ * ascending vid).  The first bucket from the top with >= 32 vertices is the
+ * top; higher vertices are demoted into it after its own, by ascending source
+ * level, keeping their rows at levels <= top; entry = the top-bucket vertex
+ * closest by `metric` to the FP64 centroid of the top bucket (computed on the
+ * GPU; ties keep the earliest bucket position).  No assigned vertex: an empty
+ * graph without entry. */
+int artea_index_compact_graph(artea_index* idx, uint32_t max_level, uint32_t ul_max, uint32_t bl_max,
+                              const uint32_t* highest, const uint64_t* const* row_off, const uint32_t* const* row_ids,
+                              const uint32_t* order, int metric);
+
 /* Hierarchy build on the GPU, attached to idx.  level_fraction = 1: every
  * vertex on levels 0..max_level (the r-net hierarchy when no vertex is
  * absorbed, index/stacked_rgraph/index_factory.hpp:364-381).  Below 1:

@@ -308,6 +308,18 @@ orc_graph* orc_compact_graph(uint32_t n, uint32_t max_level, uint32_t ul_max, ui
                              const uint32_t* highest, const uint64_t* const* row_off,
                              const uint32_t* const* row_ids, const float* vecs, uint32_t D, int metric,
                              char* err, size_t errlen) {
+    return orc_compact_graph_ordered(n, max_level, ul_max, bl_max, highest, row_off, row_ids, NULL, vecs, D, metric,
+                                     err, errlen);
+}
+
+/* The dynamic graph's buckets list vertices in the order they were assigned
+ * (assign_layer appends, dynamic_structure/hierarchical_graph.hpp); `order`
+ * gives that order (NULL = ascending vid) and the compactor keeps it within
+ * every bucket (compactor/hierarchical_graph_compactor.hpp:146-179). */
+orc_graph* orc_compact_graph_ordered(uint32_t n, uint32_t max_level, uint32_t ul_max, uint32_t bl_max,
+                                     const uint32_t* highest, const uint64_t* const* row_off,
+                                     const uint32_t* const* row_ids, const uint32_t* order, const float* vecs,
+                                     uint32_t D, int metric, char* err, size_t errlen) {
     uint32_t src_top = ORC_INVALID_LEVEL;
     for (uint32_t v = 0; v < n; ++v) {
         if (highest[v] == ORC_INVALID_LEVEL) continue;
@@ -344,7 +356,8 @@ orc_graph* orc_compact_graph(uint32_t n, uint32_t max_level, uint32_t ul_max, ui
     uint64_t* fill = (uint64_t*)calloc(new_top + 1, sizeof(uint64_t));
     for (uint32_t src_h = 0; src_h <= src_top; ++src_h) {
         const uint32_t grp = src_h < new_top ? src_h : new_top;
-        for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t v = order ? order[i] : i;
             if (highest[v] != src_h) continue;
             g->bucket[grp][fill[grp]] = v;
             g->highest[v] = grp;

@@ -62,6 +62,12 @@ orc_graph* orc_compact_graph(uint32_t n, uint32_t max_level, uint32_t ul_max, ui
                              const uint32_t* highest, const uint64_t* const* row_off,
                              const uint32_t* const* row_ids, const float* vecs, uint32_t dim_pad,
                              int metric, char* err, size_t errlen);
+/* The same with the order the vertices were assigned to their buckets
+ * (a permutation of 0..n-1; NULL = ascending vid), which every bucket keeps. */
+orc_graph* orc_compact_graph_ordered(uint32_t n, uint32_t max_level, uint32_t ul_max, uint32_t bl_max,
+                                     const uint32_t* highest, const uint64_t* const* row_off,
+                                     const uint32_t* const* row_ids, const uint32_t* order, const float* vecs,
+                                     uint32_t dim_pad, int metric, char* err, size_t errlen);
 /* Direct CSR construction, as the builder-only mutators of
  * compact_structure/hierarchical_graph.hpp:135-162 allow: vertex_info[2v] =
  * highest level (ORC_INVALID_LEVEL = unassigned), vertex_info[2v+1] = local

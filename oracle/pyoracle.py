@@ -57,6 +57,7 @@ def _declare(l):
     d("orc_distance_f64", C.c_double, i32, P, P, u32)
     d("orc_distance_matrix", None, i32, P, u32, P, u32, u32, P, i32)
     d("orc_compact_graph", P, u32, u32, u32, u32, P, P, P, P, u32, i32, C.c_char_p, C.c_size_t)
+    d("orc_compact_graph_ordered", P, u32, u32, u32, u32, P, P, P, P, P, u32, i32, C.c_char_p, C.c_size_t)
     d("orc_graph_from_csr", P, u32, u32, u32, u32, u32, P, P, P, u64, C.c_char_p, C.c_size_t)
     d("orc_graph_snapshot", i32, P, C.c_char_p, C.c_char_p, C.c_size_t)
     d("orc_graph_restore", P, C.c_char_p, C.c_char_p, C.c_size_t)
@@ -146,8 +147,9 @@ class Graph:
             self.h = None
 
     @classmethod
-    def compact(cls, highest, levels, vecs, metric, ul_max, bl_max):
-        """levels: list over level l of list over vid of neighbour-id lists."""
+    def compact(cls, highest, levels, vecs, metric, ul_max, bl_max, order=None):
+        """levels: list over level l of list over vid of neighbour-id lists;
+        order: the order the vertices were assigned to their buckets."""
         n = len(highest)
         max_level = len(levels) - 1
         hl = np.ascontiguousarray(highest, dtype=np.uint32)
@@ -168,8 +170,10 @@ class Graph:
         id_ptrs = (C.c_void_p * (max_level + 1))(*[i.ctypes.data for i in ids])
         v = _f32(vecs)
         err = C.create_string_buffer(512)
-        h = lib().orc_compact_graph(n, max_level, ul_max, bl_max, _ptr(hl), off_ptrs, id_ptrs, _ptr(v),
-                                    v.shape[1] if v.ndim == 2 else 16, metric, err, 512)
+        od = None if order is None else np.ascontiguousarray(order, np.uint32)
+        h = lib().orc_compact_graph_ordered(n, max_level, ul_max, bl_max, _ptr(hl), off_ptrs, id_ptrs,
+                                            _ptr(od) if od is not None else None, _ptr(v),
+                                            v.shape[1] if v.ndim == 2 else 16, metric, err, 512)
         if not h:
             raise RuntimeError(err.value.decode())
         return cls(h)

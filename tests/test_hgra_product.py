@@ -157,3 +157,51 @@ def test_index_restore_rejects_and_accepts(tmp_path):
     assert q.read_bytes() == p.read_bytes()
     v, d = aa.HierarchicalGraphRouter(idx, None, 2, 4).batch_query(vecs[2:3])
     assert list(v[0]) == [2, 1] and list(d[0]) == [0.0, 1.0]
+
+
+def test_restore_rejects_the_reference_one_vertex_corruptions(tmp_path):
+    # test_hierarchical_graph_compactor.cpp:293-311: one L0 vertex with a
+    # self edge; seven single-word corruptions and every truncation fail.
+    valid = [MAGIC, 1, 1, 0, 3, 5, 0, 0, 1, 0]
+    p = tmp_path / "valid.graph"
+    p.write_bytes(words(valid))
+    rc, err, h = restore(p)
+    assert rc == 0, err
+    from artea_amd._lib import lib
+
+    lib().artea_graph_destroy(h)
+    bad = []
+    for word, repl in ((0, 0), (1, 2), (3, 0xFFFFFFFF), (6, 1), (7, 1), (8, 6), (9, 0xFFFFFFFF)):
+        w = list(valid)
+        w[word] = repl
+        bad.append(w)
+    bad += [valid[:k] for k in range(len(valid))]
+    for i, w in enumerate(bad):
+        q = tmp_path / ("bad%d.graph" % i)
+        q.write_bytes(words(w))
+        rc, err, _ = restore(q)
+        assert rc == 2 and "HierarchicalGraphFileManager::restore" in err, (i, w, rc, err)
+
+
+def test_unassigned_records_round_trip(tmp_path):
+    # :276-291: vertices with no level restore as unassigned, no entry.
+    from artea_amd._lib import lib
+
+    for n in (0, 3):
+        p = tmp_path / ("unassigned%d.graph" % n)
+        p.write_bytes(words([MAGIC, 1, n, 0, 3, 5, 0xFFFFFFFF] + [0xFFFFFFFF] * n))
+        rc, err, h = restore(p)
+        assert rc == 0, err
+        try:
+            vals = [C.c_uint32() for _ in range(5)]
+            nn = C.c_uint64()
+            assert lib().artea_graph_info(h, *[C.byref(x) for x in vals], C.byref(nn)) == 0
+            assert vals[0].value == n and vals[4].value == 0xFFFFFFFF and nn.value == 0
+            vi = lib().artea_graph_vertex_info(h)
+            for v in range(n):
+                assert vi[2 * v] == 0xFFFFFFFF and vi[2 * v + 1] == 0xFFFFFFFF
+            q = tmp_path / ("again%d.graph" % n)
+            assert lib().artea_graph_snapshot(h, str(q).encode()) == 0
+            assert q.read_bytes() == p.read_bytes()
+        finally:
+            lib().artea_graph_destroy(h)
