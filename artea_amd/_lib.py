@@ -102,6 +102,12 @@ SIGNATURES = {
     "artea_refine_last_stats": (i32, [P, P, P, P]),
     "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, P]),
     "artea_index_compact_graph": (i32, [P, u32, u32, u32, P, P, P, P, i32]),
+    "artea_recall_at_k_device": (i32, [P, u32, P, u32, u32, u32, C.POINTER(u64), P]),
+    "artea_adr_at_1_device": (i32, [i32, P, u32, P, u32, u32, P, u32, P, u32, C.POINTER(C.c_double),
+                                    C.POINTER(u64), P]),
+    "artea_probe_knn_device": (i32, [i32, P, u32, u32, P, u32, P, C.POINTER(f32), P]),
+    "artea_probe_farthest_device": (i32, [i32, P, u32, u32, P, u32, P, P]),
+    "artea_distance_rows_device": (i32, [i32, P, u32, P, u32, P, u32, u32, u32, P, P]),
 }
 
 STATUS = {0: "ARTEA_OK", 1: "ARTEA_EINVAL", 2: "ARTEA_EIO", 3: "ARTEA_EHIP", 4: "ARTEA_ENOMEM", 5: "ARTEA_EOVERFLOW"}

@@ -380,6 +380,48 @@ int artea_bruteforce_knn_mfma_device(int metric, const float* d_base, uint32_t n
                                      uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* d_out_vids,
                                      double* d_out_dists, void* stream, uint64_t* stats);
 
+/* ------------------------------------------------ scoring and data probes */
+/* RecallEstimator::calculate_recall_at_k (utils/recall_estimator.hpp:54-230)
+ * on device id rows: *matches = sum over the nq queries of the number of
+ * distinct ids among the first k of the prediction row that occur among the
+ * first k of the ground-truth row; Recall@K = matches / (nq * k).  k <=
+ * 1024; a ground-truth row shorter than k fails as the reference does
+ * ("Ground truth dimension ... is less than k=...").  Synchronous on
+ * `stream`. */
+int artea_recall_at_k_device(const uint32_t* d_pred, uint32_t pred_stride, const uint32_t* d_gt, uint32_t gt_stride,
+                             uint32_t nq, uint32_t k, uint64_t* matches, void* stream);
+/* ADREstimator::calculate_adr_at_1 (utils/adr_estimator.hpp): the mean over
+ * queries of double(dist(q, pred[q][0])) / double(dist(q, gt[q][0])) with
+ * dist_func_t's FP32 chain; queries whose ground-truth distance is <= 0
+ * are skipped (a NaN one is not), as are queries whose top-1 slot holds no
+ * vertex (the invalid entry); the terms are summed in query order.  *counted = the
+ * queries summed; *adr = 0 when none.  Synchronous on `stream`. */
+int artea_adr_at_1_device(int metric, const float* d_base, uint32_t n, const float* d_queries, uint32_t nq,
+                          uint32_t dim_pad, const uint32_t* d_pred, uint32_t pred_stride, const uint32_t* d_gt,
+                          uint32_t gt_stride, double* adr, uint64_t* counted, void* stream);
+/* DatasetProber::_compute_knn_table (utils/dataset_prober.hpp:471-540): for
+ * each sample vid (host array, every id < n), the 128 smallest
+ * dist(base[vid], base[j]) over j != vid, ascending, into knn (host,
+ * num_samples x 128); *min_nearest (optional) = the smallest first column
+ * (probe's min_nearest, :253-306).  A build's auto-probed l0_min_distance
+ * is the first column's entry at one quantile
+ * (unit_tests/test_hierarchical_graph_persistence.cpp:125-137).  n must
+ * exceed 128.  The reference draws
+ * the samples from MKL streams; here the caller passes them. */
+int artea_probe_knn_device(int metric, const float* d_base, uint32_t n, uint32_t dim_pad, const uint32_t* samples,
+                           uint32_t num_samples, float* knn, float* min_nearest, void* stream);
+/* DatasetProber::_compute_farthest (:542-585): per sample, the largest
+ * dist(base[vid], base[j]) over j != vid, into farthest (host). */
+int artea_probe_farthest_device(int metric, const float* d_base, uint32_t n, uint32_t dim_pad,
+                                const uint32_t* samples, uint32_t num_samples, float* farthest, void* stream);
+/* out[i * k + j] = dist(a_i, b[ids[i * ids_stride + j]]) for device arrays
+ * (the query-to-ground-truth distances of DatasetProber::probe_query,
+ * :327-432); an id at or beyond nb (the invalid vertex of a
+ * padded result row) gives +inf. */
+int artea_distance_rows_device(int metric, const float* d_a, uint32_t na, const float* d_b, uint32_t nb,
+                               const uint32_t* d_ids, uint32_t ids_stride, uint32_t k, uint32_t dim_pad, float* d_out,
+                               void* stream);
+
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching request server (no reference counterpart: the
  * reference's batch_query, router/hierarchical_graph_router.hpp:293-385, is

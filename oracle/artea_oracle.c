@@ -1388,3 +1388,140 @@ void orc_bruteforce_knn(int metric, const float* base, uint32_t n, const float* 
         free(best);
     }
 }
+
+/* ======================================================================
+ * Scoring and dataset probes.
+ * ====================================================================== */
+static int cmp_u32(const void* a, const void* b) {
+    const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* RecallEstimator::calculate_recall_at_k (utils/recall_estimator.hpp:54-137):
+ * per query, sort the first k ground-truth ids and the first k predicted
+ * ids, drop repeated predictions (std::unique), and count the sorted
+ * intersection (std::set_intersection); the sum over queries. */
+uint64_t orc_recall_matches(const uint32_t* pred, uint32_t pstride, const uint32_t* gt, uint32_t gstride, uint32_t nq,
+                            uint32_t k) {
+    uint64_t total = 0;
+    uint32_t* sg = (uint32_t*)malloc(sizeof(uint32_t) * (k ? k : 1));
+    uint32_t* sp = (uint32_t*)malloc(sizeof(uint32_t) * (k ? k : 1));
+    for (uint32_t q = 0; q < nq; ++q) {
+        memcpy(sg, gt + (size_t)q * gstride, sizeof(uint32_t) * k);
+        memcpy(sp, pred + (size_t)q * pstride, sizeof(uint32_t) * k);
+        qsort(sg, k, sizeof(uint32_t), cmp_u32);
+        qsort(sp, k, sizeof(uint32_t), cmp_u32);
+        uint32_t u = 0; /* std::unique */
+        for (uint32_t i = 0; i < k; ++i)
+            if (u == 0 || sp[u - 1] != sp[i]) sp[u++] = sp[i];
+        uint32_t i = 0, j = 0; /* std::set_intersection */
+        while (i < k && j < u) {
+            if (sg[i] < sp[j]) ++i;
+            else if (sp[j] < sg[i]) ++j;
+            else {
+                ++total;
+                ++i;
+                ++j;
+            }
+        }
+    }
+    free(sg);
+    free(sp);
+    return total;
+}
+
+/* ADREstimator::calculate_adr_at_1 (utils/adr_estimator.hpp): mean over
+ * queries of double(dist(q, retrieved)) / double(dist(q, true NN)), queries
+ * with a non-positive ground-truth distance skipped; summed in query order.
+ * A top-1 slot without a vertex (id >= n) is skipped too (the reference
+ * would read past the base there). */
+double orc_adr_at_1(int metric, const float* base, uint32_t n, const float* queries, uint32_t nq, uint32_t D,
+                    const uint32_t* pred, uint32_t pstride, const uint32_t* gt, uint32_t gstride, uint64_t* counted) {
+    double sum = 0.0;
+    uint64_t c = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint32_t r = pred[(size_t)q * pstride], t = gt[(size_t)q * gstride];
+        if (r >= n || t >= n) continue;
+        const float* x = queries + (size_t)q * D;
+        const float dr = orc_distance(metric, x, base + (size_t)r * D, D);
+        const float dt = orc_distance(metric, x, base + (size_t)t * D, D);
+        if (dt <= 0.0f) continue;
+        sum += (double)dr / (double)dt;
+        ++c;
+    }
+    if (counted) *counted = c;
+    return c ? sum / (double)c : 0.0;
+}
+
+/* Max-heap of floats (std::priority_queue<distance_t>). */
+static void fheap_push(float* h, uint32_t* n, float v) {
+    uint32_t i = (*n)++;
+    h[i] = v;
+    while (i > 0) {
+        const uint32_t p = (i - 1) / 2;
+        if (!(h[p] < h[i])) break;
+        const float t = h[p];
+        h[p] = h[i];
+        h[i] = t;
+        i = p;
+    }
+}
+static float fheap_pop(float* h, uint32_t* n) {
+    const float top = h[0];
+    h[0] = h[--(*n)];
+    uint32_t i = 0;
+    for (;;) {
+        const uint32_t l = 2 * i + 1, r = l + 1;
+        uint32_t m = i;
+        if (l < *n && h[m] < h[l]) m = l;
+        if (r < *n && h[m] < h[r]) m = r;
+        if (m == i) break;
+        const float t = h[m];
+        h[m] = h[i];
+        h[i] = t;
+        i = m;
+    }
+    return top;
+}
+
+/* DatasetProber::_compute_knn_table (utils/dataset_prober.hpp:471-540) for
+ * given samples: per sample, a max-heap of the 128 smallest dist(sample, j)
+ * over j != sample, extracted ascending into out[s * 128 ..]. */
+void orc_probe_knn(int metric, const float* base, uint32_t n, uint32_t D, const uint32_t* samples, uint32_t ns,
+                   float* out, int nthreads) {
+    enum { K = 128 };
+#pragma omp parallel for schedule(dynamic, 1) num_threads(pick_threads(nthreads))
+    for (int64_t s = 0; s < (int64_t)ns; ++s) {
+        float heap[K];
+        uint32_t hn = 0;
+        const uint32_t vid = samples[s];
+        const float* q = base + (size_t)vid * D;
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == vid) continue;
+            const float d = orc_distance(metric, q, base + (size_t)j * D, D);
+            if (hn < K) {
+                fheap_push(heap, &hn, d);
+            } else if (d < heap[0]) {
+                fheap_pop(heap, &hn);
+                fheap_push(heap, &hn, d);
+            }
+        }
+        for (uint32_t k = hn; k > 0; --k) out[(size_t)s * K + k - 1] = fheap_pop(heap, &hn);
+    }
+}
+
+/* DatasetProber::_compute_farthest (:542-585) for given samples. */
+void orc_probe_farthest(int metric, const float* base, uint32_t n, uint32_t D, const uint32_t* samples, uint32_t ns,
+                        float* out, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(pick_threads(nthreads))
+    for (int64_t s = 0; s < (int64_t)ns; ++s) {
+        const uint32_t vid = samples[s];
+        float m = -FLT_MAX;
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == vid) continue;
+            const float d = orc_distance(metric, base + (size_t)vid * D, base + (size_t)j * D, D);
+            if (d > m) m = d;
+        }
+        out[s] = m;
+    }
+}

@@ -222,6 +222,19 @@ void orc_bruteforce_knn(int metric, const float* base, uint32_t n, const float* 
                         uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* out_vids,
                         double* out_dists, int nthreads);
 
+/* Scoring and dataset probes: RecallEstimator (utils/recall_estimator.hpp:
+ * 54-137) as the total of per-query matches, ADREstimator::calculate_adr_at_1
+ * (utils/adr_estimator.hpp), DatasetProber::_compute_knn_table and
+ * _compute_farthest (utils/dataset_prober.hpp:471-585) for given samples. */
+uint64_t orc_recall_matches(const uint32_t* pred, uint32_t pstride, const uint32_t* gt, uint32_t gstride, uint32_t nq,
+                            uint32_t k);
+double orc_adr_at_1(int metric, const float* base, uint32_t n, const float* queries, uint32_t nq, uint32_t dim_pad,
+                    const uint32_t* pred, uint32_t pstride, const uint32_t* gt, uint32_t gstride, uint64_t* counted);
+void orc_probe_knn(int metric, const float* base, uint32_t n, uint32_t dim_pad, const uint32_t* samples, uint32_t ns,
+                   float* out, int nthreads);
+void orc_probe_farthest(int metric, const float* base, uint32_t n, uint32_t dim_pad, const uint32_t* samples,
+                        uint32_t ns, float* out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

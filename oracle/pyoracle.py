@@ -95,6 +95,10 @@ def _declare(l):
     d("orc_next_truncate", None, P, i32)
     d("orc_next_reverse", None, P, i32)
     d("orc_next_triangle_range", None, P, f32, f32, u32, u32, i32)
+    d("orc_recall_matches", u64, P, u32, P, u32, u32, u32)
+    d("orc_adr_at_1", C.c_double, i32, P, u32, P, u32, u32, P, u32, P, u32, P)
+    d("orc_probe_knn", None, i32, P, u32, u32, P, u32, P, i32)
+    d("orc_probe_farthest", None, i32, P, u32, u32, P, u32, P, i32)
     d("orc_next_reverse_range", None, P, u32, u32, i32)
     d("orc_refine_schedule", None, P, u32, u32, f32, f32, i32)
     d("orc_layer_last_logs", u64, P)
@@ -445,3 +449,37 @@ def build_full(vecs, metric=L2, ul_max=32, bl_max=64, max_level_=None, build_loo
 
 def build_level_of(seed: int, v: int, max_level_: int, level_fraction: float) -> int:
     return int(lib().orc_build_level_of(seed, v, max_level_, level_fraction))
+
+
+def recall_matches(pred, gt, k):
+    """Total RecallEstimator matches of rows pred[:, :k] against gt[:, :k]."""
+    p = np.ascontiguousarray(pred, np.uint32)
+    g = np.ascontiguousarray(gt, np.uint32)
+    return int(lib().orc_recall_matches(_ptr(p), p.shape[1], _ptr(g), g.shape[1], p.shape[0], k))
+
+
+def adr_at_1(metric, base, queries, pred, gt):
+    """(ADR@1, queries counted) of the top-1 column of pred against gt's."""
+    b, q = _f32(base), _f32(queries)
+    p = np.ascontiguousarray(pred, np.uint32)
+    g = np.ascontiguousarray(gt, np.uint32)
+    c = C.c_uint64()
+    a = lib().orc_adr_at_1(metric, _ptr(b), b.shape[0], _ptr(q), q.shape[0], b.shape[1], _ptr(p), p.shape[1], _ptr(g),
+                           g.shape[1], C.byref(c))
+    return float(a), c.value
+
+
+def probe_knn(metric, base, samples, nthreads=0):
+    b = _f32(base)
+    s = np.ascontiguousarray(samples, np.uint32)
+    out = np.zeros((len(s), 128), np.float32)
+    lib().orc_probe_knn(metric, _ptr(b), b.shape[0], b.shape[1], _ptr(s), len(s), _ptr(out), nthreads)
+    return out
+
+
+def probe_farthest(metric, base, samples, nthreads=0):
+    b = _f32(base)
+    s = np.ascontiguousarray(samples, np.uint32)
+    out = np.zeros(len(s), np.float32)
+    lib().orc_probe_farthest(metric, _ptr(b), b.shape[0], b.shape[1], _ptr(s), len(s), _ptr(out), nthreads)
+    return out
