@@ -22,7 +22,6 @@ namespace {
 
 constexpr uint32_t kProbeK = 128;     // DatasetProber::MAX_K
 constexpr uint32_t kRecallMaxK = 1024;
-constexpr uint32_t kSelBuf = 2048;    // candidate buffer of the k-NN selection (best kProbeK first)
 
 // Bitonic sort of s[0..P) ascending (P a power of two <= 2048), 256 threads.
 template <typename T>
@@ -106,56 +105,161 @@ __global__ __launch_bounds__(256) void distance_rows_kernel(int metric, const fl
     if (ok && sub == 0) out[size_t(i) * k + j] = in ? d : INFINITY;
 }
 
-// Per sample (one block): the kProbeK smallest dist(sample, j) over every
-// j != sample, ascending, as a multiset of values (the reference keeps them
-// in a max-heap; ties do not change the multiset).  The block streams the
-// base 64 rows per pass (a quad per row); a row enters the candidate buffer
-// when fewer than kProbeK values are held or it is below the current
-// kProbeK-th smallest; a full buffer is sorted together with the best
-// kProbeK and cut back to them.
-__global__ __launch_bounds__(256) void probe_knn_kernel(int metric, const float* __restrict__ base, uint32_t n,
-                                                        uint32_t D, const uint32_t* __restrict__ samples,
-                                                        float* __restrict__ out) {
-    __shared__ float buf[kSelBuf];
-    __shared__ uint32_t s_fill;
-    __shared__ float s_thr;
-    const uint32_t s = blockIdx.x, vid = samples[s];
-    const uint32_t sub = threadIdx.x & 3u, C = D / 16;
-    const float* q = base + size_t(vid) * D;
-    const float qn2 = metric == kCosine ? quad_norm2(q, int(C), sub) : 0.f;
-    if (threadIdx.x == 0) {
-        s_fill = 0;
-        s_thr = INFINITY;
+// The 16 accumulators of one AVX-512 register summed in the order of GCC's
+// _mm512_reduce_add_ps (the tree reduce16_quad spreads over a quad), in one
+// lane: t8[i] = a[i+8] + a[i], t4[i] = t8[i+4] + t8[i], then
+// (t4[0] + t4[2]) + (t4[1] + t4[3]).
+__device__ __forceinline__ float reduce16_lane(const float (&a)[16]) {
+    float t8[8], t4[4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t8[i] = a[i + 8] + a[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t4[i] = t8[i + 4] + t8[i];
+    return (t4[0] + t4[2]) + (t4[1] + t4[3]);
+}
+
+constexpr uint32_t kProbeG = 8;       // samples per block of the k-NN probe
+constexpr uint32_t kProbeBuf = 1024;  // candidate buffer per sample (best kProbeK first)
+
+// The kProbeK smallest dist(sample, j) over every j != sample, ascending, as
+// a multiset of values (the reference keeps them in a max-heap; ties do not
+// change the multiset), for kProbeG samples per block.  Each lane takes one
+// base row per pass (256 rows) and runs dist_func_t's 16 accumulator chains
+// for all kProbeG samples in its own registers, so every row is read once
+// for the kProbeG samples; the sample rows are wave-uniform.  A value enters
+// its sample's candidate buffer when it is below that sample's current
+// kProbeK-th smallest (+inf until kProbeK are held); a buffer that could
+// overflow in the next pass is sorted and cut back to its best kProbeK.
+//
+// The rows are cut into gridDim.y slices of `slice` rows (a block per sample
+// group and slice, so a few hundred samples still fill the chip); each block
+// writes each sample's slice best (padded with +inf) to
+// out[(s * gridDim.y + y) * kProbeK ..], which probe_merge_kernel reduces.
+template <int METRIC>
+__global__ __launch_bounds__(256) void probe_knn_kernel(const float* __restrict__ base, uint32_t n, uint32_t D,
+                                                        const uint32_t* __restrict__ samples, uint32_t ns,
+                                                        uint32_t slice, float* __restrict__ out) {
+    __shared__ float buf[kProbeG][kProbeBuf];
+    __shared__ uint32_t s_fill[kProbeG];
+    __shared__ float s_thr[kProbeG];
+    const uint32_t C = D / 16;
+    const uint32_t r_begin = blockIdx.y * slice, r_end = min(n, r_begin + slice);
+    uint32_t vid[kProbeG];
+    const float* q[kProbeG];
+    float qn2[kProbeG];
+#pragma unroll
+    for (uint32_t g = 0; g < kProbeG; ++g) {
+        vid[g] = samples[min(blockIdx.x * kProbeG + g, ns - 1)];
+        q[g] = base + size_t(vid[g]) * D;
+        qn2[g] = 0.f;
+        if (METRIC == kCosine) {
+            float a[16] = {};
+            for (uint32_t c = 0; c < C; ++c)
+#pragma unroll
+                for (int l = 0; l < 16; ++l) a[l] = __builtin_fmaf(q[g][c * 16 + l], q[g][c * 16 + l], a[l]);
+            qn2[g] = reduce16_lane(a);
+        }
+    }
+    if (threadIdx.x < kProbeG) {
+        s_fill[threadIdx.x] = 0;
+        s_thr[threadIdx.x] = INFINITY;
     }
     __syncthreads();
-    // Entered by every thread with the same fill f (read between barriers).
-    auto compact = [&](uint32_t f) {
+    // Entered by every thread with the same g and fill f (read between barriers).
+    auto compact = [&](uint32_t g, uint32_t f) {
         const uint32_t P = max(pow2_at_least(f), kProbeK);
-        for (uint32_t i = f + threadIdx.x; i < P; i += blockDim.x) buf[i] = INFINITY;
+        for (uint32_t i = f + threadIdx.x; i < P; i += blockDim.x) buf[g][i] = INFINITY;
         __syncthreads();
-        block_bitonic(buf, P);
+        block_bitonic(buf[g], P);
         if (threadIdx.x == 0) {
-            s_fill = min(f, kProbeK);
-            if (f >= kProbeK) s_thr = buf[kProbeK - 1];
+            s_fill[g] = min(f, kProbeK);
+            if (f >= kProbeK) s_thr[g] = buf[g][kProbeK - 1];
         }
         __syncthreads();
     };
-    for (uint32_t row0 = 0; row0 < n; row0 += 64) {
-        const uint32_t j = row0 + threadIdx.x / 4;
-        float d = INFINITY;
-        if (j < n && j != vid) d = quad_distance<0>(q, base + size_t(j) * D, int(C), metric, sub, qn2);
-        const float thr = s_thr;
-        if (sub == 0 && j < n && j != vid && (d < thr)) {
-            const uint32_t at = atomicAdd(&s_fill, 1u);
-            buf[at] = d;  // at < kSelBuf: compacted below whenever 64 more could overflow
+    for (uint32_t row0 = r_begin; row0 < r_end; row0 += 256) {
+        const uint32_t j = row0 + threadIdx.x;
+        const bool ok = j < r_end;
+        const float* x = base + size_t(ok ? j : r_begin) * D;
+        float acc[kProbeG][16], xn[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            xn[l] = 0.f;
+#pragma unroll
+            for (uint32_t g = 0; g < kProbeG; ++g) acc[g][l] = 0.f;
+        }
+        for (uint32_t c = 0; c < C; ++c) {
+            float xv[16];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float4 t = *reinterpret_cast<const float4*>(x + c * 16 + v * 4);
+                xv[4 * v] = t.x;
+                xv[4 * v + 1] = t.y;
+                xv[4 * v + 2] = t.z;
+                xv[4 * v + 3] = t.w;
+            }
+#pragma unroll
+            for (uint32_t g = 0; g < kProbeG; ++g) {
+#pragma unroll
+                for (int l = 0; l < 16; ++l) {
+                    const float qv = q[g][c * 16 + l];
+                    if (METRIC == kL2Sqr || METRIC == kL2) {
+                        const float d = qv - xv[l];
+                        acc[g][l] = __builtin_fmaf(d, d, acc[g][l]);
+                    } else {
+                        acc[g][l] = __builtin_fmaf(qv, xv[l], acc[g][l]);
+                    }
+                }
+            }
+            if (METRIC == kCosine) {
+#pragma unroll
+                for (int l = 0; l < 16; ++l) xn[l] = __builtin_fmaf(xv[l], xv[l], xn[l]);
+            }
+        }
+        const float sq2 = METRIC == kCosine ? reduce16_lane(xn) : 0.f;
+#pragma unroll
+        for (uint32_t g = 0; g < kProbeG; ++g) {
+            float d = reduce16_lane(acc[g]);
+            if (METRIC == kL2) d = __builtin_sqrtf(d);
+            if (METRIC == kDot) d = -d;
+            if (METRIC == kCosine) {
+                const float denom = __builtin_sqrtf(qn2[g] * sq2);
+                d = denom > 0.0f ? 1.0f - d / denom : 1.0f;
+            }
+            if (ok && j != vid[g] && d < s_thr[g]) {
+                const uint32_t at = atomicAdd(&s_fill[g], 1u);
+                buf[g][at] = d;  // at < kProbeBuf: compacted below whenever 256 more could overflow
+            }
         }
         __syncthreads();
-        // Every thread reads the fill before any thread appends again.
-        const uint32_t f = s_fill;
+        // Every thread reads the fills before any thread appends again.
+        uint32_t f[kProbeG];
+#pragma unroll
+        for (uint32_t g = 0; g < kProbeG; ++g) f[g] = s_fill[g];
         __syncthreads();
-        if (f + 64 > kSelBuf) compact(f);
+        for (uint32_t g = 0; g < kProbeG; ++g)
+            if (f[g] + 256 > kProbeBuf) compact(g, f[g]);
     }
-    compact(s_fill);
+    for (uint32_t g = 0; g < kProbeG; ++g) compact(g, s_fill[g]);
+    for (uint32_t g = 0; g < kProbeG; ++g) {
+        const uint32_t s = blockIdx.x * kProbeG + g;
+        if (s >= ns) break;
+        float* o = out + (size_t(s) * gridDim.y + blockIdx.y) * kProbeK;
+        for (uint32_t i = threadIdx.x; i < kProbeK; i += blockDim.x) o[i] = buf[g][i];
+    }
+}
+
+constexpr uint32_t kMergeBuf = 8192;  // slices * kProbeK values merged per sample (slices <= 64)
+
+// Per sample: the best kProbeK of its `slices` partial rows (slices *
+// kProbeK <= kMergeBuf values), ascending.
+__global__ __launch_bounds__(256) void probe_merge_kernel(const float* __restrict__ part, uint32_t slices,
+                                                          float* __restrict__ out) {
+    __shared__ float buf[kMergeBuf];
+    const uint32_t s = blockIdx.x, f = slices * kProbeK, P = pow2_at_least(f);
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) buf[i] = i < f ? part[size_t(s) * f + i] : INFINITY;
+    __syncthreads();
+    block_bitonic(buf, P);
     for (uint32_t i = threadIdx.x; i < kProbeK; i += blockDim.x) out[size_t(s) * kProbeK + i] = buf[i];
 }
 
@@ -298,12 +402,29 @@ int artea_probe_knn_device(int metric, const float* d_base, uint32_t n, uint32_t
     uint32_t* d_s = nullptr;
     rc = upload_samples(samples, num_samples, n, s, &d_s, "artea_probe_knn_device");
     if (rc) return rc;
+    // Slices: enough blocks for several rounds over the chip (>= 4096, so the
+    // last round's tail is short), slices of at least 4096 rows, at most
+    // kMergeBuf / kProbeK of them per sample.
+    const uint32_t groups = (num_samples + kProbeG - 1) / kProbeG;
+    uint32_t slices = 1;
+    while (slices < kMergeBuf / kProbeK && uint64_t(groups) * slices < 4096 && n / (2 * slices) >= 4096) slices *= 2;
+    const uint32_t slice = (n + slices - 1) / slices;
     float* d_out = nullptr;
-    hipError_t e = hipMalloc(&d_out, size_t(num_samples) * kProbeK * 4);
+    const size_t out_bytes = size_t(num_samples) * kProbeK * 4;
+    hipError_t e = hipMalloc(&d_out, out_bytes * (slices > 1 ? 1 + slices : 1));
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(probe_knn_kernel, dim3(num_samples), dim3(256), 0, s, metric, d_base, n, dim_pad, d_s,
-                           d_out);
+        float* part = slices > 1 ? d_out + size_t(num_samples) * kProbeK : d_out;
+        auto* kern = metric == kL2Sqr ? probe_knn_kernel<kL2Sqr>
+                     : metric == kDot  ? probe_knn_kernel<kDot>
+                     : metric == kCosine ? probe_knn_kernel<kCosine>
+                                         : probe_knn_kernel<kL2>;
+        hipLaunchKernelGGL(kern, dim3(groups, slices), dim3(256), 0, s, d_base, n, dim_pad, d_s, num_samples, slice,
+                           part);
         e = hipGetLastError();
+        if (e == hipSuccess && slices > 1) {
+            hipLaunchKernelGGL(probe_merge_kernel, dim3(num_samples), dim3(256), 0, s, part, slices, d_out);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess)
         e = hipMemcpyAsync(knn, d_out, size_t(num_samples) * kProbeK * 4, hipMemcpyDeviceToHost, s);

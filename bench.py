@@ -258,6 +258,74 @@ def workload_c0(args):
     print(json.dumps(out))
 
 
+def workload_probe(args):
+    """The L0 minimum-distance auto-probe that precedes a build
+    (DatasetProber::probe with the reference CLI's defaults, 500 samples and
+    quantile 0.9: each sample's 128 nearest distances over the whole base) on
+    the 10,000,000 x 96 unit-norm mixture, L2.  One step = one probe call
+    (sample upload, the sliced scan, the merge, the 256 KB download); the
+    roofline counts 3 FP32 operations per coordinate (subtract, fused
+    multiply-add) against the 157.3 TFLOP/s FP32 peak."""
+    import artea_amd as aa
+    from artea_amd import estimators as E
+    from artea_amd.device import DeviceBuffer, Event, Stream
+
+    n, dim, seed, ns, quantile = args.n, 96, 20261203 + 3, 500, 0.9
+    t0 = time.time()
+    base = aa.generate(2, seed, 0, n, dim)
+    log("generated %d x %d in %.1fs" % (n, dim, time.time() - t0))
+    stream = Stream()
+    d = DeviceBuffer.from_numpy(base)
+    samples = np.random.default_rng(seed).integers(0, n, ns).astype(np.uint32)
+    for _ in range(max(args.warmup, 1)):
+        knn, mn = E.knn_table(aa.DistanceMetrics.EUCLIDEAN_SQR, d, n, dim, samples, stream)
+    e0, e1 = Event(), Event()
+    stream.synchronize()
+    t0 = time.time()
+    e0.record(stream)
+    for _ in range(args.steps):
+        knn, mn = E.knn_table(aa.DistanceMetrics.EUCLIDEAN_SQR, d, n, dim, samples, stream)
+    e1.record(stream)
+    stream.synchronize()
+    wall = (time.time() - t0) / args.steps
+    ev = e0.elapsed_ms(e1) / args.steps / 1e3
+    table = E.quantile_columns(knn, [quantile])
+    flops = float(ns) * (n - 1) * dim * 3
+    achieved = flops / ev / 1e12
+    out = {"metric": "L0 minimum-distance probe (500 samples x 10M x 96 L2, top-128 each)", "value": round(ns / wall, 1),
+           "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (1024-component mixture, unit-norm)",
+           "config": {"workload": "L0 auto-probe, %d x 96 L2, %d samples, quantile %.1f" % (n, ns, quantile), "n": n,
+                      "samples": ns},
+           "l0_min_distance": table[0][0], "min_nearest": mn,
+           "roofline": {"bound": "fp32 valu", "achieved": round(achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": round(achieved / 157.3, 4), "traffic": None,
+                        "span": "HIP events around each call (kernels plus its small copies and allocation gaps)"}}
+    log("probe: %.2f ms per call (events %.2f ms), %.1f TFLOP/s, l0_min_distance %.6f" % (wall * 1e3, ev * 1e3,
+                                                                                       achieved, table[0][0]))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        done, same, t0 = 0, True, time.time()
+        while time.time() - t0 < args.cpu_seconds or done == 0:
+            sl = samples[done % ns:done % ns + cores]
+            ck = orc.probe_knn(orc.L2SQR, base, sl, nthreads=cores)
+            same = same and bool(np.array_equal(ck, knn[done % ns:done % ns + len(sl)]))
+            done += len(sl)
+        rate = done / (time.time() - t0)
+        out["cpu_baseline"] = {"value": round(rate, 2), "unit": "samples/s", "cores": cores, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
+                               "sample": "%d of the same samples (%d per batch, one per thread), oracle restatement "
+                                         "of _compute_knn_table" % (done, cores),
+                               "results_equal_gpu": same}
+        log("probe CPU oracle: %.2f samples/s on %d cores, rows equal %s" % (rate, cores, same))
+    print(json.dumps(out))
+
+
 def workload_build(args):
     """10,000,000 x 96 unit-norm 1024-component mixture, L2: the hierarchy
     build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) with sparse
@@ -424,7 +492,7 @@ def main():
                     help="triangle-join pair distances of the search workload's build (the graph is the same)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
-    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0"), default="search",
+    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe"), default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 hierarchy build (every level) FP32 vs BF16 join, then serving on that index; "
                          "c0: 100K x 128 build and search, GPU vs the CPU oracle")
@@ -465,6 +533,10 @@ def main():
         return workload_cosine(args)
     if args.workload == "c0":
         return workload_c0(args)
+    if args.workload == "probe":
+        if args.n == 1_000_000:
+            args.n = 10_000_000
+        return workload_probe(args)
     if args.workload == "build":
         if args.n == 1_000_000:
             args.n = 10_000_000
