@@ -101,6 +101,11 @@ SIGNATURES = {
     "artea_refine_set_recheck_queue": (i32, [P, u32]),
     "artea_refine_last_stats": (i32, [P, P, P, P]),
     "artea_build_full": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, P]),
+    "artea_build_full_routing": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, u32, u32, u32,
+                                       P]),
+    "artea_refine_routing_loops": (i32, [P, u32, u32, u32, f32]),
+    "artea_refine_prune": (i32, [P, f32]),
+    "artea_refine_routing_stats": (i32, [P, P, P, P]),
     "artea_index_compact_graph": (i32, [P, u32, u32, u32, P, P, P, P, i32]),
     "artea_recall_at_k_device": (i32, [P, u32, P, u32, u32, u32, C.POINTER(u64), P]),
     "artea_adr_at_1_device": (i32, [i32, P, u32, P, u32, u32, P, u32, P, u32, C.POINTER(C.c_double),

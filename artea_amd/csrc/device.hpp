@@ -337,6 +337,62 @@ __device__ __forceinline__ float quad_norm2(const float* __restrict__ q, int C, 
     return reduce16_quad(a0, a1, a2, a3);
 }
 
+// The 16 accumulators of one AVX-512 register summed in the order of GCC's
+// _mm512_reduce_add_ps (the tree reduce16_quad spreads over a quad), in one
+// lane: t8[i] = a[i+8] + a[i], t4[i] = t8[i+4] + t8[i], then
+// (t4[0] + t4[2]) + (t4[1] + t4[3]).
+__device__ __forceinline__ float reduce16_lane(const float (&a)[16]) {
+    float t8[8], t4[4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t8[i] = a[i + 8] + a[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t4[i] = t8[i + 4] + t8[i];
+    return (t4[0] + t4[2]) + (t4[1] + t4[3]);
+}
+
+// dist_func_t(a, b) (utils/simd_distance.hpp:64-79, 241-275) by one lane:
+// the 16 accumulator chains over chunks 0..C-1 in order, then the reduce
+// tree; the same bits as quad_distance.
+__device__ __forceinline__ float lane_distance(const float* __restrict__ a, const float* __restrict__ b, uint32_t C,
+                                               int metric) {
+    float acc[16], na[16], nb[16];
+#pragma unroll
+    for (int l = 0; l < 16; ++l) acc[l] = na[l] = nb[l] = 0.f;
+    for (uint32_t c = 0; c < C; ++c) {
+        float x[16], y[16];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float4 s = *reinterpret_cast<const float4*>(a + c * 16 + v * 4);
+            const float4 t = *reinterpret_cast<const float4*>(b + c * 16 + v * 4);
+            x[4 * v] = s.x, x[4 * v + 1] = s.y, x[4 * v + 2] = s.z, x[4 * v + 3] = s.w;
+            y[4 * v] = t.x, y[4 * v + 1] = t.y, y[4 * v + 2] = t.z, y[4 * v + 3] = t.w;
+        }
+        if (metric == kL2Sqr || metric == kL2) {
+#pragma unroll
+            for (int l = 0; l < 16; ++l) {
+                const float d = x[l] - y[l];
+                acc[l] = __builtin_fmaf(d, d, acc[l]);
+            }
+        } else {
+#pragma unroll
+            for (int l = 0; l < 16; ++l) acc[l] = __builtin_fmaf(x[l], y[l], acc[l]);
+            if (metric == kCosine) {
+#pragma unroll
+                for (int l = 0; l < 16; ++l) {
+                    na[l] = __builtin_fmaf(x[l], x[l], na[l]);
+                    nb[l] = __builtin_fmaf(y[l], y[l], nb[l]);
+                }
+            }
+        }
+    }
+    const float s = reduce16_lane(acc);
+    if (metric == kL2Sqr) return s;
+    if (metric == kL2) return __builtin_sqrtf(s);
+    if (metric == kDot) return -s;
+    const float denom = __builtin_sqrtf(reduce16_lane(na) * reduce16_lane(nb));
+    return denom > 0.0f ? 1.0f - s / denom : 1.0f;
+}
+
 // Total order of CandidateEntry / Neighbor (candidate_entry.hpp:170-180,
 // neighbor.hpp:247-258): distance, then vid.
 __device__ __forceinline__ bool key_less(float da, uint32_t va, float db, uint32_t vb) {

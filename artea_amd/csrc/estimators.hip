@@ -105,19 +105,6 @@ __global__ __launch_bounds__(256) void distance_rows_kernel(int metric, const fl
     if (ok && sub == 0) out[size_t(i) * k + j] = in ? d : INFINITY;
 }
 
-// The 16 accumulators of one AVX-512 register summed in the order of GCC's
-// _mm512_reduce_add_ps (the tree reduce16_quad spreads over a quad), in one
-// lane: t8[i] = a[i+8] + a[i], t4[i] = t8[i+4] + t8[i], then
-// (t4[0] + t4[2]) + (t4[1] + t4[3]).
-__device__ __forceinline__ float reduce16_lane(const float (&a)[16]) {
-    float t8[8], t4[4];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) t8[i] = a[i + 8] + a[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) t4[i] = t8[i + 4] + t8[i];
-    return (t4[0] + t4[2]) + (t4[1] + t4[3]);
-}
-
 constexpr uint32_t kProbeG = 8;       // samples per block of the k-NN probe
 constexpr uint32_t kProbeBuf = 1024;  // candidate buffer per sample (best kProbeK first)
 

@@ -26,42 +26,15 @@
 
 #include "common.hpp"
 #include "device.hpp"
+#include "refine_ctx.hpp"
 
 using namespace artea;
 using namespace artea::dev;
+using namespace artea::rf;
 
 namespace {
 
-constexpr uint32_t kNew = 0x80000000u;
-constexpr uint32_t kMask = 0x7FFFFFFFu;
-constexpr uint32_t kMaxCap = 128;      // triangle / merge LDS sizing
 constexpr uint32_t kSortN = 512;       // merge sort width
-
-struct Ctx {
-    const float* vecs;
-    uint32_t n_global, D, C;
-    int metric;
-    uint32_t n, cap;
-    const uint32_t* l2g;  // null = identity
-    const uint32_t* g2l;  // null = identity; 0xFFFFFFFF = not in layer
-    uint2* rows;          // n * cap
-    uint32_t* cnt;        // n
-    uint32_t* e_t;        // n * cap emission targets
-    uint2* e_vd;          // n * cap emission (vid, dist bits)
-    uint32_t* e_cnt;      // n
-    uint32_t* t_cnt;      // n + 1
-    uint32_t* t_off;      // n + 1
-    uint32_t* t_fill;     // n
-    uint2* seg;           // n * cap
-    unsigned long long* stats;  // [0] logs, [1] FP32 pair distances, [2] BF16 pair distances
-    uint32_t list_cap;          // triangle recheck queue entries per row (<= kListCap)
-    uint32_t rb, re;            // rows the triangle / reverse producers run on (next_range); [0, n) otherwise
-};
-
-__device__ __forceinline__ uint32_t storage_of(const Ctx& c, uint32_t local) { return c.l2g ? c.l2g[local] : local; }
-__device__ __forceinline__ uint32_t local_of(const Ctx& c, uint32_t g) {
-    return c.g2l ? c.g2l[g] : (g < c.n ? g : 0xFFFFFFFFu);
-}
 
 // The same counter-based draw as the oracle: SplitMix64 finaliser chained
 // over (seed, stream, row, j), mapped to [lo, hi) by multiply-high.
@@ -1029,6 +1002,9 @@ struct artea_refine {
     uint32_t* d_g2l = nullptr;
     int dev_cus = 0;
     int join = ARTEA_JOIN_FP32;
+    // routing loops since the last artea_refine_routing_stats: distances,
+    // expansions, rows whose candidate list still dropped a tied entry
+    unsigned long long route_stats[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1158,6 +1134,83 @@ int do_schedule(artea_refine* r, uint32_t loops, uint32_t triu, float scale, flo
             if (rc) return rc;
         }
     }
+    return ARTEA_OK;
+}
+
+// The reverse logs of a routing pass (counted by routing_finish) merged into
+// the rows: merge_logs with the pass's log buffer; the merge keeps the first
+// cap entries, i.e. the arc updater that follows.
+int merge_routing_reverse(artea_refine* r, const RoutingPass& p, uint2* seg) {
+    Ctx c = r->c;
+    c.seg = seg;
+    hipStream_t s = r->s;
+    ARTEA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(r->scan_tmp, r->scan_bytes, c.t_cnt, c.t_off, c.n + 1, s));
+    ARTEA_HIP_TRY(routing_scatter(c, p, seg, s));
+    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 1, 8)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(count_logs_kernel, dim3(1), dim3(64), 0, s, c.t_off, c.n, c.stats);
+    ARTEA_HIP_TRY(hipGetLastError());
+    ARTEA_HIP_TRY(hipMemsetAsync(c.t_cnt, 0, sizeof(uint32_t) * (c.n + 1), s));
+    ARTEA_HIP_TRY(hipMemsetAsync(c.t_fill, 0, sizeof(uint32_t) * c.n, s));
+    return ARTEA_OK;
+}
+
+int check_routing(const artea_refine* r, uint32_t topk, uint32_t queue, float scale, const char* who) {
+    ARTEA_CHECK_ARG(r, std::string(who) + ": null handle");
+    ARTEA_CHECK_ARG(topk >= 1 && topk <= 128, std::string(who) + ": routing topk must be in [1, 128]");
+    if (queue < topk) {
+        set_error(std::string(who) + ": candidate_queue_size (" + std::to_string(queue) + ") must be >= topk (" +
+                  std::to_string(topk) + ")");
+        return ARTEA_EINVAL;
+    }
+    ARTEA_CHECK_ARG(queue <= 256, std::string(who) + ": routing queue must be <= 256");
+    ARTEA_CHECK_ARG(scale > 0.0f, std::string(who) + ": scale must be positive");
+    return ARTEA_OK;
+}
+
+// The routing loops of refine_layer: next(routing).next(pruning), then
+// next(reverse).next(arc) between loops.  Each routing pass merges and
+// prunes every row into ext (up to cap + topk entries); the rows then take
+// their first cap entries (after the last loop: the writeback keeps the
+// first slots anyway), and between loops the reverse logs of the full ext
+// rows are merged into them.
+int do_routing_loops(artea_refine* r, uint32_t loops, uint32_t topk, uint32_t queue, float scale) {
+    Ctx& c = r->c;
+    if (loops == 0 || c.n == 0) return ARTEA_OK;
+    RoutingPass p{};
+    p.topk = topk;
+    p.queue = queue;
+    p.inv_scale = 1.0f / scale;
+    p.ext_stride = c.cap + topk;
+    const size_t slots = size_t(c.n) * p.ext_stride;
+    uint2* seg = nullptr;
+    hipError_t e = hipMalloc(&p.ext, slots * sizeof(uint2));
+    if (e == hipSuccess) e = hipMalloc(&p.ext_cnt, size_t(c.n) * 4);
+    if (e == hipSuccess) e = hipMalloc(&p.stats, kRouteStatsBytes);
+    if (e == hipSuccess && loops > 1) e = hipMalloc(&seg, slots * sizeof(uint2));
+    struct Free {
+        void* q[4];
+        ~Free() {
+            for (void* x : q)
+                if (x) (void)hipFree(x);
+        }
+    } guard{{p.ext, p.ext_cnt, p.stats, seg}};
+    ARTEA_HIP_TRY(e);
+    for (uint32_t l = 0; l < loops; ++l) {
+        const bool last = l + 1 == loops;
+        uint32_t unresolved = 0;
+        ARTEA_HIP_TRY(routing_pass(c, p, r->s, r->dev_cus, &unresolved));
+        unsigned long long h[2];
+        ARTEA_HIP_TRY(hipMemcpy(h, p.stats, sizeof h, hipMemcpyDeviceToHost));
+        r->route_stats[0] += h[0];
+        r->route_stats[1] += h[1];
+        r->route_stats[2] += unresolved;
+        ARTEA_HIP_TRY(routing_finish(c, p, !last, r->s));
+        if (!last) {
+            int rc = merge_routing_reverse(r, p, seg);
+            if (rc) return rc;
+        }
+    }
+    ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
     return ARTEA_OK;
 }
 
@@ -1601,12 +1654,53 @@ int artea_refine_last_stats(artea_refine* r, uint64_t* logs, uint64_t* exact_pai
     return ARTEA_OK;
 }
 
+int artea_refine_routing_loops(artea_refine* r, uint32_t loops, uint32_t topk, uint32_t queue, float scale_coeffs) {
+    clear_error();
+    int rc = check_routing(r, topk, queue, scale_coeffs, "artea_refine_routing_loops");
+    if (rc) return rc;
+    return do_routing_loops(r, loops, topk, queue, scale_coeffs);
+}
+
+int artea_refine_prune(artea_refine* r, float scale_coeffs) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_prune: null handle");
+    ARTEA_CHECK_ARG(scale_coeffs > 0.0f, "artea_refine_prune: scale must be positive");
+    ARTEA_HIP_TRY(prune_rows(r->c, 1.0f / scale_coeffs, r->s, r->dev_cus));
+    ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+    return ARTEA_OK;
+}
+
+int artea_refine_routing_stats(artea_refine* r, uint64_t* distances, uint64_t* expansions, uint64_t* unresolved) {
+    clear_error();
+    ARTEA_CHECK_ARG(r, "artea_refine_routing_stats: null handle");
+    if (distances) *distances = r->route_stats[0];
+    if (expansions) *expansions = r->route_stats[1];
+    if (unresolved) *unresolved = r->route_stats[2];
+    r->route_stats[0] = r->route_stats[1] = r->route_stats[2] = 0;
+    return ARTEA_OK;
+}
+
 int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level, uint32_t build_loops,
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
                      float l0_min_distance, uint64_t seed, int join_precision, float level_fraction,
                      double* seconds_per_level) {
+    return artea_build_full_routing(idx, ul_max, bl_max, max_level, build_loops, triu_iters, prefill_ratio,
+                                    scale_coeffs, shifted_coeffs, l0_min_distance, seed, join_precision,
+                                    level_fraction, 0, 64, 96, seconds_per_level);
+}
+
+int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level,
+                             uint32_t build_loops, uint32_t triu_iters, float prefill_ratio, float scale_coeffs,
+                             float shifted_coeffs, float l0_min_distance, uint64_t seed, int join_precision,
+                             float level_fraction, uint32_t routing_loops, uint32_t routing_topk,
+                             uint32_t routing_queue, double* seconds_per_level) {
     clear_error();
     ARTEA_CHECK_ARG(idx, "artea_build_full: null index");
+    if (routing_loops > 0) {
+        ARTEA_CHECK_ARG(routing_topk >= 1 && routing_topk <= 128, "artea_build_full: routing topk must be in [1, 128]");
+        ARTEA_CHECK_ARG(routing_queue >= routing_topk && routing_queue <= 256,
+                        "artea_build_full: routing queue must be in [topk, 256]");
+    }
     ARTEA_CHECK_ARG(join_precision == ARTEA_JOIN_FP32 || join_precision == ARTEA_JOIN_BF16,
                     "artea_build_full: join_precision must be ARTEA_JOIN_FP32 or ARTEA_JOIN_BF16");
     ARTEA_CHECK_ARG(max_level < 64, "artea_build_full: max_level must be < 64");
@@ -1662,6 +1756,8 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
         }
         const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
         rc = do_schedule(r, build_loops, triu_iters, scale_coeffs, shift);
+        if (rc) return rc;
+        rc = do_routing_loops(r, routing_loops, routing_topk, routing_queue, scale_coeffs);
         if (rc) return rc;
         // writeback_layer_from_single_layer_graph: first slot_cap entries.
         std::vector<uint2> rows(std::max<size_t>(size_t(nl) * cap, 1));

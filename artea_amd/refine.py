@@ -89,6 +89,20 @@ class PropagateEngine:
     def schedule(self, build_loops, triu_iters, scale_coeffs, effective_shift):
         check(lib().artea_refine_schedule(self.h, build_loops, triu_iters, scale_coeffs, effective_shift))
 
+    def routing_loops(self, loops, topk, queue, scale_coeffs):
+        """refine_layer's routing loops: routing + pruning, reverse + arc
+        between loops (index/artea_graph/index_factory.hpp:400-404)."""
+        check(lib().artea_refine_routing_loops(self.h, loops, topk, queue, scale_coeffs))
+
+    def next_pruning(self, scale_coeffs):
+        """next(PruningUpdater): rows pruned in place, no logs."""
+        check(lib().artea_refine_prune(self.h, scale_coeffs))
+
+    def routing_stats(self):
+        a, b, d = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().artea_refine_routing_stats(self.h, C.byref(a), C.byref(b), C.byref(d)))
+        return {"distances": a.value, "expansions": b.value, "unresolved": d.value}
+
     def last_stats(self):
         a, b, d = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(lib().artea_refine_last_stats(self.h, C.byref(a), C.byref(b), C.byref(d)))
@@ -106,14 +120,17 @@ def max_level_for(n: int) -> int:
 
 
 def build_full(index, ul_max=32, bl_max=64, max_level=None, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-               shift=1.5, l0_min_distance=1.0, seed=0, join=JOIN_FP32, level_fraction=1.0):
+               shift=1.5, l0_min_distance=1.0, seed=0, join=JOIN_FP32, level_fraction=1.0, routing_loops=0,
+               routing_topk=64, routing_queue=96):
     """Hierarchy build on the GPU; attaches the graph to ``index``.
     ``join`` picks the triangle join's pair-distance path (the graph is the
     same either way); ``level_fraction`` < 1 draws each vertex's level
     geometrically (sparse upper layers) instead of putting every vertex on
-    every level.  Returns seconds per level."""
+    every level; ``routing_loops`` routing loops (routing topk and queue)
+    follow each level's build loops.  Returns seconds per level."""
     ml = max_level_for(index.n) if max_level is None else max_level
     secs = np.zeros(ml + 1, np.float64)
-    check(lib().artea_build_full(index.h, ul_max, bl_max, ml, build_loops, triu_iters, prefill, scale, shift,
-                                 l0_min_distance, seed, join, level_fraction, _ptr(secs)))
+    check(lib().artea_build_full_routing(index.h, ul_max, bl_max, ml, build_loops, triu_iters, prefill, scale, shift,
+                                         l0_min_distance, seed, join, level_fraction, routing_loops, routing_topk,
+                                         routing_queue, _ptr(secs)))
     return secs

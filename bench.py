@@ -326,6 +326,93 @@ def workload_probe(args):
     print(json.dumps(out))
 
 
+def workload_routing(args):
+    """One routing loop of refine_layer (RoutingUpdater topk 64 / queue 96,
+    merge, PruningUpdater scale 1.10) on the L0 layer of the 1,000,000 x 128
+    L2 workload: cap 96 (bl_max 64 x 1.5), rows from prefill 0.4 and the 15 x 4
+    build loops (shift 1.5).  One step = one routing loop over every row,
+    restored to the same rows before each step.  The roofline counts 512 bytes
+    (one FP32 row) per distance evaluated (router and pruning pairs)."""
+    import artea_amd as aa
+    from artea_amd.device import Event, Stream
+    from artea_amd.refine import PropagateEngine
+
+    n, dim, seed, cap, topk, queue = args.n, 128, 20261204, 96, 64, 96
+    vecs = aa.generate(0, seed, 0, n, dim)
+    idx = aa.DeviceIndex(vecs, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    eng = PropagateEngine(idx, cap=cap, metric=aa.DistanceMetrics.EUCLIDEAN, seed=seed)
+    t0 = time.time()
+    eng.next_prefill(int(np.float32(cap) * np.float32(0.4)), 0, n, stream=0)
+    eng.schedule(15, 4, 1.10, 1.5)
+    log("L0 build loops: %.1fs" % (time.time() - t0))
+    counts, ents = eng.download()
+    for _ in range(max(args.warmup, 1)):
+        eng.upload(counts, ents)
+        eng.routing_loops(1, topk, queue, 1.10)
+    eng.routing_stats()
+    secs = []
+    for _ in range(args.steps):
+        eng.upload(counts, ents)
+        t0 = time.time()
+        eng.routing_loops(1, topk, queue, 1.10)
+        secs.append(time.time() - t0)
+    st = eng.routing_stats()
+    out_c, out_e = eng.download()
+    step = float(np.median(secs))
+    dist_per_step = st["distances"] / args.steps
+    achieved = dist_per_step * 512 / step / 1e9
+    out = {"metric": "routing loop (RoutingUpdater + PruningUpdater) rows/s, L0 of 1M x 128 L2",
+           "value": round(n / step, 1), "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(step * 1e3, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (integer-rounded 256-component mixture)",
+           "config": {"workload": "one routing loop, L0 of %d x 128 L2, cap 96, topk 64, queue 96" % n, "n": n},
+           "distances_per_row": round(dist_per_step / n, 1), "expansions_per_row": round(st["expansions"] /
+                                                                                         args.steps / n, 1),
+           "unresolved_rows": st["unresolved"], "mean_row_after": round(float(out_c.mean()), 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "span": "host clock around each synchronous call (routing pass, rerun check, row copy)"}}
+    log("routing loop: %.1f ms, %.0f rows/s, %.0f distances/row, %.1f GB/s by 512 B per distance" % (
+        step * 1e3, n / step, dist_per_step / n, achieved))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        # Every sampled pivot must route over the rows as they were before
+        # the pass: a pilot on one copy sizes the sample, one call on a
+        # fresh copy is timed.
+        entries = ents.view(orc.NBR_DTYPE).reshape(-1)
+        L = orc.Layer(vecs, orc.L2, cap)
+        L.set_rows_raw(counts, entries)
+        t0 = time.time()
+        L.routing_range(topk, queue, 0, 4 * cores, nthreads=cores)
+        pilot = 4 * cores / max(time.time() - t0, 1e-6)
+        rows = int(min(n, max(4 * cores, pilot * args.cpu_seconds)))
+        L = orc.Layer(vecs, orc.L2, cap)
+        L.set_rows_raw(counts, entries)
+        t0 = time.time()
+        L.routing_range(topk, queue, 0, rows, nthreads=cores)
+        rate = rows / (time.time() - t0)
+        # the pruned rows of the sampled pivots against the GPU's
+        L.pruning(1.10)
+        same = True
+        oc, oe = L.export()
+        off = np.concatenate([[0], np.cumsum(oc)]).astype(np.int64)
+        goff = np.concatenate([[0], np.cumsum(out_c)]).astype(np.int64)
+        for r in range(0, rows, 97):
+            want = oe[off[r]:off[r] + min(int(oc[r]), cap)].view(np.uint32).reshape(-1, 2)
+            same = same and np.array_equal(want, out_e[goff[r]:goff[r + 1]])
+        out["cpu_baseline"] = {"value": round(rate, 1), "unit": "rows/s", "cores": cores, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
+                               "sample": "routing of the first %d pivot rows (oracle restatement, OpenMP), then the "
+                                         "pruning of every row" % rows,
+                               "results_equal_gpu": bool(same)}
+        log("routing CPU oracle: %.0f rows/s on %d cores, sampled rows equal %s" % (rate, cores, same))
+    print(json.dumps(out))
+
+
 def workload_build(args):
     """10,000,000 x 96 unit-norm 1024-component mixture, L2: the hierarchy
     build (15 x 4 schedule, prefill 0.4, scale 1.10, shift 1.5) with sparse
@@ -492,7 +579,7 @@ def main():
                     help="triangle-join pair distances of the search workload's build (the graph is the same)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
-    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe"), default="search",
+    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing"), default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 hierarchy build (every level) FP32 vs BF16 join, then serving on that index; "
                          "c0: 100K x 128 build and search, GPU vs the CPU oracle")
@@ -533,6 +620,8 @@ def main():
         return workload_cosine(args)
     if args.workload == "c0":
         return workload_c0(args)
+    if args.workload == "routing":
+        return workload_routing(args)
     if args.workload == "probe":
         if args.n == 1_000_000:
             args.n = 10_000_000

@@ -296,6 +296,28 @@ int artea_refine_reverse(artea_refine* r);
  * build_loops x (triu_iters x (triangle, arc), reverse except last, arc). */
 int artea_refine_schedule(artea_refine* r, uint32_t build_loops, uint32_t triu_iters, float scale_coeffs,
                           float effective_shift);
+/* refine_layer's routing loops (index/artea_graph/index_factory.hpp:378-403):
+ * `loops` x (next(RoutingUpdater(topk, queue)), next(PruningUpdater(scale)))
+ * with next(ReverseUpdater), next(arc) between loops.  RoutingUpdater
+ * (refiner/updaters/routing_updater.hpp:84-114): per row, the warm-start
+ * beam search of SingleLayerRouter (router/single_layer_router.hpp:259-282;
+ * queue capacity max(topk, queue), seeded with the row) over the layer's
+ * rows; its results except the row's own vertex, and except those not
+ * closer than the row's entry cap - 1 when the row is full, are merged into
+ * the row (no degree cap).  PruningUpdater (pruning_updater.hpp:63-177):
+ * scaled rule, pairs of two old entries skipped, survivors old.  Rows keep
+ * their first cap entries at the end (the writeback's truncation).  topk in
+ * [1, 128], queue in [topk, 256] (the reference requires queue >= topk and
+ * fails with "candidate_queue_size (q) must be >= topk (k)"). */
+int artea_refine_routing_loops(artea_refine* r, uint32_t loops, uint32_t topk, uint32_t queue, float scale_coeffs);
+/* next(PruningUpdater(scale)) alone (refiner/updaters/pruning_updater.hpp
+ * :170-177): every row pruned in place, no logs. */
+int artea_refine_prune(artea_refine* r, float scale_coeffs);
+/* Since the last call: routing distances (router and pruning pairs),
+ * router expansions, and rows whose candidate list dropped an entry tied with
+ * its bound even with 512 entries (their results may differ from the
+ * reference's; none expected on real data). */
+int artea_refine_routing_stats(artea_refine* r, uint64_t* distances, uint64_t* expansions, uint64_t* unresolved);
 /* Precision of the triangle join's pair distances (no reference
  * counterpart; the reference's join is FP32 only).  ARTEA_JOIN_FP32 (the
  * default): every needed pair by dist_func_t's FP32 chain.  ARTEA_JOIN_BF16:
@@ -350,6 +372,15 @@ int artea_build_full(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_
                      uint32_t triu_iters, float prefill_ratio, float scale_coeffs, float shifted_coeffs,
                      float l0_min_distance, uint64_t seed, int join_precision, float level_fraction,
                      double* seconds_per_level);
+/* The same with `routing_loops` routing loops (routing topk and queue) after
+ * each level's build loops (PropagateConfig's num_routing_loops,
+ * routing_topk, routing_queue_size; index/conv_graph/configs.hpp:33-77,
+ * defaults 0, 64, 96); artea_build_full = routing_loops 0. */
+int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max, uint32_t max_level,
+                             uint32_t build_loops, uint32_t triu_iters, float prefill_ratio, float scale_coeffs,
+                             float shifted_coeffs, float l0_min_distance, uint64_t seed, int join_precision,
+                             float level_fraction, uint32_t routing_loops, uint32_t routing_topk,
+                             uint32_t routing_queue, double* seconds_per_level);
 
 /* -------------------------------------------------------------- utilities */
 /* dist_func_t on the device (utils/simd_distance.hpp:64-79): dists[i*nb+j]

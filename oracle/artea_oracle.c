@@ -1028,6 +1028,15 @@ void orc_layer_set_row_raw(orc_layer* L, uint32_t local, const orc_nbr* e, uint3
     for (uint32_t i = 0; i < count; ++i) nvec_push(r, e[i]);
 }
 
+/* Every row at once: counts[i] entries of `entries` (row-major), as given. */
+void orc_layer_set_rows_raw(orc_layer* L, const uint32_t* counts, const orc_nbr* entries) {
+    uint64_t pos = 0;
+    for (uint32_t i = 0; i < L->n_local; ++i) {
+        orc_layer_set_row_raw(L, i, entries + pos, counts[i]);
+        pos += counts[i];
+    }
+}
+
 uint32_t orc_layer_row_size(const orc_layer* L, uint32_t local) { return L->rows[local].n; }
 
 uint32_t orc_layer_get_row(const orc_layer* L, uint32_t local, orc_nbr* out, uint32_t max_out) {
@@ -1260,6 +1269,143 @@ void orc_refine_schedule(orc_layer* L, uint32_t build_loops, uint32_t triu_iters
     }
 }
 
+/* ======================================================================
+ * Routing loops: RoutingUpdater then PruningUpdater, with reverse + ARC
+ * between loops (index/artea_graph/index_factory.hpp:378-403).
+ * ====================================================================== */
+
+/* SingleLayerRouter::_query_impl(seed_nbrs) (router/single_layer_router.hpp
+ * :259-282): the pivot's own row seeds the queue (row order, distances from
+ * the row, each seed marked visited), then beam_loop_body
+ * (router/detail/beam_loop.hpp:90-119) over the layer's current rows
+ * (DynamicFlatRange: a row's entries until the first invalid one), then
+ * extract_results(topk).  The pivot itself is not marked visited. */
+static uint32_t route_warm(const orc_layer* L, uint32_t ri, uint32_t topk, uint32_t queue, orc_queue* q, vtt_t* vis,
+                           uint32_t* ov, float* od) {
+    vtt_clear(vis);
+    queue_clear(q);
+    q->capacity = topk > queue ? topk : queue;
+    const nvec* row = &L->rows[ri];
+    const float* pv = vec_of(L, storage_vid(L, ri));
+    for (uint32_t i = 0; i < row->n; ++i) {
+        const uint32_t sv = row->a[i].vid_status & ORC_INVALID_VID;
+        if (sv == ORC_INVALID_VID) break;
+        if (vtt_test_and_set(vis, sv)) continue;
+        queue_try_push(q, sv, row->a[i].dist);
+    }
+    while (q->U.n != 0) {
+        if (queue_should_terminate(q)) break;
+        const centry cur = heap_pop(&q->U);
+        if (cur.vid == ORC_INVALID_VID) break;
+        const uint32_t lr = local_of(L, cur.vid);
+        if (lr == ORC_INVALID_VID || lr >= L->n_local) continue;
+        const nvec* nb = &L->rows[lr];
+        for (uint32_t j = 0; j < nb->n; ++j) {
+            const uint32_t u = nb->a[j].vid_status & ORC_INVALID_VID;
+            if (u == ORC_INVALID_VID) break;
+            if (vtt_test_and_set(vis, u)) continue;
+            queue_try_push(q, u, orc_distance(L->metric, pv, vec_of(L, u), L->D));
+        }
+    }
+    const uint32_t k = topk < q->T.n ? topk : q->T.n;
+    return k ? orc_queue_extract(q, k, ov, od) : 0;
+}
+
+/* RoutingUpdater::update_impl (refiner/updaters/routing_updater.hpp:84-114):
+ * every result except the pivot itself is logged into the pivot's row,
+ * unless the row is full and its last entry (position max_nbr_size - 1) is
+ * not farther than the result; then merge_logs (no degree cap). */
+void orc_next_routing(orc_layer* L, uint32_t topk, uint32_t queue, int nthreads) {
+    orc_next_routing_range(L, topk, queue, 0, L->n_local, nthreads);
+}
+
+/* The same updater on the pivot rows [row_begin, row_end) only
+ * (next_range, refiner/propagate_engine.hpp:217-225). */
+void orc_next_routing_range(orc_layer* L, uint32_t topk, uint32_t queue, uint32_t row_begin, uint32_t row_end,
+                            int nthreads) {
+    const int T = pick_threads(nthreads);
+    if (row_end > L->n_local) row_end = L->n_local;
+    if (row_begin > row_end) row_begin = row_end;
+#pragma omp parallel num_threads(T)
+    {
+        orc_queue q;
+        vtt_t vis;
+        queue_init(&q, topk > queue ? topk : queue);
+        vtt_init(&vis, L->n_global);
+        const uint32_t kk = topk ? topk : 1;
+        uint32_t* ov = (uint32_t*)malloc(sizeof(uint32_t) * kk);
+        float* od = (float*)malloc(sizeof(float) * kk);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t ri = row_begin; ri < (int64_t)row_end; ++ri) {
+            const uint32_t sv = storage_vid(L, (uint32_t)ri);
+            const uint32_t m = route_warm(L, (uint32_t)ri, topk, queue, &q, &vis, ov, od);
+            const nvec* row = &L->rows[ri];
+            for (uint32_t i = 0; i < m; ++i) {
+                if (ov[i] == ORC_INVALID_VID || ov[i] == sv) continue;
+                if (row->n >= L->cap && row->a[L->cap - 1].dist <= od[i]) continue;
+                write_log(L, (uint32_t)ri, ov[i], od[i]);
+            }
+        }
+        free(ov);
+        free(od);
+        queue_release(&q);
+        free(vis.tags);
+    }
+    merge_logs(L, nthreads);
+}
+
+/* PruningUpdater::update_impl (refiner/updaters/pruning_updater.hpp:170-177)
+ * = CandidatePruningUpdater<scaled_ineq, SkipOldPairs = true> (:63-102):
+ * entries in row order; a candidate is dropped when some retained entry r
+ * (pairs where both are old skipped) has dist(candidate, r) <
+ * candidate.dist * (1 / scale); survivors marked old.  No logs. */
+void orc_next_pruning(orc_layer* L, float scale_coeffs, int nthreads) {
+    const float inv = 1.0f / scale_coeffs;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(pick_threads(nthreads))
+    for (int64_t ri = 0; ri < (int64_t)L->n_local; ++ri) {
+        nvec* row = &L->rows[ri];
+        if (row->n == 0) continue;
+        orc_nbr* kept = (orc_nbr*)malloc(sizeof(orc_nbr) * row->n);
+        uint32_t nk = 0;
+        for (uint32_t i = 0; i < row->n; ++i) {
+            const orc_nbr c = row->a[i];
+            const int c_old = (c.vid_status & ORC_NBR_NEW) == 0;
+            const float thr = c.dist * inv;
+            const float* cv = vec_of(L, c.vid_status & ORC_INVALID_VID);
+            int keep = 1;
+            for (uint32_t k = 0; k < nk; ++k) {
+                if (c_old && (kept[k].vid_status & ORC_NBR_NEW) == 0) continue;
+                const float t = orc_distance(L->metric, cv, vec_of(L, kept[k].vid_status & ORC_INVALID_VID), L->D);
+                if (t < thr) {
+                    keep = 0;
+                    break;
+                }
+            }
+            if (keep) kept[nk++] = c;
+        }
+        for (uint32_t k = 0; k < nk; ++k) kept[k].vid_status &= ORC_INVALID_VID;
+        memcpy(row->a, kept, sizeof(orc_nbr) * nk);
+        row->n = nk;
+        free(kept);
+    }
+    merge_logs(L, nthreads);
+}
+
+/* The routing loops of refine_layer (artea_graph/index_factory.hpp:400-404):
+ * next(routing).next(pruning), then next(reverse).next(arc) between loops.
+ * Rows may end longer than the cap; the writeback keeps the first slots. */
+void orc_routing_loops(orc_layer* L, uint32_t loops, uint32_t topk, uint32_t queue, float scale_coeffs,
+                       int nthreads) {
+    for (uint32_t l = 0; l < loops; ++l) {
+        orc_next_routing(L, topk, queue, nthreads);
+        orc_next_pruning(L, scale_coeffs, nthreads);
+        if (l + 1 != loops) {
+            orc_next_reverse(L, nthreads);
+            orc_next_truncate(L, nthreads);
+        }
+    }
+}
+
 /* compute_max_allowed_level_id (index/stacked_rgraph/configs.hpp:241-249) */
 uint32_t orc_max_level(uint32_t n) {
     if (n == 0) return 1;
@@ -1299,6 +1445,17 @@ orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t D, int build_m
                           uint32_t bl_max, uint32_t max_level, uint32_t build_loops, uint32_t triu_iters,
                           float prefill_ratio, float scale_coeffs, float shifted_coeffs, float l0_min_distance,
                           uint64_t seed, float level_fraction, int nthreads, double* secs) {
+    return orc_build_full_routing(vecs, n, D, build_metric, ul_max, bl_max, max_level, build_loops, triu_iters,
+                                  prefill_ratio, scale_coeffs, shifted_coeffs, l0_min_distance, seed, level_fraction,
+                                  0, 64, 96, nthreads, secs);
+}
+
+/* The same with routing loops after each level's build loops. */
+orc_graph* orc_build_full_routing(const float* vecs, uint32_t n, uint32_t D, int build_metric, uint32_t ul_max,
+                                  uint32_t bl_max, uint32_t max_level, uint32_t build_loops, uint32_t triu_iters,
+                                  float prefill_ratio, float scale_coeffs, float shifted_coeffs,
+                                  float l0_min_distance, uint64_t seed, float level_fraction, uint32_t routing_loops,
+                                  uint32_t routing_topk, uint32_t routing_queue, int nthreads, double* secs) {
     uint32_t* highest = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
     uint32_t src_top = 0;
     for (uint32_t v = 0; v < n; ++v) {
@@ -1327,6 +1484,7 @@ orc_graph* orc_build_full(const float* vecs, uint32_t n, uint32_t D, int build_m
         if (g > 0 && nl > 0) orc_next_prefill(L, g, 0, nl, 0, nl, seed, l, nthreads);
         const float shift = l == 0 ? shifted_coeffs * l0_min_distance : 0.0f;
         orc_refine_schedule(L, build_loops, triu_iters, scale_coeffs, shift, nthreads);
+        orc_routing_loops(L, routing_loops, routing_topk, routing_queue, scale_coeffs, nthreads);
         for (uint32_t i = 0; i < nl; ++i) {
             const uint32_t c = L->rows[i].n < slot_cap ? L->rows[i].n : slot_cap;
             offs[l][members[i] + 1] = c;

@@ -222,6 +222,23 @@ void orc_bruteforce_knn(int metric, const float* base, uint32_t n, const float* 
                         uint32_t nq, uint32_t dim_pad, uint32_t k, uint32_t* out_vids,
                         double* out_dists, int nthreads);
 
+/* Routing loops (index/artea_graph/index_factory.hpp:378-403): the warm-start
+ * RoutingUpdater (refiner/updaters/routing_updater.hpp:84-114), the
+ * PruningUpdater (refiner/updaters/pruning_updater.hpp:63-177) and the loop
+ * schedule; orc_build_full_routing runs them after each level's build loops. */
+void orc_next_routing(orc_layer* L, uint32_t topk, uint32_t queue, int nthreads);
+void orc_next_routing_range(orc_layer* L, uint32_t topk, uint32_t queue, uint32_t row_begin, uint32_t row_end,
+                            int nthreads);
+void orc_layer_set_rows_raw(orc_layer* L, const uint32_t* counts, const orc_nbr* entries);
+void orc_next_pruning(orc_layer* L, float scale_coeffs, int nthreads);
+void orc_routing_loops(orc_layer* L, uint32_t loops, uint32_t topk, uint32_t queue, float scale_coeffs,
+                       int nthreads);
+orc_graph* orc_build_full_routing(const float* vecs, uint32_t n, uint32_t D, int build_metric, uint32_t ul_max,
+                                  uint32_t bl_max, uint32_t max_level, uint32_t build_loops, uint32_t triu_iters,
+                                  float prefill_ratio, float scale_coeffs, float shifted_coeffs,
+                                  float l0_min_distance, uint64_t seed, float level_fraction, uint32_t routing_loops,
+                                  uint32_t routing_topk, uint32_t routing_queue, int nthreads, double* secs);
+
 /* Scoring and dataset probes: RecallEstimator (utils/recall_estimator.hpp:
  * 54-137) as the total of per-query matches, ADREstimator::calculate_adr_at_1
  * (utils/adr_estimator.hpp), DatasetProber::_compute_knn_table and

@@ -106,6 +106,13 @@ def _declare(l):
     d("orc_rand_draw", u32, u64, u64, u32, u32, u32, u32)
     d("orc_bruteforce_knn", None, i32, P, u32, P, u32, u32, u32, P, P, i32)
     d("orc_build_full", P, P, u32, u32, i32, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, f32, i32, P)
+    d("orc_build_full_routing", P, P, u32, u32, i32, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, f32, u32,
+      u32, u32, i32, P)
+    d("orc_next_routing", None, P, u32, u32, i32)
+    d("orc_next_routing_range", None, P, u32, u32, u32, u32, i32)
+    d("orc_layer_set_rows_raw", None, P, P, P)
+    d("orc_next_pruning", None, P, f32, i32)
+    d("orc_routing_loops", None, P, u32, u32, u32, f32, i32)
     d("orc_build_level_of", u32, u64, u32, u32, f32)
     d("orc_max_level", u32, u32)
 
@@ -407,6 +414,23 @@ class Layer:
     def schedule(self, build_loops, triu_iters, scale, shift, nthreads=0):
         lib().orc_refine_schedule(self.h, build_loops, triu_iters, scale, shift, nthreads)
 
+    def routing(self, topk, queue, nthreads=0):
+        lib().orc_next_routing(self.h, topk, queue, nthreads)
+
+    def routing_range(self, topk, queue, row_begin, row_end, nthreads=0):
+        lib().orc_next_routing_range(self.h, topk, queue, row_begin, row_end, nthreads)
+
+    def set_rows_raw(self, counts, entries):
+        c = np.ascontiguousarray(counts, np.uint32)
+        e = np.ascontiguousarray(entries, NBR_DTYPE)
+        lib().orc_layer_set_rows_raw(self.h, _ptr(c), _ptr(e) if e.size else None)
+
+    def pruning(self, scale, nthreads=0):
+        lib().orc_next_pruning(self.h, scale, nthreads)
+
+    def routing_loops(self, loops, topk, queue, scale, nthreads=0):
+        lib().orc_routing_loops(self.h, loops, topk, queue, scale, nthreads)
+
     @property
     def last_logs(self):
         return lib().orc_layer_last_logs(self.h)
@@ -435,15 +459,18 @@ def max_level(n):
 
 
 def build_full(vecs, metric=L2, ul_max=32, bl_max=64, max_level_=None, build_loops=15, triu_iters=4,
-               prefill=0.4, scale=1.10, shift=1.5, l0_min_distance=1.0, seed=0, level_fraction=1.0, nthreads=0):
+               prefill=0.4, scale=1.10, shift=1.5, l0_min_distance=1.0, seed=0, level_fraction=1.0, nthreads=0,
+               routing_loops=0, routing_topk=64, routing_queue=96):
     """Hierarchy build: all vertices on every level (level_fraction 1) or the
-    seeded geometric level assignment; returns (Graph, seconds per level)."""
+    seeded geometric level assignment, then `routing_loops` routing loops per
+    level; returns (Graph, seconds per level)."""
     v = _f32(vecs)
     n = v.shape[0]
     ml = max_level(n) if max_level_ is None else max_level_
     secs = np.zeros(ml + 1, np.float64)
-    h = lib().orc_build_full(_ptr(v), n, v.shape[1], metric, ul_max, bl_max, ml, build_loops, triu_iters, prefill,
-                             scale, shift, l0_min_distance, seed, level_fraction, nthreads, _ptr(secs))
+    h = lib().orc_build_full_routing(_ptr(v), n, v.shape[1], metric, ul_max, bl_max, ml, build_loops, triu_iters,
+                                     prefill, scale, shift, l0_min_distance, seed, level_fraction, routing_loops,
+                                     routing_topk, routing_queue, nthreads, _ptr(secs))
     return Graph(h), secs
 
 
