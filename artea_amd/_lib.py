@@ -105,6 +105,7 @@ SIGNATURES = {
                                        P]),
     "artea_refine_routing_loops": (i32, [P, u32, u32, u32, f32]),
     "artea_refine_prune": (i32, [P, f32]),
+    "artea_refine_prune_range": (i32, [P, f32, u32, u32]),
     "artea_refine_routing_stats": (i32, [P, P, P, P]),
     "artea_index_compact_graph": (i32, [P, u32, u32, u32, P, P, P, P, i32]),
     "artea_recall_at_k_device": (i32, [P, u32, P, u32, u32, u32, C.POINTER(u64), P]),

@@ -1662,10 +1662,14 @@ int artea_refine_routing_loops(artea_refine* r, uint32_t loops, uint32_t topk, u
 }
 
 int artea_refine_prune(artea_refine* r, float scale_coeffs) {
+    return artea_refine_prune_range(r, scale_coeffs, 0, r ? r->c.n : 0);
+}
+
+int artea_refine_prune_range(artea_refine* r, float scale_coeffs, uint32_t row_begin, uint32_t row_end) {
     clear_error();
     ARTEA_CHECK_ARG(r, "artea_refine_prune: null handle");
     ARTEA_CHECK_ARG(scale_coeffs > 0.0f, "artea_refine_prune: scale must be positive");
-    ARTEA_HIP_TRY(prune_rows(r->c, 1.0f / scale_coeffs, r->s, r->dev_cus));
+    ARTEA_HIP_TRY(prune_rows(r->c, 1.0f / scale_coeffs, row_begin, row_end, r->s, r->dev_cus));
     ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
     return ARTEA_OK;
 }

@@ -66,8 +66,8 @@ hipError_t routing_pass(const Ctx& c, const RoutingPass& p, hipStream_t s, int d
 hipError_t routing_finish(const Ctx& c, const RoutingPass& p, bool reverse, hipStream_t s);
 // Places those reverse logs into `seg` at t_off (t_fill cleared).
 hipError_t routing_scatter(const Ctx& c, const RoutingPass& p, uint2* seg, hipStream_t s);
-// PruningUpdater over every row in place (no logs).
-hipError_t prune_rows(const Ctx& c, float inv_scale, hipStream_t s, int dev_cus);
+// PruningUpdater over the rows [rb, re) in place (no logs).
+hipError_t prune_rows(const Ctx& c, float inv_scale, uint32_t rb, uint32_t re, hipStream_t s, int dev_cus);
 
 }  // namespace rf
 }  // namespace artea

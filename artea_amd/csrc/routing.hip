@@ -283,12 +283,12 @@ __global__ void reverse_scatter_kernel(Ctx c, RoutingPass p, uint2* seg) {
 }
 
 // PruningUpdater over every row in place, a wavefront per row.
-__global__ __launch_bounds__(256) void prune_kernel(Ctx c, float inv_scale) {
+__global__ __launch_bounds__(256) void prune_kernel(Ctx c, float inv_scale, uint32_t rb, uint32_t re) {
     __shared__ uint32_t s_av[4][kMaxCap], s_bv[4][kMaxCap];
     __shared__ float s_ad[4][kMaxCap], s_bd[4][kMaxCap];
     const uint32_t wid = threadIdx.x / 64, lane = lane_id();
     unsigned long long pairs = 0;
-    for (uint32_t row = blockIdx.x * 4 + wid; row < c.n; row += gridDim.x * 4) {
+    for (uint32_t row = rb + blockIdx.x * 4 + wid; row < re; row += gridDim.x * 4) {
         const uint32_t n0 = c.cnt[row];
         uint2* rp = c.rows + size_t(row) * c.cap;
         for (uint32_t j = lane; j < n0; j += 64) {
@@ -405,10 +405,11 @@ hipError_t routing_scatter(const Ctx& c, const RoutingPass& p, uint2* seg, hipSt
     return hipGetLastError();
 }
 
-hipError_t prune_rows(const Ctx& c, float inv_scale, hipStream_t s, int dev_cus) {
-    if (c.n == 0) return hipSuccess;
-    const uint32_t blocks = std::max(1u, std::min((c.n + 3) / 4, uint32_t(dev_cus) * 8u));
-    hipLaunchKernelGGL(prune_kernel, dim3(blocks), dim3(256), 0, s, c, inv_scale);
+hipError_t prune_rows(const Ctx& c, float inv_scale, uint32_t rb, uint32_t re, hipStream_t s, int dev_cus) {
+    re = std::min(re, c.n);
+    if (rb >= re) return hipSuccess;
+    const uint32_t blocks = std::max(1u, std::min((re - rb + 3) / 4, uint32_t(dev_cus) * 8u));
+    hipLaunchKernelGGL(prune_kernel, dim3(blocks), dim3(256), 0, s, c, inv_scale, rb, re);
     return hipGetLastError();
 }
 

@@ -313,6 +313,9 @@ int artea_refine_routing_loops(artea_refine* r, uint32_t loops, uint32_t topk, u
 /* next(PruningUpdater(scale)) alone (refiner/updaters/pruning_updater.hpp
  * :170-177): every row pruned in place, no logs. */
 int artea_refine_prune(artea_refine* r, float scale_coeffs);
+/* next_range(PruningUpdater(scale), row_begin, row_end): the rows of the
+ * window only (clamped to the row count). */
+int artea_refine_prune_range(artea_refine* r, float scale_coeffs, uint32_t row_begin, uint32_t row_end);
 /* Since the last call: routing distances (router and pruning pairs),
  * router expansions, and rows whose candidate list dropped an entry tied with
  * its bound even with 512 entries (their results may differ from the

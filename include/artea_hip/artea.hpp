@@ -514,6 +514,23 @@ template <DistanceMetricsT Metric, uint32_t Dim>
 struct truncate_updater_t {};
 template <DistanceMetricsT Metric, uint32_t Dim>
 using arc_updater_t = truncate_updater_t<Metric, Dim>;
+// PruningUpdater(scale_coeffs, shifted_coeffs) (pruning_updater.hpp:130-177):
+// the scaled rule, pairs of two old entries skipped; the shift is not used
+// by that rule.  Rows are pruned in place, no logs.
+template <DistanceMetricsT Metric, uint32_t Dim>
+struct pruning_updater_t {
+    float scale_coeffs = 1.0f;
+    float shifted_coeffs = 0.0f;
+};
+// RoutingUpdater(topk, candidate_queue_size) (routing_updater.hpp:57-114).
+// It only logs into its pivot's own row, so the device runs it fused with
+// the merge and the pruning that refine_layer always applies next:
+// propagate_engine_t::routing_loops takes both updaters.
+template <DistanceMetricsT Metric, uint32_t Dim>
+struct routing_updater_t {
+    vertex_num_t topk = 64;
+    vertex_num_t candidate_queue_size = 96;
+};
 
 // PropagateEngine (refiner/propagate_engine.hpp:38-325) on the GPU.  Every
 // pass (next, next_range, each iteration of run) runs the updater on every
@@ -591,6 +608,22 @@ class propagate_engine_t {
         _download();
         return *this;
     }
+    // refine_layer's routing loops (index/artea_graph/index_factory.hpp
+    // :400-404), which the reference writes as
+    //   for (l = 0; l < loops; ++l) {
+    //       next(routing_updater).next(pruning_updater);
+    //       if (l != loops - 1) next(reverse_updater).next(arc_updater);
+    //   }
+    // Rows longer than max_nbr_size after the last loop keep their first
+    // max_nbr_size entries (the writeback's truncation).
+    propagate_engine_t& routing_loops(uint32_t loops, const routing_updater_t<Metric, Dim>& routing,
+                                      const pruning_updater_t<Metric, Dim>& pruning) {
+        _upload();
+        throw_on(artea_refine_routing_loops(_r, loops, routing.topk, routing.candidate_queue_size,
+                                            pruning.scale_coeffs));
+        _download();
+        return *this;
+    }
     // Every pass merged its logs already; nothing is pending.
     size_t merge_logs() { return 0; }
     // Logs merged by the passes since the previous call.
@@ -648,6 +681,9 @@ class propagate_engine_t {
     }
     void _apply(const truncate_updater_t<Metric, Dim>&, vertex_num_t, vertex_num_t) {
         throw_on(artea_refine_truncate(_r));
+    }
+    void _apply(const pruning_updater_t<Metric, Dim>& u, vertex_num_t s, vertex_num_t e) {
+        throw_on(artea_refine_prune_range(_r, u.scale_coeffs, s, e));
     }
 
     dynamic::single_layer_graph_t* _g = nullptr;

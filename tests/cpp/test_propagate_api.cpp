@@ -336,6 +336,64 @@ static void batch_query_with_configs_equals_explicit_thresholds() {
     std::printf("PASS batch_query_with_configs_equals_explicit_thresholds\n");
 }
 
+static void pruning_updater_reference_kats() {
+    // unit_tests/test_exact_artea_arc_candidates.cpp:382-399: data {0, 1,
+    // 1.25}, scale 2, shift 1.5 (unused by the refiner's scaled rule).
+    vector_array_t v(3, DIM);
+    v.get(1)[0] = 1.0f;
+    v.get(2)[0] = 1.25f;
+    dist_func_t<L2, DIM> df;
+    dynamic::single_layer_graph_t g(v, sl_config_t(4));
+    engine_t engine(df);
+    engine.set_graph(g);
+    auto prune = engine.make_updater<pruning_updater_t<L2, DIM>>(2.0f, 1.5f);
+    g.fetch_nbrs(0) = {Neighbor(1, 1.0f, false), Neighbor(2, 1.25f, false)};
+    engine.next(prune);
+    EXPECT_EQ(g.fetch_nbrs(0).size(), 2u);  // two old entries: the pair is skipped
+    g.fetch_nbrs(0) = {Neighbor(1, 1.0f, true), Neighbor(2, 1.25f, true)};
+    engine.next(prune);
+    const auto& row = g.fetch_nbrs(0);
+    EXPECT_TRUE(row.size() == 1 && row[0].get_vid() == 1u && row[0].is_old());
+    g.fetch_nbrs(0).clear();
+    engine.next(prune);
+    EXPECT_TRUE(g.fetch_nbrs(0).empty());
+    std::printf("PASS pruning_updater_reference_kats\n");
+}
+
+static void routing_loop_grid_closed_form() {
+    // Rows of the chain 0-1-...-11 over the grid; one routing loop with
+    // topk = queue = 12 merges every other vertex into each row, and the
+    // scale-1 pruning leaves exactly the grid's axis neighbours.
+    auto v = grid();
+    dist_func_t<L2, DIM> df;
+    dynamic::single_layer_graph_t g(v, sl_config_t(16));
+    for (uint32_t u = 0; u < 12; ++u) {
+        std::vector<uint32_t> ids;
+        if (u > 0) ids.push_back(u - 1);
+        if (u < 11) ids.push_back(u + 1);
+        set_row(g, u, v, ids);
+        for (auto& e : g.fetch_nbrs(u)) e.mark_as_old();
+    }
+    engine_t engine(df);
+    engine.set_graph(g);
+    auto routing = engine.make_updater<routing_updater_t<L2, DIM>>(12u, 12u);
+    auto prune = engine.make_updater<pruning_updater_t<L2, DIM>>(1.0f, 0.0f);
+    engine.routing_loops(1, routing, prune);
+    for (uint32_t u = 0; u < 12; ++u) {
+        const uint32_t r = u / 4, c = u % 4;
+        std::vector<uint32_t> want;
+        if (r > 0) want.push_back(u - 4);
+        if (c > 0) want.push_back(u - 1);
+        if (c < 3) want.push_back(u + 1);
+        if (r < 2) want.push_back(u + 4);
+        std::vector<uint32_t> got;
+        for (const auto& e : g.fetch_nbrs(u)) got.push_back(e.get_vid());
+        std::sort(got.begin(), got.end());
+        EXPECT_TRUE(got == want);
+    }
+    std::printf("PASS routing_loop_grid_closed_form\n");
+}
+
 int main() {
     try {
         rgraph_thresholds_closed_forms_and_errors();
@@ -347,6 +405,8 @@ int main() {
         next_range_limits_the_updater_to_the_window();
         sparse_layer_rows_hold_global_ids();
         batch_query_with_configs_equals_explicit_thresholds();
+        pruning_updater_reference_kats();
+        routing_loop_grid_closed_form();
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());
         ++g_failures;

@@ -32,7 +32,8 @@ def test_cpp_propagate_api():
                  "triangle_selects_closest_recommendation_without_pruning", "triangle_uses_alpha_dependent_shift",
                  "integrated_random_and_reverse", "reverse_edges_between_build_rounds",
                  "next_range_limits_the_updater_to_the_window", "sparse_layer_rows_hold_global_ids",
-                 "batch_query_with_configs_equals_explicit_thresholds"):
+                 "batch_query_with_configs_equals_explicit_thresholds", "pruning_updater_reference_kats",
+                 "routing_loop_grid_closed_form"):
         assert "PASS " + case in r.stdout, case
 
 

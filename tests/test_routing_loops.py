@@ -95,6 +95,43 @@ def test_oracle_routing_full_queue_equals_exact_knn(orc):
         assert got == want, p
 
 
+def grid_vecs():
+    v = np.zeros((12, 16), np.float32)
+    for i in range(12):
+        v[i, 0], v[i, 1] = i % 4, i // 4
+    return v
+
+
+def grid_chain_rows(orc, v):
+    """Rows of the chain 0-1-...-11 over the 3 x 4 grid (old entries)."""
+    d = lambda a, b: float(np.sqrt(((v[a] - v[b]) ** 2).sum()))
+    rows = []
+    for u in range(12):
+        nb = sorted([x for x in (u - 1, u + 1) if 0 <= x < 12], key=lambda x: (d(u, x), x))
+        rows.append(np.array([(x, d(u, x)) for x in nb], dtype=orc.NBR_DTYPE))
+    return rows
+
+
+def grid_neighbours(u):
+    r, c = u // 4, u % 4
+    return sorted(rr * 4 + cc for rr, cc in ((r - 1, c), (r + 1, c), (r, c - 1), (r, c + 1))
+                  if 0 <= rr < 3 and 0 <= cc < 4)
+
+
+def test_oracle_routing_loop_grid_closed_form(orc):
+    """A routing loop with topk = queue = n over a connected layer merges
+    every other vertex into each row; the scale-1 pruning then leaves exactly
+    the 3 x 4 grid's axis neighbours (a diagonal or farther vertex always has
+    a retained axis neighbour strictly closer to it than the pivot is)."""
+    v = grid_vecs()
+    L = orc.Layer(v, orc.L2, 16)
+    for u, row in enumerate(grid_chain_rows(orc, v)):
+        L.set_row_raw(u, row)
+    L.routing_loops(1, 12, 12, 1.0)
+    for u in range(12):
+        assert sorted(row_ids(L, u)) == grid_neighbours(u), u
+
+
 def test_oracle_routing_loops_bounds_and_flags(orc):
     n, D, cap = 300, 16, 24
     _, L = random_layer(orc, n, D, cap, 5)
@@ -206,6 +243,23 @@ def test_gpu_routing_loops_match_oracle(orc, metric, sparse, topk, queue, cap):
         assert st["unresolved"] == 0 and st["expansions"] > 0
         got, want = dev_rows(eng), orc_rows(L2, cap)
         assert got == want, (loops, first_difference(got, want))
+
+
+@pytest.mark.gpu
+def test_gpu_routing_loop_grid_closed_form(orc):
+    import artea_amd as aa
+    from artea_amd.refine import PropagateEngine
+
+    v = grid_vecs()
+    eng = PropagateEngine(aa.DeviceIndex(v, 3), cap=16, metric=3)
+    rows = grid_chain_rows(orc, v)
+    counts = np.array([len(r) for r in rows], np.uint32)
+    eng.upload(counts, np.concatenate(rows).view(np.uint32).reshape(-1, 2))
+    eng.routing_loops(1, 12, 12, 1.0)
+    got = dev_rows(eng)
+    for u in range(12):
+        assert sorted(e[0] & 0x7FFFFFFF for e in got[u]) == grid_neighbours(u), u
+        assert all(e[0] & NEW == 0 for e in got[u])
 
 
 @pytest.mark.gpu
