@@ -104,6 +104,12 @@ SIGNATURES = {
     "artea_build_full_routing": (i32, [P, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, i32, f32, u32, u32, u32,
                                        P]),
     "artea_refine_routing_loops": (i32, [P, u32, u32, u32, f32]),
+    "artea_rnet_build": (i32, [P, i32, P, P]),
+    "artea_rnet_info": (i32, [P, P, P, P, P, P]),
+    "artea_rnet_levels": (i32, [P, P]),
+    "artea_rnet_level_rows": (i32, [P, u32, P, P, u64, P]),
+    "artea_rnet_destroy": (i32, [P]),
+    "artea_build_full_rnet": (i32, [P, P, u32, u32, f32, f32, i32, u32, u32, u32, P]),
     "artea_refine_prune": (i32, [P, f32]),
     "artea_refine_prune_range": (i32, [P, f32, u32, u32]),
     "artea_refine_routing_stats": (i32, [P, P, P, P]),

@@ -27,6 +27,7 @@
 #include "common.hpp"
 #include "device.hpp"
 #include "refine_ctx.hpp"
+#include "rnet.hpp"
 
 using namespace artea;
 using namespace artea::dev;
@@ -986,6 +987,57 @@ __global__ __launch_bounds__(256) void merge_kernel(Ctx c) {
     }
 }
 
+// fill_single_layer_graph_from_layer (refiner/refiner_utils.hpp:60-108):
+// each local row takes its vertex's row of `src` (by vid, src_cap slots),
+// sorted by (distance, vid), cut to cap.  A wavefront per row.
+__global__ __launch_bounds__(256) void import_rows_kernel(Ctx c, const uint2* src, const uint32_t* src_cnt,
+                                                          uint32_t src_cap) {
+    __shared__ unsigned long long s_key[4][kMaxCap];
+    __shared__ uint32_t s_pay[4][kMaxCap];
+    const uint32_t wid = threadIdx.x / 64, lane = lane_id();
+    for (uint32_t row = blockIdx.x * 4 + wid; row < c.n; row += gridDim.x * 4) {
+        const uint32_t v = storage_of(c, row);
+        const uint32_t m = src ? min(src_cnt[v], src_cap) : 0u;
+        for (uint32_t j = lane; j < kMaxCap; j += 64) {
+            if (j < m) {
+                const uint2 e = src[size_t(v) * src_cap + j];
+                s_key[wid][j] = sort_key(__uint_as_float(e.y), e.x, 0);
+                s_pay[wid][j] = e.x;
+            } else {
+                s_key[wid][j] = ~0ull;
+                s_pay[wid][j] = 0xFFFFFFFFu;
+            }
+        }
+        wave_sync();
+        // bitonic sort of kMaxCap keys by one wavefront
+        for (uint32_t k2 = 2; k2 <= kMaxCap; k2 <<= 1) {
+            for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+                for (uint32_t t = lane; t < kMaxCap / 2; t += 64) {
+                    const uint32_t i = 2 * j * (t / j) + (t % j), l = i + j;
+                    const bool up = (i & k2) == 0;
+                    const unsigned long long a = s_key[wid][i], b = s_key[wid][l];
+                    if ((a > b) == up) {
+                        s_key[wid][i] = b;
+                        s_key[wid][l] = a;
+                        const uint32_t pa = s_pay[wid][i];
+                        s_pay[wid][i] = s_pay[wid][l];
+                        s_pay[wid][l] = pa;
+                    }
+                }
+                wave_sync();
+            }
+        }
+        const uint32_t keep = min(m, c.cap);
+        for (uint32_t j = lane; j < keep; j += 64) {
+            uint32_t u = uint32_t(s_key[wid][j] >> 32);
+            u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+            c.rows[size_t(row) * c.cap + j] = make_uint2(s_pay[wid][j], u);
+        }
+        if (lane == 0) c.cnt[row] = keep;
+        wave_sync();
+    }
+}
+
 __global__ void count_logs_kernel(const uint32_t* t_off, uint32_t n, unsigned long long* stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) stats[0] += t_off[n];
 }
@@ -1792,3 +1844,105 @@ int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max,
 }
 
 }  // extern "C"
+
+extern "C" int artea_build_full_rnet(artea_index* idx, const artea_rnet_params* p, uint32_t build_loops,
+                                     uint32_t triu_iters, float prefill_ratio, float shifted_coeffs, int join_precision,
+                                     uint32_t routing_loops, uint32_t routing_topk, uint32_t routing_queue,
+                                     double* seconds) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx && p, "artea_build_full_rnet: null argument");
+    ARTEA_CHECK_ARG(join_precision == ARTEA_JOIN_FP32 || join_precision == ARTEA_JOIN_BF16,
+                    "artea_build_full_rnet: join_precision must be ARTEA_JOIN_FP32 or ARTEA_JOIN_BF16");
+    ARTEA_CHECK_ARG(uint32_t(float(p->bl_max_nbr_size) * 1.5f) <= kMaxCap &&
+                        uint32_t(float(p->ul_max_nbr_size) * 1.5f) <= kMaxCap,
+                    "artea_build_full_rnet: refine caps (1.5 x max) must be <= 128");
+    if (routing_loops > 0) {
+        ARTEA_CHECK_ARG(routing_topk >= 1 && routing_topk <= 128,
+                        "artea_build_full_rnet: routing topk must be in [1, 128]");
+        ARTEA_CHECK_ARG(routing_queue >= routing_topk && routing_queue <= 256,
+                        "artea_build_full_rnet: routing queue must be in [topk, 256]");
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = idx->n;
+    const int bmetric = idx->metric == ARTEA_L2SQR ? ARTEA_L2 : idx->metric;
+    artea_rnet* R = nullptr;
+    int rc = artea::rnet_run(idx->d_vecs, n, idx->dim_pad, bmetric, *p, idx->stream, &R);
+    if (rc) return rc;
+    std::unique_ptr<artea_rnet, void (*)(artea_rnet*)> rguard(R, artea::rnet_free);
+    if (seconds) seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const uint32_t L = R->L;
+    const uint32_t top = R->top == kInvalidLevel ? 0u : R->top;
+    std::vector<std::vector<uint64_t>> roff(L + 1);
+    std::vector<std::vector<uint32_t>> rids(L + 1);
+    for (uint32_t l = 0; l <= L; ++l) {
+        std::vector<uint32_t> members;
+        for (uint32_t v = 0; v < n; ++v)
+            if (R->highest[v] != kInvalidLevel && R->highest[v] >= l) members.push_back(v);
+        const uint32_t nl = uint32_t(members.size());
+        const uint32_t slot_cap = l == 0 ? p->bl_max_nbr_size : p->ul_max_nbr_size;
+        roff[l].assign(size_t(n) + 1, 0);
+        std::vector<uint2> rows;
+        std::vector<uint32_t> cnt;
+        uint32_t stride = 0;
+        if (l <= top && nl >= 32) {
+            const uint32_t cap = uint32_t(float(slot_cap) * 1.5f);
+            artea_refine* r = nullptr;
+            rc = refine_make(idx->d_vecs, n, idx->dim_pad, bmetric, cap, nl == n ? nullptr : members.data(), nl,
+                             p->seed, &r);
+            if (rc) return rc;
+            std::unique_ptr<artea_refine, int (*)(artea_refine*)> guard(r, artea_refine_destroy);
+            r->join = join_precision;
+            hipLaunchKernelGGL(import_rows_kernel, dim3(grid_for(r, 4, 8)), dim3(256), 0, r->s, r->c, R->rows[l],
+                               R->cnt[l], slot_cap);
+            ARTEA_HIP_TRY(hipGetLastError());
+            if (l == 0) {
+                const uint32_t gen = uint32_t(float(cap) * prefill_ratio);
+                if (gen > 0) {
+                    rc = do_prefill(r, gen, 0, nl, 0, nl, 0);
+                    if (rc) return rc;
+                }
+            }
+            const float shift = l == 0 ? shifted_coeffs * p->l0_min_distance : 0.0f;
+            rc = do_schedule(r, build_loops, triu_iters, p->scale_coeffs, shift);
+            if (rc) return rc;
+            rc = do_routing_loops(r, routing_loops, routing_topk, routing_queue, p->scale_coeffs);
+            if (rc) return rc;
+            rows.resize(std::max<size_t>(size_t(nl) * cap, 1));
+            cnt.resize(std::max<uint32_t>(nl, 1));
+            ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), r->c.rows, size_t(nl) * cap * sizeof(uint2),
+                                         hipMemcpyDeviceToHost, r->s));
+            ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), r->c.cnt, size_t(nl) * 4, hipMemcpyDeviceToHost, r->s));
+            ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
+            stride = cap;
+            for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[i], slot_cap);
+        } else if (R->rows[l] && nl > 0) {
+            // not refined: the r-net rows as they are (by vid)
+            rows.resize(size_t(n) * slot_cap);
+            cnt.resize(n);
+            ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), R->rows[l], size_t(n) * slot_cap * sizeof(uint2),
+                                         hipMemcpyDeviceToHost, R->s));
+            ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), R->cnt[l], size_t(n) * 4, hipMemcpyDeviceToHost, R->s));
+            ARTEA_HIP_TRY(hipStreamSynchronize(R->s));
+            stride = slot_cap;
+            for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[members[i]], slot_cap);
+        }
+        for (uint32_t v = 0; v < n; ++v) roff[l][v + 1] += roff[l][v];
+        rids[l].resize(roff[l][n]);
+        const bool by_vid = !(l <= top && nl >= 32);
+        for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t v = members[i];
+            const size_t base = size_t(by_vid ? v : i) * stride;
+            for (uint64_t k = 0; k < roff[l][v + 1] - roff[l][v]; ++k) rids[l][roff[l][v] + k] = rows[base + k].x & kMask;
+        }
+    }
+    std::vector<const uint64_t*> offp(L + 1);
+    std::vector<const uint32_t*> idp(L + 1);
+    for (uint32_t l = 0; l <= L; ++l) {
+        offp[l] = roff[l].data();
+        idp[l] = rids[l].data();
+    }
+    rc = compact_and_attach(idx, R->highest.data(), nullptr, offp.data(), idp.data(), p->ul_max_nbr_size,
+                            p->bl_max_nbr_size, bmetric);
+    if (seconds) seconds[1] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}

@@ -134,3 +134,70 @@ def build_full(index, ul_max=32, bl_max=64, max_level=None, build_loops=15, triu
                                          l0_min_distance, seed, join, level_fraction, routing_loops, routing_topk,
                                          routing_queue, _ptr(secs)))
     return secs
+
+
+class RnetParams(C.Structure):
+    """artea_rnet_params: RGraphConfig (index/stacked_rgraph/configs.hpp
+    :65-138), the insertion path's RNG scale and the wave schedule."""
+    _fields_ = [("rnet_beta", C.c_float), ("tau_k", C.c_float), ("l0_min_distance", C.c_float),
+                ("scale_coeffs", C.c_float), ("search_nn_qs", C.c_uint32), ("ul_select_nbrs_qs", C.c_uint32),
+                ("bl_select_nbrs_qs", C.c_uint32), ("ul_max_nbr_size", C.c_uint32), ("bl_max_nbr_size", C.c_uint32),
+                ("insert_on_l0", C.c_uint32), ("wave_div", C.c_uint32), ("wave_max", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+def rnet_params(rnet_beta=2.0, tau_k=0.0, l0_min_distance=1.0, scale_coeffs=1.10, search_nn_qs=64,
+                ul_select_nbrs_qs=100, bl_select_nbrs_qs=100, ul_max_nbr_size=32, bl_max_nbr_size=64,
+                insert_on_l0=False, wave_div=8, wave_max=32768, seed=0) -> RnetParams:
+    return RnetParams(rnet_beta, tau_k, l0_min_distance, scale_coeffs, search_nn_qs, ul_select_nbrs_qs,
+                      bl_select_nbrs_qs, ul_max_nbr_size, bl_max_nbr_size, int(insert_on_l0), wave_div, wave_max,
+                      seed)
+
+
+class RNet:
+    """R-net insertion on the GPU (stacked_rgraph::IndexFactory): levels and
+    rows per level of the dynamic hierarchy, before refinement."""
+
+    def __init__(self, index, params: RnetParams, metric: Optional[int] = None):
+        m = (3 if index.metric == 0 else index.metric) if metric is None else metric
+        h = C.c_void_p()
+        check(lib().artea_rnet_build(index.h, m, C.byref(params), C.byref(h)))
+        self.h = h
+        self.n = index.n
+        self.params = params
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().artea_rnet_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        ml, top, waves = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        dist, unres = C.c_uint64(), C.c_uint64()
+        check(lib().artea_rnet_info(self.h, C.byref(ml), C.byref(top), C.byref(waves), C.byref(dist), C.byref(unres)))
+        return {"max_level": ml.value, "top": top.value, "waves": waves.value, "distances": dist.value,
+                "unresolved": unres.value}
+
+    def highest(self):
+        out = np.zeros(max(self.n, 1), np.uint32)
+        check(lib().artea_rnet_levels(self.h, _ptr(out)))
+        return out[: self.n]
+
+    def level_rows(self, level):
+        counts = np.zeros(max(self.n, 1), np.uint32)
+        total = C.c_uint64()
+        check(lib().artea_rnet_level_rows(self.h, level, _ptr(counts), None, 0, C.byref(total)))
+        ent = np.zeros(max(total.value, 1) * 2, np.uint32)
+        check(lib().artea_rnet_level_rows(self.h, level, _ptr(counts), _ptr(ent), total.value, C.byref(total)))
+        return counts[: self.n], ent[: 2 * total.value].reshape(-1, 2)
+
+
+def build_full_rnet(index, params: RnetParams, build_loops=15, triu_iters=4, prefill=0.4, shift=1.5, join=JOIN_FP32,
+                    routing_loops=0, routing_topk=64, routing_queue=96):
+    """The artea_graph build on the GPU (index/artea_graph/index_factory.hpp
+    :135-410): r-net insertion, refine_layer per level, compaction; attaches
+    the graph to ``index``.  Returns (r-net seconds, total seconds)."""
+    secs = np.zeros(2, np.float64)
+    check(lib().artea_build_full_rnet(index.h, C.byref(params), build_loops, triu_iters, prefill, shift, join,
+                                      routing_loops, routing_topk, routing_queue, _ptr(secs)))
+    return float(secs[0]), float(secs[1])

@@ -385,6 +385,59 @@ int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max,
                              float level_fraction, uint32_t routing_loops, uint32_t routing_topk,
                              uint32_t routing_queue, double* seconds_per_level);
 
+/* --------------------------------------------------------- r-net insertion */
+/* RGraphConfig (index/stacked_rgraph/configs.hpp:65-138) with the insertion
+ * path's RNG scale (PruningConfig::scale_coeffs) and the wave schedule.
+ * Vertices are inserted in vid order in waves: each vertex of a wave plans
+ * its insertion (_insert_one's descent, covering rule and per-level select
+ * searches with HierarchicalPruningUpdater, index_factory.hpp:281-478)
+ * against the graph as it stood when the wave began; the wave is then
+ * applied in vid order (levels, forward rows, reverse edges per target in
+ * vid order, _write_reverse_edges :528-580).  A wave ends after its first
+ * vertex that grows the hierarchy.  The first 10 vertices go one by one
+ * (startup_points), then waves of min(wave_max, inserted / wave_div);
+ * wave_max = 1 is the reference's serial insertion.  The apex of each
+ * descent is the top bucket's entry at a seeded counter-based draw (the
+ * reference draws it from std::random_device).  Queues: 1 <= search_qs <=
+ * ul/bl_select_qs <= 224; max_nbr_size <= 128. */
+typedef struct {
+    float rnet_beta, tau_k, l0_min_distance, scale_coeffs;
+    uint32_t search_nn_qs, ul_select_nbrs_qs, bl_select_nbrs_qs, ul_max_nbr_size, bl_max_nbr_size;
+    uint32_t insert_on_l0;
+    uint32_t wave_div, wave_max;
+    uint64_t seed;
+} artea_rnet_params;
+typedef struct artea_rnet artea_rnet;
+/* stacked_rgraph::IndexFactory::build_from_vectors (index_factory.hpp
+ * :176-188) over idx's vectors with build metric `metric`. */
+int artea_rnet_build(artea_index* idx, int metric, const artea_rnet_params* p, artea_rnet** out);
+/* max_allowed_level_id, top occupied level (ARTEA_INVALID_LEVEL: none),
+ * waves, distances evaluated, vertices whose candidate list dropped an
+ * entry tied with its bound even with 512 entries (their plan may differ
+ * from the reference's; none expected on real data). */
+int artea_rnet_info(const artea_rnet* r, uint32_t* max_level, uint32_t* top, uint32_t* waves, uint64_t* distances,
+                    uint64_t* unresolved);
+/* highest[v] for every vertex. */
+int artea_rnet_levels(const artea_rnet* r, uint32_t* highest);
+/* Rows of `level` by vid as stored (forward entries, then appended reverse
+ * entries): counts[v] (n values) and (vid_status, distance bits) pairs back
+ * to back, at most max_entries pairs written; *total = all pairs. */
+int artea_rnet_level_rows(const artea_rnet* r, uint32_t level, uint32_t* counts, uint32_t* entries,
+                          uint64_t max_entries, uint64_t* total);
+int artea_rnet_destroy(artea_rnet* r);
+/* The artea_graph build (index/artea_graph/index_factory.hpp:135-410) on the
+ * GPU, attached to idx: r-net insertion (insert_on_l0 as given), then
+ * refine_layer for every occupied level with at least 32 vertices (rows
+ * imported sorted by (distance, vid) into caps floor(1.5 x max); L0 topped
+ * up by the random updater to floor(cap x prefill_ratio); the build loops
+ * with scale_coeffs and, on L0, shift = shifted_coeffs x l0_min_distance;
+ * routing loops), the writeback of the first max_nbr_size entries, and the
+ * compactor (vid order = insertion order).  Build metric L2 for an L2SQR
+ * index.  seconds (optional) = {r-net insertion, whole build}. */
+int artea_build_full_rnet(artea_index* idx, const artea_rnet_params* p, uint32_t build_loops, uint32_t triu_iters,
+                          float prefill_ratio, float shifted_coeffs, int join_precision, uint32_t routing_loops,
+                          uint32_t routing_topk, uint32_t routing_queue, double* seconds);
+
 /* -------------------------------------------------------------- utilities */
 /* dist_func_t on the device (utils/simd_distance.hpp:64-79): dists[i*nb+j]
  * = dist(a_i, b_j) for device arrays, bit-identical to the AVX-512 kernel's

@@ -1683,3 +1683,422 @@ void orc_probe_farthest(int metric, const float* base, uint32_t n, uint32_t D, c
         out[s] = m;
     }
 }
+
+/* ======================================================================
+ * R-net insertion (index/stacked_rgraph/index_factory.hpp:192-583) in
+ * waves: the vertices of a wave all search the graph as it stood at the
+ * wave's start and are then applied in insertion order (level, forward
+ * rows, reverse edges).  A wave ends after the first vertex that grows the
+ * hierarchy.  Waves of one vertex are the reference's serial insertion.
+ * The apex entry of a descent is drawn from the counter-based stream (the
+ * reference draws it from a thread-local std::mt19937_64 seeded by
+ * std::random_device, candidate_sample_utils.hpp:108-148).
+ * ====================================================================== */
+#define RNET_STREAM 0x726E6574ull /* entry draws */
+
+typedef struct {
+    const float* vecs;
+    uint32_t n, D, max_level;
+    int metric;
+    orc_rnet_params p;
+    nvec* rows;         /* (max_level + 1) * n rows, level-major */
+    uint32_t* highest;  /* ORC_INVALID_LEVEL until assigned */
+    uint32_t** bucket;  /* vids per highest level, in assignment order */
+    uint32_t* bucket_n;
+    uint32_t top;       /* ORC_INVALID_LEVEL: empty */
+    uint32_t waves;
+} rnet_t;
+
+static inline const float* vec_of_g(const float* vecs, uint32_t D, uint32_t v) { return vecs + (uint64_t)v * D; }
+
+static inline nvec* rnet_row(const rnet_t* R, uint32_t l, uint32_t v) { return &R->rows[(size_t)l * R->n + v]; }
+static inline uint32_t rnet_cap(const rnet_t* R, uint32_t l) { return l == 0 ? R->p.bl_max : R->p.ul_max; }
+
+/* RGraphConfig::radius_at (stacked_rgraph/configs.hpp:130-138). */
+float orc_rnet_radius(float l0_min_distance, float tau_k, float beta, uint32_t h) {
+    if (h == 0) return l0_min_distance;
+    return (float)((double)l0_min_distance * (1.0 + (double)tau_k) * pow((double)beta, (double)(h - 1)));
+}
+
+/* Wave length after `inserted` vertices: the reference's serial startup
+ * (startup_points = 10, index_factory.hpp:104-105, 242-254), then
+ * inserted / wave_div vertices, at most wave_max. */
+uint32_t orc_rnet_wave(uint32_t inserted, uint32_t wave_div, uint32_t wave_max) {
+    if (inserted < 10) return 1;
+    uint32_t w = wave_div ? inserted / wave_div : wave_max;
+    if (w < 1) w = 1;
+    return w > wave_max ? wave_max : w;
+}
+
+/* HierarchicalPruningUpdater::update_impl (refiner/updaters/
+ * hierarchical_pruning_updater.hpp:96-138): the first entry is kept; each
+ * later one while fewer than max_nbr are kept is kept iff every kept entry
+ * (pairs of two old entries skipped) is at least dist * (1 / scale) away
+ * from it; kept entries marked old. */
+static uint32_t rnet_prune(const rnet_t* R, orc_nbr* a, uint32_t m, uint32_t max_nbr) {
+    if (m == 0) return 0;
+    const float inv = 1.0f / R->p.scale;
+    uint32_t nk = 1;
+    for (uint32_t i = 1; i < m && nk < max_nbr; ++i) {
+        const orc_nbr c = a[i];
+        const float thr = c.dist * inv;
+        const int c_old = (c.vid_status & ORC_NBR_NEW) == 0;
+        const float* cv = vec_of_g(R->vecs, R->D, c.vid_status & ORC_INVALID_VID);
+        int keep = 1;
+        for (uint32_t k = 0; k < nk; ++k) {
+            if (c_old && (a[k].vid_status & ORC_NBR_NEW) == 0) continue;
+            if (orc_distance(R->metric, cv, vec_of_g(R->vecs, R->D, a[k].vid_status & ORC_INVALID_VID), R->D) < thr) {
+                keep = 0;
+                break;
+            }
+        }
+        if (keep) a[nk++] = c;
+    }
+    for (uint32_t k = 0; k < nk; ++k) a[k].vid_status &= ORC_INVALID_VID;
+    return nk;
+}
+
+/* SingleLayerRouter::beam_search (single_layer_router.hpp:145-163) over
+ * level l's rows (DynamicLayerRange: entries until the first invalid one). */
+static void rnet_beam(const rnet_t* R, const float* q, uint32_t l, orc_queue* cq, vtt_t* vis) {
+    orc_queue_reset_exploration(cq);
+    if (cq->U.n == 0) return;
+    for (uint32_t i = 0; i < cq->T.n; ++i) vtt_set(vis, cq->T.a[i].vid);
+    while (cq->U.n != 0) {
+        if (queue_should_terminate(cq)) break;
+        const centry cur = heap_pop(&cq->U);
+        if (cur.vid == ORC_INVALID_VID) break;
+        const nvec* row = rnet_row(R, l, cur.vid);
+        for (uint32_t j = 0; j < row->n; ++j) {
+            const uint32_t u = row->a[j].vid_status & ORC_INVALID_VID;
+            if (vtt_test_and_set(vis, u)) continue;
+            queue_try_push(cq, u, orc_distance(R->metric, q, vec_of_g(R->vecs, R->D, u), R->D));
+        }
+    }
+}
+
+typedef struct {
+    uint32_t h;
+    int grows;
+    uint32_t* nf;   /* forward entries per level */
+    orc_nbr** f;
+} rnet_plan;
+
+static void queue_copy_top(orc_queue* dst, const orc_queue* src, uint32_t capacity) {
+    /* seed_from_queue (std_candidate_queue.hpp:456-472) */
+    queue_clear(dst);
+    dst->capacity = capacity;
+    for (uint32_t i = 0; i < src->T.n; ++i) queue_try_push(dst, src->T.a[i].vid, src->T.a[i].d);
+}
+
+/* Steps A, B and the searches of D of _insert_one (index_factory.hpp
+ * :281-478) against the current graph; nothing is written. */
+static void rnet_plan_vertex(const rnet_t* R, uint32_t qv, orc_queue* stash, orc_queue* cq, vtt_t* vis,
+                             rnet_plan* out) {
+    const orc_rnet_params* P = &R->p;
+    const uint32_t L = R->max_level;
+    const float* q = vec_of_g(R->vecs, R->D, qv);
+    float* min_dist = (float*)malloc(sizeof(float) * (L + 1));
+    for (uint32_t l = 0; l <= L; ++l) {
+        min_dist[l] = FLT_MAX;
+        queue_clear(&stash[l]);
+        stash[l].capacity = P->search_qs;
+    }
+    const uint32_t top = R->top;
+    if (top != ORC_INVALID_LEVEL && top >= 1) {
+        queue_clear(cq);
+        cq->capacity = P->search_qs;
+        const uint32_t sz = R->bucket_n[top];
+        const uint32_t e = R->bucket[top][orc_rand_draw(P->seed, RNET_STREAM, qv, 0, 0, sz)];
+        queue_try_push(cq, e, orc_distance(R->metric, q, vec_of_g(R->vecs, R->D, e), R->D));
+        vtt_clear(vis);
+        for (uint32_t l = top; l >= 1; --l) {
+            rnet_beam(R, q, l, cq, vis);
+            if (cq->T.n > 0) {
+                float b = FLT_MAX;
+                for (uint32_t i = 0; i < cq->T.n; ++i)
+                    if (cq->T.a[i].d < b) b = cq->T.a[i].d;
+                min_dist[l] = b;
+            }
+            queue_copy_top(&stash[l], cq, P->search_qs);
+            if (l == 1) break;
+            orc_queue nx;
+            queue_init(&nx, P->search_qs);
+            queue_copy_top(&nx, cq, P->search_qs);
+            queue_release(cq);
+            *cq = nx;
+        }
+    }
+    if (P->insert_on_l0 && stash[1].T.n > 0) queue_copy_top(&stash[0], &stash[1], P->search_qs);
+    uint32_t h;
+    if (top == ORC_INVALID_LEVEL) {
+        h = 1;
+    } else {
+        h = top + 1 < L ? top + 1 : L;
+        for (uint32_t k = 1; k <= top; ++k)
+            if (min_dist[k] <= orc_rnet_radius(P->l0_min_distance, P->tau_k, P->beta, k)) {
+                h = k - 1;
+                break;
+            }
+    }
+    out->h = h;
+    out->grows = top == ORC_INVALID_LEVEL || h > top;
+    out->nf = (uint32_t*)calloc(L + 1, sizeof(uint32_t));
+    out->f = (orc_nbr**)calloc(L + 1, sizeof(orc_nbr*));
+    for (uint32_t l = P->insert_on_l0 ? 0 : 1; l <= h; ++l) {
+        if (stash[l].T.n == 0) continue;
+        orc_queue sq;
+        queue_init(&sq, l == 0 ? P->bl_select_qs : P->ul_select_qs);
+        for (uint32_t i = 0; i < stash[l].T.n; ++i) queue_try_push(&sq, stash[l].T.a[i].vid, stash[l].T.a[i].d);
+        vtt_clear(vis);
+        rnet_beam(R, q, l, &sq, vis);
+        const uint32_t m = sq.T.n;
+        uint32_t* vv = (uint32_t*)malloc(sizeof(uint32_t) * (m ? m : 1));
+        float* dd = (float*)malloc(sizeof(float) * (m ? m : 1));
+        orc_queue_extract(&sq, m, vv, dd);
+        orc_nbr* c = (orc_nbr*)malloc(sizeof(orc_nbr) * (m ? m : 1));
+        uint32_t nc = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            if (vv[i] == qv) continue;
+            c[nc].vid_status = vv[i] | ORC_NBR_NEW;
+            c[nc].dist = dd[i];
+            ++nc;
+        }
+        out->nf[l] = rnet_prune(R, c, nc, rnet_cap(R, l));
+        out->f[l] = c;
+        free(vv);
+        free(dd);
+        queue_release(&sq);
+    }
+    free(min_dist);
+}
+
+/* Stable order by distance for the re-pruned full slot (the reference's
+ * std::sort by distance leaves equal distances in an unspecified order). */
+static void stable_by_dist(orc_nbr* a, uint32_t n) {
+    for (uint32_t i = 1; i < n; ++i) {
+        const orc_nbr x = a[i];
+        uint32_t j = i;
+        while (j > 0 && a[j - 1].dist > x.dist) {
+            a[j] = a[j - 1];
+            --j;
+        }
+        a[j] = x;
+    }
+}
+
+/* Step C and the writes of D: assign_layer, _write_forward_edges and
+ * _write_reverse_edges (index_factory.hpp:497-580). */
+static void rnet_apply(rnet_t* R, uint32_t qv, const rnet_plan* pl) {
+    const uint32_t h = pl->h;
+    R->highest[qv] = h;
+    if (R->bucket_n[h] % 1024 == 0) R->bucket[h] = (uint32_t*)realloc(R->bucket[h], sizeof(uint32_t) * (R->bucket_n[h] + 1024));
+    R->bucket[h][R->bucket_n[h]++] = qv;
+    if (pl->grows) R->top = h;
+    for (uint32_t l = R->p.insert_on_l0 ? 0 : 1; l <= h; ++l) {
+        const uint32_t m = pl->nf[l];
+        if (m == 0) continue;
+        const uint32_t cap = rnet_cap(R, l);
+        nvec* own = rnet_row(R, l, qv);
+        own->n = 0;
+        for (uint32_t i = 0; i < m && i < cap; ++i) nvec_push(own, pl->f[l][i]);
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t nb = pl->f[l][i].vid_status & ORC_INVALID_VID;
+            const orc_nbr back = {qv | ORC_NBR_NEW, pl->f[l][i].dist};
+            nvec* row = rnet_row(R, l, nb);
+            if (row->n < cap) {
+                nvec_push(row, back);
+                continue;
+            }
+            orc_nbr* merged = (orc_nbr*)malloc(sizeof(orc_nbr) * (cap + 1));
+            memcpy(merged, row->a, sizeof(orc_nbr) * cap);
+            merged[cap] = back;
+            stable_by_dist(merged, cap + 1);
+            const uint32_t k = rnet_prune(R, merged, cap + 1, cap);
+            memcpy(row->a, merged, sizeof(orc_nbr) * k);
+            row->n = k;
+            free(merged);
+        }
+    }
+}
+
+static void plan_free(rnet_plan* pl, uint32_t L) {
+    for (uint32_t l = 0; l <= L; ++l) free(pl->f[l]);
+    free(pl->f);
+    free(pl->nf);
+}
+
+orc_rnet* orc_rnet_build(const float* vecs, uint32_t n, uint32_t D, int metric, const orc_rnet_params* p,
+                         int nthreads) {
+    rnet_t* R = (rnet_t*)calloc(1, sizeof(rnet_t));
+    R->vecs = vecs;
+    R->n = n;
+    R->D = D;
+    R->metric = metric;
+    R->p = *p;
+    R->max_level = orc_max_level(n);
+    const uint32_t L = R->max_level;
+    R->rows = (nvec*)calloc((size_t)(L + 1) * (n ? n : 1), sizeof(nvec));
+    R->highest = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t v = 0; v < n; ++v) R->highest[v] = ORC_INVALID_LEVEL;
+    R->bucket = (uint32_t**)calloc(L + 1, sizeof(uint32_t*));
+    R->bucket_n = (uint32_t*)calloc(L + 1, sizeof(uint32_t));
+    R->top = ORC_INVALID_LEVEL;
+    const int T = pick_threads(nthreads);
+    rnet_plan* plans = (rnet_plan*)calloc(p->wave_max ? p->wave_max : 1, sizeof(rnet_plan));
+    uint32_t pos = 0;
+    R->waves = 0;
+    while (pos < n) {
+        uint32_t w = orc_rnet_wave(pos, p->wave_div, p->wave_max);
+        if (w > n - pos) w = n - pos;
+#pragma omp parallel num_threads(T)
+        {
+            orc_queue cq;
+            queue_init(&cq, p->search_qs);
+            orc_queue* stash = (orc_queue*)malloc(sizeof(orc_queue) * (L + 1));
+            for (uint32_t l = 0; l <= L; ++l) queue_init(&stash[l], p->search_qs);
+            vtt_t vis;
+            vtt_init(&vis, n);
+#pragma omp for schedule(dynamic, 1)
+            for (int64_t i = 0; i < (int64_t)w; ++i) rnet_plan_vertex(R, pos + (uint32_t)i, stash, &cq, &vis, &plans[i]);
+            for (uint32_t l = 0; l <= L; ++l) queue_release(&stash[l]);
+            free(stash);
+            queue_release(&cq);
+            free(vis.tags);
+        }
+        uint32_t end = w;
+        for (uint32_t i = 0; i < w; ++i)
+            if (plans[i].grows) {
+                end = i + 1;
+                break;
+            }
+        for (uint32_t i = 0; i < end; ++i) rnet_apply(R, pos + i, &plans[i]);
+        for (uint32_t i = 0; i < w; ++i) plan_free(&plans[i], L);
+        pos += end;
+        R->waves++;
+    }
+    free(plans);
+    return (orc_rnet*)R;
+}
+
+void orc_rnet_free(orc_rnet* h) {
+    rnet_t* R = (rnet_t*)h;
+    if (!R) return;
+    for (size_t i = 0; i < (size_t)(R->max_level + 1) * R->n; ++i) free(R->rows[i].a);
+    free(R->rows);
+    for (uint32_t l = 0; l <= R->max_level; ++l) free(R->bucket[l]);
+    free(R->bucket);
+    free(R->bucket_n);
+    free(R->highest);
+    free(R);
+}
+uint32_t orc_rnet_max_level(const orc_rnet* h) { return ((const rnet_t*)h)->max_level; }
+uint32_t orc_rnet_top(const orc_rnet* h) { return ((const rnet_t*)h)->top; }
+uint32_t orc_rnet_waves(const orc_rnet* h) { return ((const rnet_t*)h)->waves; }
+const uint32_t* orc_rnet_highest(const orc_rnet* h) { return ((const rnet_t*)h)->highest; }
+/* Rows of level l, by vid: counts[v] and the entries (vid_status, dist)
+ * back to back; returns the total. */
+uint64_t orc_rnet_export_level(const orc_rnet* h, uint32_t l, uint32_t* counts, orc_nbr* entries, uint64_t max) {
+    const rnet_t* R = (const rnet_t*)h;
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < R->n; ++v) {
+        const nvec* r = rnet_row(R, l, v);
+        if (counts) counts[v] = r->n;
+        for (uint32_t k = 0; k < r->n; ++k, ++pos)
+            if (entries && pos < max) entries[pos] = r->a[k];
+    }
+    return pos;
+}
+
+/* The artea_graph build (index/artea_graph/index_factory.hpp:135-410): r-net
+ * insertion, then refine_layer on every occupied level with at least
+ * min_layer_cap (32) vertices (rows imported sorted by (distance, vid),
+ * L0 topped up by the random updater, upper levels from their r-net rows),
+ * the build loops and the routing loops, the writeback of the first
+ * max_nbr_size entries, and the compactor with the insertion order. */
+orc_graph* orc_build_full_rnet(const float* vecs, uint32_t n, uint32_t D, int build_metric, const orc_rnet_params* rp,
+                               uint32_t build_loops, uint32_t triu_iters, float prefill_ratio, float shifted_coeffs,
+                               uint32_t routing_loops, uint32_t routing_topk, uint32_t routing_queue, int nthreads,
+                               double* secs) {
+    const double t0 = omp_get_wtime();
+    rnet_t* R = (rnet_t*)orc_rnet_build(vecs, n, D, build_metric, rp, nthreads);
+    if (secs) secs[0] = omp_get_wtime() - t0;
+    const uint32_t L = R->max_level;
+    uint32_t* members = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint64_t** offs = (uint64_t**)calloc(L + 1, sizeof(uint64_t*));
+    uint32_t** ids = (uint32_t**)calloc(L + 1, sizeof(uint32_t*));
+    const uint32_t top = R->top == ORC_INVALID_LEVEL ? 0 : R->top;
+    for (uint32_t l = 0; l <= L; ++l) {
+        offs[l] = (uint64_t*)calloc((uint64_t)n + 1, sizeof(uint64_t));
+        uint32_t nl = 0;
+        for (uint32_t v = 0; v < n; ++v)
+            if (R->highest[v] != ORC_INVALID_LEVEL && R->highest[v] >= l) members[nl++] = v;
+        const uint32_t slot_cap = rnet_cap(R, l);
+        uint32_t* cnt = (uint32_t*)calloc(nl ? nl : 1, sizeof(uint32_t));
+        orc_nbr** rowp = (orc_nbr**)calloc(nl ? nl : 1, sizeof(orc_nbr*));
+        orc_layer* Ly = NULL;
+        if (l <= top && nl >= 32) {
+            const uint32_t cap = (uint32_t)((float)slot_cap * 1.5f);
+            Ly = orc_layer_create(vecs, n, D, build_metric, cap, nl == n ? NULL : members, nl);
+            for (uint32_t i = 0; i < nl; ++i) {
+                const nvec* r = rnet_row(R, l, members[i]);
+                orc_layer_fill_row(Ly, i, r->a, r->n);
+            }
+            if (l == 0) {
+                const uint32_t g = (uint32_t)((float)cap * prefill_ratio);
+                if (g > 0) orc_next_prefill(Ly, g, 0, nl, 0, nl, rp->seed, 0, nthreads);
+            }
+            const float shift = l == 0 ? shifted_coeffs * rp->l0_min_distance : 0.0f;
+            orc_refine_schedule(Ly, build_loops, triu_iters, rp->scale, shift, nthreads);
+            orc_routing_loops(Ly, routing_loops, routing_topk, routing_queue, rp->scale, nthreads);
+            for (uint32_t i = 0; i < nl; ++i) {
+                cnt[i] = Ly->rows[i].n < slot_cap ? Ly->rows[i].n : slot_cap;
+                rowp[i] = Ly->rows[i].a;
+            }
+        } else {
+            for (uint32_t i = 0; i < nl; ++i) {
+                const nvec* r = rnet_row(R, l, members[i]);
+                cnt[i] = r->n < slot_cap ? r->n : slot_cap;
+                rowp[i] = r->a;
+            }
+        }
+        for (uint32_t i = 0; i < nl; ++i) offs[l][members[i] + 1] = cnt[i];
+        for (uint32_t v = 0; v < n; ++v) offs[l][v + 1] += offs[l][v];
+        ids[l] = (uint32_t*)malloc(sizeof(uint32_t) * (offs[l][n] ? offs[l][n] : 1));
+        for (uint32_t i = 0; i < nl; ++i) {
+            const uint32_t v = members[i];
+            for (uint32_t k = 0; k < cnt[i]; ++k) ids[l][offs[l][v] + k] = rowp[i][k].vid_status & ORC_INVALID_VID;
+        }
+        if (Ly) orc_layer_free(Ly);
+        free(cnt);
+        free(rowp);
+    }
+    if (secs) secs[1] = omp_get_wtime() - t0;
+    uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for (uint32_t v = 0; v < n; ++v) order[v] = v;
+    orc_graph* gr = orc_compact_graph_ordered(n, L, rp->ul_max, rp->bl_max, R->highest, (const uint64_t* const*)offs,
+                                              (const uint32_t* const*)ids, order, vecs, D, build_metric, NULL, 0);
+    for (uint32_t l = 0; l <= L; ++l) {
+        free(offs[l]);
+        free(ids[l]);
+    }
+    free(offs);
+    free(ids);
+    free(order);
+    free(members);
+    orc_rnet_free((orc_rnet*)R);
+    return gr;
+}
+
+/* HierarchicalPruningUpdater::update_impl on a caller's list (sorted by
+ * distance); returns the kept count, the list rewritten in place. */
+uint32_t orc_hier_prune(const float* vecs, uint32_t D, int metric, orc_nbr* a, uint32_t m, uint32_t max_nbr,
+                        float scale) {
+    rnet_t R;
+    memset(&R, 0, sizeof R);
+    R.vecs = vecs;
+    R.D = D;
+    R.metric = metric;
+    R.p.scale = scale;
+    return rnet_prune(&R, a, m, max_nbr);
+}

@@ -239,6 +239,35 @@ orc_graph* orc_build_full_routing(const float* vecs, uint32_t n, uint32_t D, int
                                   float l0_min_distance, uint64_t seed, float level_fraction, uint32_t routing_loops,
                                   uint32_t routing_topk, uint32_t routing_queue, int nthreads, double* secs);
 
+/* R-net insertion (index/stacked_rgraph/index_factory.hpp:192-583) in waves;
+ * see artea_oracle.c.  RGraphConfig (stacked_rgraph/configs.hpp:65-138):
+ * beta, tau_k, l0_min_distance, search_nn_qs, ul/bl select queues and
+ * max_nbr_size; PruningConfig's scale (the insertion path's RNG scale). */
+typedef struct {
+    float beta, tau_k, l0_min_distance, scale;
+    uint32_t search_qs, ul_select_qs, bl_select_qs, ul_max, bl_max;
+    uint32_t insert_on_l0;
+    uint32_t wave_div, wave_max;  /* wave length: min(wave_max, inserted / wave_div) after 10 serial */
+    uint64_t seed;                /* apex entry draws */
+} orc_rnet_params;
+typedef struct orc_rnet orc_rnet;
+float orc_rnet_radius(float l0_min_distance, float tau_k, float beta, uint32_t h);
+uint32_t orc_rnet_wave(uint32_t inserted, uint32_t wave_div, uint32_t wave_max);
+orc_rnet* orc_rnet_build(const float* vecs, uint32_t n, uint32_t D, int metric, const orc_rnet_params* p,
+                         int nthreads);
+void orc_rnet_free(orc_rnet* h);
+uint32_t orc_rnet_max_level(const orc_rnet* h);
+uint32_t orc_rnet_top(const orc_rnet* h);
+uint32_t orc_rnet_waves(const orc_rnet* h);
+const uint32_t* orc_rnet_highest(const orc_rnet* h);
+uint64_t orc_rnet_export_level(const orc_rnet* h, uint32_t l, uint32_t* counts, orc_nbr* entries, uint64_t max);
+uint32_t orc_hier_prune(const float* vecs, uint32_t D, int metric, orc_nbr* a, uint32_t m, uint32_t max_nbr,
+                        float scale);
+orc_graph* orc_build_full_rnet(const float* vecs, uint32_t n, uint32_t D, int build_metric, const orc_rnet_params* rp,
+                               uint32_t build_loops, uint32_t triu_iters, float prefill_ratio, float shifted_coeffs,
+                               uint32_t routing_loops, uint32_t routing_topk, uint32_t routing_queue, int nthreads,
+                               double* secs);
+
 /* Scoring and dataset probes: RecallEstimator (utils/recall_estimator.hpp:
  * 54-137) as the total of per-query matches, ADREstimator::calculate_adr_at_1
  * (utils/adr_estimator.hpp), DatasetProber::_compute_knn_table and

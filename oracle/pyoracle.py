@@ -109,6 +109,17 @@ def _declare(l):
     d("orc_build_full_routing", P, P, u32, u32, i32, u32, u32, u32, u32, u32, f32, f32, f32, f32, u64, f32, u32,
       u32, u32, i32, P)
     d("orc_next_routing", None, P, u32, u32, i32)
+    d("orc_rnet_radius", f32, f32, f32, f32, u32)
+    d("orc_rnet_wave", u32, u32, u32, u32)
+    d("orc_rnet_build", P, P, u32, u32, i32, P, i32)
+    d("orc_rnet_free", None, P)
+    d("orc_rnet_max_level", u32, P)
+    d("orc_rnet_top", u32, P)
+    d("orc_rnet_waves", u32, P)
+    d("orc_rnet_highest", C.POINTER(C.c_uint32), P)
+    d("orc_rnet_export_level", u64, P, u32, P, P, u64)
+    d("orc_hier_prune", u32, P, u32, i32, P, u32, u32, f32)
+    d("orc_build_full_rnet", P, P, u32, u32, i32, P, u32, u32, f32, f32, u32, u32, u32, i32, P)
     d("orc_next_routing_range", None, P, u32, u32, u32, u32, i32)
     d("orc_layer_set_rows_raw", None, P, P, P)
     d("orc_next_pruning", None, P, f32, i32)
@@ -510,3 +521,82 @@ def probe_farthest(metric, base, samples, nthreads=0):
     out = np.zeros(len(s), np.float32)
     lib().orc_probe_farthest(metric, _ptr(b), b.shape[0], b.shape[1], _ptr(s), len(s), _ptr(out), nthreads)
     return out
+
+
+class RnetParams(C.Structure):
+    """orc_rnet_params: RGraphConfig + the insertion path's RNG scale + waves."""
+    _fields_ = [("beta", C.c_float), ("tau_k", C.c_float), ("l0_min_distance", C.c_float), ("scale", C.c_float),
+                ("search_qs", C.c_uint32), ("ul_select_qs", C.c_uint32), ("bl_select_qs", C.c_uint32),
+                ("ul_max", C.c_uint32), ("bl_max", C.c_uint32), ("insert_on_l0", C.c_uint32),
+                ("wave_div", C.c_uint32), ("wave_max", C.c_uint32), ("seed", C.c_uint64)]
+
+
+def rnet_params(beta=2.0, tau_k=0.0, l0_min_distance=1.0, scale=1.10, search_qs=64, ul_select_qs=100,
+                bl_select_qs=100, ul_max=32, bl_max=64, insert_on_l0=False, wave_div=8, wave_max=32768, seed=0):
+    return RnetParams(beta, tau_k, l0_min_distance, scale, search_qs, ul_select_qs, bl_select_qs, ul_max, bl_max,
+                      int(insert_on_l0), wave_div, wave_max, seed)
+
+
+class RNet:
+    """The oracle's r-net insertion result: levels and rows per level."""
+
+    def __init__(self, vecs, metric, params, nthreads=0):
+        self.vecs = _f32(vecs)
+        self.params = params
+        self.h = lib().orc_rnet_build(_ptr(self.vecs), self.vecs.shape[0], self.vecs.shape[1], metric,
+                                      C.byref(params), nthreads)
+        self.n = self.vecs.shape[0]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_rnet_free(self.h)
+            self.h = None
+
+    @property
+    def max_level(self):
+        return lib().orc_rnet_max_level(self.h)
+
+    @property
+    def top(self):
+        return lib().orc_rnet_top(self.h)
+
+    @property
+    def waves(self):
+        return lib().orc_rnet_waves(self.h)
+
+    def highest(self):
+        p = lib().orc_rnet_highest(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.n,)).copy()
+
+    def level_rows(self, level):
+        counts = np.zeros(self.n, np.uint32)
+        total = lib().orc_rnet_export_level(self.h, level, _ptr(counts), None, 0)
+        ent = np.zeros(max(total, 1), NBR_DTYPE)
+        lib().orc_rnet_export_level(self.h, level, _ptr(counts), _ptr(ent), total)
+        return counts, ent[:total]
+
+
+def rnet_radius(l0_min_distance, tau_k, beta, h):
+    return float(lib().orc_rnet_radius(l0_min_distance, tau_k, beta, h))
+
+
+def rnet_wave(inserted, wave_div, wave_max):
+    return int(lib().orc_rnet_wave(inserted, wave_div, wave_max))
+
+
+def build_full_rnet(vecs, params, metric=L2, build_loops=15, triu_iters=4, prefill=0.4, shift=1.5, routing_loops=0,
+                    routing_topk=64, routing_queue=96, nthreads=0):
+    """The artea_graph build: r-net insertion, refine_layer per level,
+    compaction; returns (Graph, [r-net seconds, total seconds])."""
+    v = _f32(vecs)
+    secs = np.zeros(2, np.float64)
+    h = lib().orc_build_full_rnet(_ptr(v), v.shape[0], v.shape[1], metric, C.byref(params), build_loops, triu_iters,
+                                  prefill, shift, routing_loops, routing_topk, routing_queue, nthreads, _ptr(secs))
+    return Graph(h), secs
+
+
+def hier_prune(vecs, metric, entries, max_nbr, scale):
+    v = _f32(vecs)
+    e = np.ascontiguousarray(entries, NBR_DTYPE).copy()
+    k = lib().orc_hier_prune(_ptr(v), v.shape[1], metric, _ptr(e), len(e), max_nbr, scale)
+    return e[:k]
