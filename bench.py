@@ -62,6 +62,56 @@ def recall(pred, gt, k):
     return hits / (len(pred) * k)
 
 
+def gpu_build(idx, args, seed, join=0):
+    """The index build with the C0 parameters (build_loops 15, triu_iters 4,
+    prefill 0.4, scale 1.10, shift 1.5, l0_min_distance 1): the reference's
+    pipeline (r-net insertion with beta 2, tau_k 0, search queue 64, select
+    queues 100, then refine_layer per level and compaction), or with
+    --hierarchy geometric the seeded geometric level draw.  Returns seconds
+    per phase or per level."""
+    from artea_amd.refine import build_full, build_full_rnet, rnet_params
+
+    l0 = resolve_l0(idx, args, seed)
+    if args.hierarchy == "rnet":
+        rs, total = build_full_rnet(idx, rnet_params(seed=seed, l0_min_distance=l0), build_loops=15, triu_iters=4,
+                                    prefill=0.4, shift=1.5, join=join)
+        return np.array([rs, total - rs])
+    return build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
+                      l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction, join=join)
+
+
+def resolve_l0(idx, args, seed):
+    """l0_min_distance: the given value, or with --l0-min-distance probe the
+    reference's auto-probe (probe's first column at quantile 0.9 over 500
+    seeded samples, unit_tests/test_hierarchical_graph_persistence.cpp
+    :125-137) on the GPU, in the build metric."""
+    if args.l0_min_distance != "probe":
+        return float(args.l0_min_distance)
+    if getattr(args, "_l0_probed", None) is None or args._l0_probed[0] != idx.n:
+        from artea_amd import estimators as E
+        from artea_amd.device import DeviceBuffer
+
+        view = DeviceBuffer.borrow(idx.device_vectors(), (idx.n, idx.dim_pad), np.float32)
+        samples = np.random.default_rng(seed).integers(0, idx.n, min(500, idx.n)).astype(np.uint32)
+        bmetric = 3 if idx.metric == 0 else idx.metric
+        t0 = time.time()
+        val = E.l0_min_distance(bmetric, view, idx.n, idx.dim_pad, 0.9, samples)
+        log("auto-probed l0_min_distance %.6f in %.2fs" % (val, time.time() - t0))
+        args._l0_probed = (idx.n, val)
+    return args._l0_probed[1]
+
+
+def cpu_build(orc, base, args, seed, cores, l0):
+    """The same build by the oracle on the host cores (l0_min_distance as the
+    GPU build used)."""
+    if args.hierarchy == "rnet":
+        return orc.build_full_rnet(base, orc.rnet_params(seed=seed, l0_min_distance=l0), metric=orc.L2,
+                                   build_loops=15, triu_iters=4, prefill=0.4, shift=1.5, nthreads=cores)
+    return orc.build_full(base, metric=orc.L2, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4,
+                          scale=1.10, shift=1.5, l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction,
+                          nthreads=cores)
+
+
 def sweep_queues(aa, idx, q_d, nq, gt_i, topk, queues, stream, target):
     """Batched router at each queue size (autotuned, HIP-graph replay);
     returns per-queue rows and the smallest queue reaching the target."""
@@ -108,8 +158,7 @@ def workload_cosine(args):
     D = base.shape[1]
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.COSINE)
     t0 = time.time()
-    secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                      l0_min_distance=1.0, seed=seed, level_fraction=args.level_fraction)
+    secs = gpu_build(idx, args, seed)
     build_s = time.time() - t0
     log("cosine build %.1fs (levels %s, top %d)" % (build_s, np.round(secs, 2).tolist(), idx.info()["top"]))
     stream = Stream()
@@ -121,7 +170,7 @@ def workload_cosine(args):
     out = {"metric": "QPS @ recall>=0.95 (1M x 100 cosine)", "unit": "queries/s", "n_gpus": 1,
            "higher_is_better": True, "dtype": "f32", "data": "synthetic (power-law-spectrum Gaussian, unit-norm)",
            "config": {"workload": "cosine search, 1,000,000 x 100 (pad 112), %d queries" % nq, "n": n, "nq": nq},
-           "build_seconds": round(build_s, 2), "level_fraction": args.level_fraction}
+           "build_seconds": round(build_s, 2), "hierarchy": args.hierarchy, "level_fraction": args.level_fraction}
     for topk in (10, 100):
         queues = [L for L in (64, 100, 200, 400, 800) if L >= topk]
         rows, best = sweep_queues(aa, idx, q_d, nq, gt_i[:, :topk], topk, queues, stream, args.target_recall)
@@ -179,13 +228,11 @@ def workload_c0(args):
     n, nq, D, seed, k, L = 100_000, 1000, 128, 20261203, 100, 200
     base = aa.generate(0, seed, 0, n, D)
     queries = aa.generate(0, seed, 1, nq, D)
-    params = dict(ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                  l0_min_distance=1.0, seed=seed)
     idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
     t0 = time.time()
-    aa.build_full(idx, **params)
-    gpu_build = time.time() - t0
-    log("C0 GPU build %.2fs" % gpu_build)
+    gpu_build(idx, args, seed)
+    gpu_build_s = time.time() - t0
+    log("C0 GPU build %.2fs (%s hierarchy)" % (gpu_build_s, args.hierarchy))
     stream = Stream()
     q_d = DeviceBuffer.from_numpy(queries)
     view = DeviceBuffer.borrow(idx.device_vectors(), base.shape, np.float32)
@@ -225,7 +272,8 @@ def workload_c0(args):
            "dtype": "u8" if idx.u8_rows else "f32",
            "data": "synthetic (seeded 256-component integer Gaussian mixture)",
            "config": {"workload": "C0: 100,000 x 128 L2, 1,000 queries, k 100, queue 200", "n": n, "nq": nq},
-           "gpu_build_s": round(gpu_build, 3), "recall@100": round(rec, 4), "plain_qps": round(nq / t_plain, 1),
+           "gpu_build_s": round(gpu_build_s, 3), "hierarchy": args.hierarchy, "recall@100": round(rec, 4),
+           "plain_qps": round(nq / t_plain, 1),
            "batched_qps": round(nq / t_batched, 1), "batched_equals_plain": same_plain}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -234,8 +282,8 @@ def workload_c0(args):
         orc.build()
         cores = len(os.sched_getaffinity(0))
         t0 = time.time()
-        g, _ = orc.build_full(base, metric=orc.L2, nthreads=cores, **params)
-        cpu_build = time.time() - t0
+        g, _ = cpu_build(orc, base, args, seed, cores, resolve_l0(idx, args, seed))
+        cpu_build_s = time.time() - t0
         with tempfile.TemporaryDirectory() as d:
             a, b = os.path.join(d, "gpu.graph"), os.path.join(d, "cpu.graph")
             idx.snapshot_hgra(a)
@@ -249,12 +297,13 @@ def workload_c0(args):
         cpu_qps = nq * reps / (time.time() - t0)
         same_cpu = bool(np.array_equal(cv, gv) and np.array_equal(cd, gd))
         log("C0 CPU oracle on %d cores: build %.1fs (HGRA identical to the GPU's: %s), search %.0f q/s, results equal "
-            "%s" % (cores, cpu_build, hgra_same, cpu_qps, same_cpu))
-        out["cpu_baseline"] = {"build_s": round(cpu_build, 2), "value": round(cpu_qps, 1), "unit": "queries/s",
+            "%s" % (cores, cpu_build_s, hgra_same, cpu_qps, same_cpu))
+        out["cpu_baseline"] = {"build_s": round(cpu_build_s, 2), "value": round(cpu_qps, 1), "unit": "queries/s",
                                "cores": cores, "kind": "port", "hgra_identical_to_gpu": hgra_same,
                                "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
                                "results_equal_gpu": same_cpu,
-                               "sample": "the whole C0 workload: oracle build_full and batch_query, OpenMP"}
+                               "sample": "the whole C0 workload: oracle build (same pipeline) and batch_query, "
+                                         "OpenMP"}
     print(json.dumps(out))
 
 
@@ -420,8 +469,9 @@ def workload_build(args):
     join and with the BF16 MFMA join; edge-set equality, recall@100 of the
     resulting index, then the serving leg on the same resident index
     (open-loop Poisson arrivals, topk {10,50,100}, queue {64,128,256,512,800}).
-    On unit-norm data the r-net's covering radii (1, 2, ...) absorb almost
-    every vertex, so the reference's upper levels are sparse; layer 0 alone
+    The hierarchy comes from r-net insertion (--hierarchy rnet, the
+    reference's pipeline): on unit-norm data its covering radii (1, 2, ...)
+    absorb almost every vertex, so the upper levels are small; layer 0 alone
     does not route here (the 1.5 shift exceeds every distance, leaving a
     plain k-NN graph whose clusters do not link up)."""
     import hashlib
@@ -440,12 +490,13 @@ def workload_build(args):
     graphs = {}
     for name, join in (("fp32", JOIN_FP32), ("bf16", JOIN_BF16)):
         t0 = time.time()
-        lv = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-                        shift=1.5, l0_min_distance=1.0, seed=seed, join=join, level_fraction=args.level_fraction)
+        lv = gpu_build(idx, args, seed, join=join)
         secs = time.time() - t0
         g = idx.export_graph()
         digest = hashlib.sha256(g.nbrs.tobytes() + b"".join(o.tobytes() for o in g.group_offsets)).hexdigest()[:16]
-        builds[name] = {"seconds": round(secs, 2), "layer_seconds": np.round(lv, 2).tolist(),
+        builds[name] = {"seconds": round(secs, 2),
+                        ("rnet_then_refine_seconds" if args.hierarchy == "rnet" else "layer_seconds"):
+                            np.round(lv, 2).tolist(),
                         "edges": int(g.nbrs.size), "digest": digest}
         graphs[name] = g
         log("%s build: %.1fs (layers %s), %d edges, digest %s" % (name, secs, np.round(lv, 2).tolist(), g.nbrs.size,
@@ -468,7 +519,9 @@ def workload_build(args):
            "data": "synthetic (1024-component mixture, unit-norm)",
            "config": {"workload": "hierarchy build (every level) + serving, %d x 96 L2" % n, "n": n, "nq": nq},
            "builds": builds, "bf16_speedup": round(builds["fp32"]["seconds"] / builds["bf16"]["seconds"], 3),
-           "edge_sets_identical": same, "top": idx.info()["top"], "level_fraction": args.level_fraction,
+           "edge_sets_identical": same, "top": idx.info()["top"], "hierarchy": args.hierarchy,
+           "l0_min_distance": resolve_l0(idx, args, seed),
+           "level_fraction": args.level_fraction,
            "ground_truth_s": round(gt_s, 2), "search_sweep": rows, "search_chosen": best}
     if args.serve_requests > 0:
         from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
@@ -549,13 +602,10 @@ def workload_build(args):
         sub = np.ascontiguousarray(base[:m])
         sidx = aa.DeviceIndex(sub, aa.DistanceMetrics.EUCLIDEAN_SQR)
         t0 = time.time()
-        build_full(sidx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                   l0_min_distance=1.0, seed=seed, join=JOIN_FP32, level_fraction=args.level_fraction)
+        gpu_build(sidx, args, seed, join=JOIN_FP32)
         gpu_s = time.time() - t0
         t0 = time.time()
-        g, _ = orc.build_full(sub, metric=orc.L2, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4,
-                              scale=1.10, shift=1.5, l0_min_distance=1.0, seed=seed,
-                              level_fraction=args.level_fraction, nthreads=cores)
+        g, _ = cpu_build(orc, sub, args, seed, cores, resolve_l0(sidx, args, seed))
         cpu_s = time.time() - t0
         with tempfile.TemporaryDirectory() as d:
             a, b = os.path.join(d, "gpu.graph"), os.path.join(d, "cpu.graph")
@@ -564,8 +614,8 @@ def workload_build(args):
             hgra_same = open(a, "rb").read() == open(b, "rb").read()
         out["cpu_baseline"] = {"value": round(cpu_s, 2), "unit": "s", "cores": cores, "kind": "port",
                                "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
-                               "sample": "whole build of the first %d vectors (FP32 join), oracle build_full with "
-                                         "OpenMP" % m,
+                               "sample": "whole build of the first %d vectors (FP32 join), the oracle's build "
+                                         "(same pipeline) with OpenMP" % m,
                                "gpu_seconds_same_sample": round(gpu_s, 2), "hgra_identical_to_gpu": hgra_same}
         log("CPU oracle build of the first %d vectors: %.1fs on %d cores (GPU %.2fs), HGRA identical %s" % (
             m, cpu_s, cores, gpu_s, hgra_same))
@@ -577,6 +627,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--join", choices=("fp32", "bf16"), default="fp32",
                     help="triangle-join pair distances of the search workload's build (the graph is the same)")
+    ap.add_argument("--hierarchy", choices=("rnet", "geometric"), default="rnet",
+                    help="levels and upper-level rows from r-net insertion (the reference's pipeline) or from the "
+                         "seeded geometric draw with a random prefill")
+    ap.add_argument("--l0-min-distance", default="1.0",
+                    help="a value, or 'probe' for the reference's auto-probe (quantile 0.9, 500 samples)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
     ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing"), default="search",
@@ -658,9 +713,7 @@ def main():
     else:
         from artea_amd.refine import JOIN_BF16, JOIN_FP32
 
-        secs = build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10,
-                          shift=1.5, l0_min_distance=1.0, seed=seed, level_fraction=args.level_fraction,
-                          join=JOIN_BF16 if args.join == "bf16" else JOIN_FP32)
+        secs = gpu_build(idx, args, seed, join=JOIN_BF16 if args.join == "bf16" else JOIN_FP32)
         if args.graph_cache:
             idx.snapshot_hgra(args.graph_cache)
     build_s = time.time() - t0
