@@ -375,6 +375,73 @@ def workload_probe(args):
     print(json.dumps(out))
 
 
+def workload_rnet(args):
+    """R-net insertion of the 1,000,000 x 128 L2 workload (beta 2, tau_k 0,
+    l0_min_distance 1, search queue 64, select queues 100, max 32 / 64, scale
+    1.10, waves of inserted/8 up to 32768 after 10 serial vertices).  One step
+    = the whole insertion.  The roofline counts 512 bytes (one FP32 row) per
+    distance evaluated."""
+    import artea_amd as aa
+    from artea_amd.refine import RNet, rnet_params
+
+    n, dim, seed = args.n, 128, 20261204
+    base = aa.generate(0, seed, 0, n, dim)
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    p = rnet_params(seed=seed)
+    for _ in range(args.warmup):
+        RNet(idx, p)
+    secs, info, levels = [], None, None
+    for _ in range(args.steps):
+        t0 = time.time()
+        r = RNet(idx, p)
+        secs.append(time.time() - t0)
+        info, levels = r.info(), np.bincount(r.highest()).tolist()
+        del r
+    step = float(np.median(secs))
+    achieved = info["distances"] * 512 / step / 1e9
+    out = {"metric": "r-net insertion vertices/s (1M x 128 L2)", "value": round(n / step, 1), "unit": "vertices/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step * 1e3, 1),
+           "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (integer-rounded 256-component mixture)",
+           "config": {"workload": "r-net insertion, %d x 128 L2, waves of inserted/8 up to 32768" % n, "n": n},
+           "levels": levels, "waves": info["waves"], "distances_per_vertex": round(info["distances"] / n, 1),
+           "unresolved": info["unresolved"],
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "span": "host clock around the whole insertion (every wave's plan, apply and host sync)"}}
+    log("r-net: %.2fs, %.0f vertices/s, %d waves, %.0f distances per vertex, levels %s" % (
+        step, n / step, info["waves"], info["distances"] / n, levels))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        cores = len(os.sched_getaffinity(0))
+        m = min(n, 100_000)
+        sub = np.ascontiguousarray(base[:m])
+        op = orc.rnet_params(seed=seed)
+        t0 = time.time()
+        R = orc.RNet(sub, orc.L2, op, nthreads=cores)
+        cpu_s = time.time() - t0
+        sidx = aa.DeviceIndex(sub, aa.DistanceMetrics.EUCLIDEAN_SQR)
+        t0 = time.time()
+        g = RNet(sidx, p)
+        gpu_s = time.time() - t0
+        same = bool(np.array_equal(g.highest(), R.highest()))
+        for lvl in range(R.max_level + 1):
+            gc, ge = g.level_rows(lvl)
+            oc, oe = R.level_rows(lvl)
+            same = same and np.array_equal(gc, oc) and np.array_equal(ge, oe.view(np.uint32).reshape(-1, 2))
+        out["cpu_baseline"] = {"value": round(m / cpu_s, 1), "unit": "vertices/s", "cores": cores, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
+                               "sample": "insertion of the first %d vectors (the oracle's wave restatement, OpenMP "
+                                         "over each wave)" % m,
+                               "gpu_seconds_same_sample": round(gpu_s, 2), "levels_and_rows_equal_gpu": same}
+        log("r-net CPU oracle on the first %d: %.1fs on %d cores (GPU %.2fs), levels and rows equal %s" % (
+            m, cpu_s, cores, gpu_s, same))
+    print(json.dumps(out))
+
+
 def workload_routing(args):
     """One routing loop of refine_layer (RoutingUpdater topk 64 / queue 96,
     merge, PruningUpdater scale 1.10) on the L0 layer of the 1,000,000 x 128
@@ -634,7 +701,8 @@ def main():
                     help="a value, or 'probe' for the reference's auto-probe (quantile 0.9, 500 samples)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
-    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing"), default="search",
+    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing", "rnet"),
+                    default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 hierarchy build (every level) FP32 vs BF16 join, then serving on that index; "
                          "c0: 100K x 128 build and search, GPU vs the CPU oracle")
@@ -677,6 +745,8 @@ def main():
         return workload_c0(args)
     if args.workload == "routing":
         return workload_routing(args)
+    if args.workload == "rnet":
+        return workload_rnet(args)
     if args.workload == "probe":
         if args.n == 1_000_000:
             args.n = 10_000_000
