@@ -42,6 +42,7 @@ constexpr uint64_t kRnetStream = 0x726E6574ull;  // entry draws
 constexpr uint32_t kTouched = 16384;            // bitmap words remembered for clearing
 constexpr uint32_t kCandMax = 256;              // select results per level
 constexpr uint32_t kRowMax = 128;               // row capacity
+constexpr uint32_t kRowSlots = kRowMax / 64;    // row entries per lane
 constexpr uint32_t kMaxLevels = 64;
 constexpr uint32_t kRnetOverflowList = 1024;    // vertices to plan again with a longer list
 constexpr size_t kRnetStatsBytes = 4 * sizeof(unsigned long long) + kRnetOverflowList * sizeof(uint32_t);
@@ -468,15 +469,27 @@ __global__ void seg_head_kernel(const unsigned long long* keys, uint32_t m, uint
 // _write_reverse_edges (index_factory.hpp:528-580) for one target row: the
 // records in vertex order; an append while the row has room, else the row
 // plus the new entry ordered by distance (stable) and re-pruned.
+//
+// A full row whose entries are all old (every row after its first re-prune)
+// takes a shortcut with the same result: old/old pairs are never compared,
+// so the old entries before the new one stay, the new one stays iff none of
+// them is closer to it than its distance / scale, and a later old entry
+// goes iff the new one stayed and is closer to it than its own distance /
+// scale; the first cap survivors are kept.  The new entry's distances to
+// the row are computed in one pass, one lane per entry.
 __global__ __launch_bounds__(256) void reverse_apply_kernel(RnetArgs r, const unsigned long long* keys,
                                                             const uint32_t* vals, const uint32_t* heads,
-                                                            const uint32_t* n_heads, uint32_t m) {
+                                                            const uint32_t* n_heads, uint32_t m, uint32_t* seg_work) {
     __shared__ uint32_t s_rv[4][kRowMax + 1], s_tv[4][kRowMax + 1];
     __shared__ float s_rd[4][kRowMax + 1], s_td[4][kRowMax + 1];
     const uint32_t wid = threadIdx.x / 64, lane = lane_id();
     const uint32_t nh = *n_heads;
     unsigned long long pairs = 0;
-    for (uint32_t sgi = blockIdx.x * 4 + wid; sgi < nh; sgi += gridDim.x * 4) {
+    while (true) {
+        uint32_t sgi = 0;
+        if (lane == 0) sgi = atomicAdd(seg_work, 1u);
+        sgi = readfirst_u(sgi);
+        if (sgi >= nh) break;
         const uint32_t b = heads[sgi], e = sgi + 1 < nh ? heads[sgi + 1] : m;
         const unsigned long long k0 = keys[b];
         const uint32_t l = uint32_t(k0 >> 58), t = uint32_t((k0 >> 27) & kMaskR);
@@ -503,9 +516,59 @@ __global__ __launch_bounds__(256) void reverse_apply_kernel(RnetArgs r, const un
                 wave_sync();
                 continue;
             }
-            // cap + 1 entries ordered by distance, ties in place order
             uint32_t* tv = s_tv[wid];
             float* td = s_td[wid];
+            bool irregular = false;  // a new entry, or a row out of distance order
+            uint32_t pn = 0;         // the new entry's place: after every entry not farther
+            for (uint32_t j = lane; j < cap; j += 64) {
+                irregular |= (rv[j] & kNewR) != 0 || (j > 0 && rd[j - 1] > rd[j]);
+                pn += rd[j] <= qd ? 1u : 0u;
+            }
+            if (ballot(irregular) == 0) {
+                for (int off = 32; off >= 1; off >>= 1) pn += __shfl_xor(pn, off);
+                if (pn >= cap) continue;  // the new entry is past the cap: nothing changes
+                const float* qvec = r.vecs + size_t(qv & kMaskR) * r.D;
+                const float thr_new = qd * r.inv_scale;
+                float dn[kRowSlots];
+                bool hit_before = false;
+#pragma unroll
+                for (uint32_t s = 0; s < kRowSlots; ++s) {
+                    const uint32_t j = s * 64 + lane;
+                    dn[s] = 0.f;
+                    if (j < cap) {
+                        dn[s] = lane_distance(qvec, r.vecs + size_t(rv[j] & kMaskR) * r.D, r.C, r.metric);
+                        hit_before |= j < pn && dn[s] < thr_new;
+                    }
+                }
+                pairs += cap;
+                if (pn > 0 && ballot(hit_before) != 0) continue;  // the new entry is pruned: nothing changes
+                // The new entry stays; the old entries from pn on stay unless
+                // it is closer to them than their distance / scale.
+                uint32_t out = pn + 1;
+#pragma unroll
+                for (uint32_t s = 0; s < kRowSlots; ++s) {
+                    const uint32_t j = s * 64 + lane;
+                    const bool keep = j >= pn && j < cap && !(dn[s] < rd[j] * r.inv_scale);
+                    const uint64_t km = ballot(keep);
+                    const uint32_t at = out + lanes_below(km);
+                    if (keep && at < cap) {
+                        tv[at] = rv[j];
+                        td[at] = rd[j];
+                    }
+                    out += uint32_t(__popcll(km));
+                }
+                wave_sync();
+                const uint32_t nn = min(out, cap);
+                for (uint32_t j = pn + lane; j < nn; j += 64) {
+                    rv[j] = j == pn ? (qv & kMaskR) : tv[j];
+                    rd[j] = j == pn ? qd : td[j];
+                }
+                n = nn;
+                wave_sync();
+                continue;
+            }
+            // general case: cap + 1 entries ordered by distance, ties in
+            // place order, then the hierarchical pruning
             for (uint32_t p = lane; p <= cap; p += 64) {
                 const float dp = p < cap ? rd[p] : qd;
                 uint32_t rank = 0;
@@ -711,6 +774,8 @@ int rnet_run(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric, cons
     RNET_TRY(B.alloc(&flags, max_rec));
     RNET_TRY(B.alloc(&heads, max_rec));
     RNET_TRY(B.alloc(&n_heads, 1));
+    uint32_t* seg_work = nullptr;
+    RNET_TRY(B.alloc(&seg_work, 1));
     size_t tmp_sort = 0, tmp_scan = 0, tmp_sel = 0;
     RNET_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_sort, keys, keys2, vals, vals2, int(max_rec), 0, 64, st));
     RNET_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_scan, rc, roff, int(wmax + 1), st));
@@ -806,8 +871,9 @@ int rnet_run(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric, cons
             hipLaunchKernelGGL(seg_head_kernel, dim3((m + 255) / 256), dim3(256), 0, st, keys2, m, flags);
             tb = tmp_bytes;
             RNET_TRY(hipcub::DeviceSelect::Flagged(tmp, tb, iota, flags, heads, n_heads, int(m), st));
+            RNET_TRY(hipMemsetAsync(seg_work, 0, 4, st));
             hipLaunchKernelGGL(reverse_apply_kernel, dim3(std::max(1u, std::min((m + 3) / 4, uint32_t(cus) * 8u))),
-                               dim3(256), 0, st, a, keys2, vals2, heads, n_heads, m);
+                               dim3(256), 0, st, a, keys2, vals2, heads, n_heads, m, seg_work);
             RNET_TRY(hipGetLastError());
         }
         // levels and the top bucket (assignment order)
