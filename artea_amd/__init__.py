@@ -169,6 +169,22 @@ class DeviceIndex:
     def snapshot_hgra(self, path: str):
         check(lib().artea_index_snapshot_hgra(self.h, path.encode()))
 
+    def layout(self):
+        """Device layout of the attached graph: ELL strides (level 0, upper
+        levels) and the number of upper-level rows."""
+        s0, su, ru = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().artea_index_layout(self.h, C.byref(s0), C.byref(su), C.byref(ru)))
+        return {"stride0": s0.value, "stride_upper": su.value, "rows_upper": ru.value}
+
+    def hgra_seconds(self):
+        """Host-clock phases of the last device HGRA restore (file read and
+        copy, segmented parse, checks and ELL fill) and snapshot (layout on
+        the device, copy back and write, 0)."""
+        load = (C.c_double * 3)()
+        store = (C.c_double * 3)()
+        check(lib().artea_index_hgra_seconds(self.h, load, store))
+        return list(load), list(store)
+
     def compact_graph(self, highest, levels, ul_max: int, bl_max: int, order=None, metric: Optional[int] = None):
         """HierarchicalGraphCompactor::compact_graph on the device
         (compactor/hierarchical_graph_compactor.hpp:118-327): ``highest``

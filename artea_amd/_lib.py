@@ -34,6 +34,9 @@ SIGNATURES = {
     "artea_index_create_device": (i32, [P, u32, u32, i32, PP]),
     "artea_index_set_graph_csr": (i32, [P, u32, u32, u32, u32, P, P, P, u64]),
     "artea_index_set_graph_hgra": (i32, [P, C.c_char_p]),
+    "artea_index_set_graph_hgra_segmented": (i32, [P, C.c_char_p, u32, u32]),
+    "artea_index_hgra_seconds": (i32, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "artea_index_layout": (i32, [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64)]),
     "artea_index_snapshot_hgra": (i32, [P, C.c_char_p]),
     "artea_index_info": (i32, [P, P, P, P, P, P, P]),
     "artea_index_device_vectors": (P, [P]),

@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ARTEA_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["index.hip", "router.hip", "router_noalloc.hip", "refine.hip", "util.hip", "bruteforce.hip", "server.hip",
-               "estimators.hip", "routing.hip", "rnet.hip"]
+               "estimators.hip", "routing.hip", "rnet.hip", "hgra_device.hip"]
 CXX_SOURCES = ["hgra.cpp", "datagen.cpp"]
 # router_inst.hip is compiled once per (metric family, candidate-list blocks)
 # pair; router_kernels.hpp lists the same pairs.

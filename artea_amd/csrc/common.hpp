@@ -36,11 +36,24 @@ struct artea_index {
     float* d_vecs = nullptr;
     uint8_t* d_u8 = nullptr;  // 8-bit copy of d_vecs when every value is an integer in [0, 255]
     DeviceGraph dg;
-    artea::HostCsr csr;  // host copy for snapshot / info
+    artea::HostCsr csr;        // host copy of a graph attached from a host CSR
+    bool csr_current = false;  // csr describes dg (false after a device HGRA load)
+    // Host-clock phases of the last device HGRA call: load = {file read and
+    // copy to the device, segmented parse and walk, checks and ELL fill};
+    // snapshot = {record layout on the device, copy back and file write, 0}.
+    double hgra_load_s[3] = {0, 0, 0}, hgra_store_s[3] = {0, 0, 0};
     hipStream_t stream = nullptr;
 };
 
 namespace artea {
+// Replaces the index's device graph (the previous one is freed).
+void replace_device_graph(artea_index* idx, const DeviceGraph& g);
+// HGRA v1 restore straight into the device layout / snapshot from it
+// (hgra_device.hip).  segment_words = 0 and trial_records = 0 pick the defaults.
+int hgra_load_device(artea_index* idx, const char* path, uint32_t segment_words, uint32_t trial_records);
+int hgra_store_device(artea_index* idx, const char* path);
+// Host grouped CSR of the device graph (local ids in vid order per group).
+int graph_from_device(const artea_index* idx, HostCsr* out);
 // Router internals used by the server: the length of a launch's overflow
 // list, and an asynchronous copy of that list (count, then batch indices)
 // into host memory, ordered after the router's latest launch on `stream`.

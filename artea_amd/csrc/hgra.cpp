@@ -26,10 +26,9 @@ struct File {
 bool rd32(FILE* f, uint32_t* x) { return fread(x, 4, 1, f) == 1; }
 }  // namespace
 
-namespace {
-
-int hgra_read_impl(const char* path, HostCsr* out) {
+int hgra_open(const char* path, FILE** out, HgraHeader* hh) {
     const std::string p = path ? path : "";
+    *out = nullptr;
     File fh;
     fh.f = fopen(p.c_str(), "rb");
     if (!fh.f) {
@@ -65,28 +64,84 @@ int hgra_read_impl(const char* path, HostCsr* out) {
         set_error(buf + p);
         return ARTEA_EIO;
     }
-    HostCsr g;
-    g.n = hdr[2];
-    g.top = hdr[3];
-    g.ul_max = hdr[4];
-    g.bl_max = hdr[5];
-    g.entry = hdr[6];
-    if (g.top == kInvalidLevel || (g.entry != kNoVertex && g.entry >= g.n)) {
+    HgraHeader h;
+    h.n = hdr[2];
+    h.top = hdr[3];
+    h.ul_max = hdr[4];
+    h.bl_max = hdr[5];
+    h.entry = hdr[6];
+    if (h.top == kInvalidLevel || (h.entry != kNoVertex && h.entry >= h.n)) {
         set_error("HierarchicalGraphFileManager::restore: invalid header in " + p);
         return ARTEA_EIO;
     }
     // The device layout addresses at most 64 levels (the reference has no
     // such bound; a graph of ln(N / 1000) levels needs 64 only past 2^92
     // vertices).
-    if (g.top >= 64) {
+    if (h.top >= 64) {
         snprintf(buf, sizeof buf,
                  "HierarchicalGraphFileManager::restore: top_level_id %u exceeds the supported maximum of 63 in ",
-                 g.top);
+                 h.top);
         set_error(buf + p);
         return ARTEA_EIO;
     }
+    h.body_words = file_bytes > 28 ? (file_bytes - 28) / 4 : 0;
+    *hh = h;
+    *out = fh.f;
+    fh.f = nullptr;
+    return ARTEA_OK;
+}
+
+int hgra_fault(uint32_t fault, uint32_t vid, uint32_t level, uint32_t a, uint32_t b, uint32_t top,
+               const std::string& p) {
+    char buf[256];
+    switch (fault) {
+        case kHgraTruncVid:
+            snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u in ", vid);
+            break;
+        case kHgraHighest:
+            snprintf(buf, sizeof buf,
+                     "HierarchicalGraphFileManager::restore: vid %u has highest_level_id %u > top_level_id %u in ",
+                     vid, a, top);
+            break;
+        case kHgraTruncLevel:
+            snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u level %u in ", vid,
+                     level);
+            break;
+        case kHgraCapacity:
+            snprintf(buf, sizeof buf,
+                     "HierarchicalGraphFileManager::restore: vid %u level %u count %u > capacity %u in ", vid, level,
+                     a, b);
+            break;
+        case kHgraTruncNbrs:
+            snprintf(buf, sizeof buf,
+                     "HierarchicalGraphFileManager::restore: truncated reading nbrs of vid %u level %u in ", vid,
+                     level);
+            break;
+        default:
+            snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: invalid neighbor %u in ", a);
+            break;
+    }
+    set_error(buf + p);
+    return ARTEA_EIO;
+}
+
+namespace {
+
+int hgra_read_impl(const char* path, HostCsr* out) {
+    const std::string p = path ? path : "";
+    HgraHeader hh;
+    File fh;
+    int rc = hgra_open(path, &fh.f, &hh);
+    if (rc) return rc;
+    FILE* f = fh.f;
+    HostCsr g;
+    g.n = hh.n;
+    g.top = hh.top;
+    g.ul_max = hh.ul_max;
+    g.bl_max = hh.bl_max;
+    g.entry = hh.entry;
+    const uint64_t body_words = hh.body_words;
     const uint32_t groups = g.top + 1;
-    const uint64_t body_words = file_bytes > 28 ? (file_bytes - 28) / 4 : 0;
     const size_t table = size_t(std::min<uint64_t>(g.n, body_words + 1));
     g.highest.assign(table, kInvalidLevel);
     g.offsets.assign(groups, {});
@@ -97,60 +152,24 @@ int hgra_read_impl(const char* path, HostCsr* out) {
     lens.reserve(table * 2);
     for (uint32_t v = 0; v < g.n; ++v) {
         uint32_t h;
-        if (!rd32(f, &h)) {
-            snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u in ", v);
-            set_error(buf + p);
-            return ARTEA_EIO;
-        }
+        if (!rd32(f, &h)) return hgra_fault(kHgraTruncVid, v, 0, 0, 0, g.top, p);
         g.highest[v] = h;  // v < table: vertices 0..v took at least 4(v + 1) bytes of the body
         if (h == kInvalidLevel) continue;
-        if (h > g.top) {
-            snprintf(buf, sizeof buf,
-                     "HierarchicalGraphFileManager::restore: vid %u has highest_level_id %u > top_level_id %u in ",
-                     v, h, g.top);
-            set_error(buf + p);
-            return ARTEA_EIO;
-        }
+        if (h > g.top) return hgra_fault(kHgraHighest, v, 0, h, 0, g.top, p);
         hist[h]++;
         for (uint32_t l = h;; --l) {
             uint32_t c;
-            if (!rd32(f, &c)) {
-                snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: truncated at vid %u level %u in ",
-                         v, l);
-                set_error(buf + p);
-                return ARTEA_EIO;
-            }
+            if (!rd32(f, &c)) return hgra_fault(kHgraTruncLevel, v, l, 0, 0, g.top, p);
             const uint32_t cap = l == 0 ? g.bl_max : g.ul_max;
-            if (c > cap) {
-                snprintf(buf, sizeof buf,
-                         "HierarchicalGraphFileManager::restore: vid %u level %u count %u > capacity %u in ", v, l,
-                         c, cap);
-                set_error(buf + p);
-                return ARTEA_EIO;
-            }
+            if (c > cap) return hgra_fault(kHgraCapacity, v, l, c, cap, g.top, p);
             lens.push_back(c);
             const size_t at = ids.size();
-            if (uint64_t(at) + c > body_words) {  // more ids than the file holds
-                snprintf(buf, sizeof buf,
-                         "HierarchicalGraphFileManager::restore: truncated reading nbrs of vid %u level %u in ", v, l);
-                set_error(buf + p);
-                return ARTEA_EIO;
-            }
+            if (uint64_t(at) + c > body_words)  // more ids than the file holds
+                return hgra_fault(kHgraTruncNbrs, v, l, 0, 0, g.top, p);
             ids.resize(at + c);
-            if (c && fread(ids.data() + at, 4, c, f) != c) {
-                snprintf(buf, sizeof buf,
-                         "HierarchicalGraphFileManager::restore: truncated reading nbrs of vid %u level %u in ", v, l);
-                set_error(buf + p);
-                return ARTEA_EIO;
-            }
-            for (uint32_t i = 0; i < c; ++i) {
-                if (ids[at + i] >= g.n) {
-                    snprintf(buf, sizeof buf, "HierarchicalGraphFileManager::restore: invalid neighbor %u in ",
-                             ids[at + i]);
-                    set_error(buf + p);
-                    return ARTEA_EIO;
-                }
-            }
+            if (c && fread(ids.data() + at, 4, c, f) != c) return hgra_fault(kHgraTruncNbrs, v, l, 0, 0, g.top, p);
+            for (uint32_t i = 0; i < c; ++i)
+                if (ids[at + i] >= g.n) return hgra_fault(kHgraNeighbor, v, l, ids[at + i], 0, g.top, p);
             if (l == 0) break;
         }
     }

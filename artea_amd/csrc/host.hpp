@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,30 @@ struct HostCsr {
     }
 };
 
+// HGRA v1 restore (persistence/hierarchical_graph_file_manager.hpp:152-324),
+// shared by the host reader and the device loader.
+struct HgraHeader {
+    uint32_t n = 0, top = 0, ul_max = 0, bl_max = 0, entry = 0;
+    uint64_t body_words = 0;  // whole 4-byte words after the 28-byte header
+};
+// Opens `path` and reads and checks the header in the reference's order;
+// on success *f is the open file, positioned at the body.
+int hgra_open(const char* path, std::FILE** f, HgraHeader* h);
+// Failures inside the body, in the order the reference meets them for one
+// record: its highest level (missing, above top), then per level from the
+// highest down its count (missing, above capacity) and its ids (missing,
+// out of range).
+enum HgraFault : uint32_t {
+    kHgraTruncVid = 1,    // vid
+    kHgraHighest = 2,     // vid, a = highest, top
+    kHgraTruncLevel = 3,  // vid, level
+    kHgraCapacity = 4,    // vid, level, a = count, b = capacity
+    kHgraTruncNbrs = 5,   // vid, level
+    kHgraNeighbor = 6,    // a = the id
+};
+// Sets the reference's text for a body failure; returns ARTEA_EIO.
+int hgra_fault(uint32_t fault, uint32_t vid, uint32_t level, uint32_t a, uint32_t b, uint32_t top,
+               const std::string& path);
 int hgra_read(const char* path, HostCsr* g);         // restore semantics + messages
 int hgra_write(const HostCsr& g, const char* path);  // snapshot semantics
 int csr_validate(const HostCsr& g);                 // consistency of a caller-built CSR

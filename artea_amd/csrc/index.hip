@@ -27,6 +27,15 @@ void free_graph(DeviceGraph& g) {
     g = DeviceGraph{};
 }
 
+}  // namespace
+
+void artea::replace_device_graph(artea_index* idx, const DeviceGraph& g) {
+    free_graph(idx->dg);
+    idx->dg = g;
+}
+
+namespace {
+
 // Converts the grouped CSR into the device ELL layout and uploads it.
 int upload_graph(artea_index* idx, const HostCsr& c) {
     DeviceGraph g;
@@ -281,24 +290,21 @@ int artea_index_set_graph_csr(artea_index* idx, uint32_t top, uint32_t ul_max, u
     rc = upload_graph(idx, c);
     if (rc) return rc;
     idx->csr = std::move(c);
+    idx->csr_current = true;
     return ARTEA_OK;
 }
 
 int artea_index_set_graph_hgra(artea_index* idx, const char* path) {
     clear_error();
     ARTEA_CHECK_ARG(idx && path, "artea_index_set_graph_hgra: null argument");
-    HostCsr c;
-    int rc = hgra_read(path, &c);
-    if (rc) return rc;
-    if (c.n != idx->n) {
-        set_error("artea_index_set_graph_hgra: graph has " + std::to_string(c.n) + " vertices, index has " +
-                  std::to_string(idx->n) + " vectors");
-        return ARTEA_EINVAL;
-    }
-    rc = upload_graph(idx, c);
-    if (rc) return rc;
-    idx->csr = std::move(c);
-    return ARTEA_OK;
+    return hgra_load_device(idx, path, 0, 0);
+}
+
+int artea_index_set_graph_hgra_segmented(artea_index* idx, const char* path, uint32_t segment_words,
+                                         uint32_t trial_records) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx && path, "artea_index_set_graph_hgra_segmented: null argument");
+    return hgra_load_device(idx, path, segment_words, trial_records);
 }
 
 int artea_index_set_graph(artea_index* idx, const artea_graph* g) {
@@ -313,6 +319,7 @@ int artea_index_set_graph(artea_index* idx, const artea_graph* g) {
     int rc = upload_graph(idx, c);
     if (rc) return rc;
     idx->csr = std::move(c);
+    idx->csr_current = true;
     return ARTEA_OK;
 }
 
@@ -320,7 +327,14 @@ int artea_index_get_graph(const artea_index* idx, artea_graph** out) {
     clear_error();
     ARTEA_CHECK_ARG(idx && out, "artea_index_get_graph: null argument");
     ARTEA_CHECK_ARG(idx->dg.attached, "artea_index_get_graph: no graph attached");
-    *out = new artea_graph{idx->csr};
+    if (idx->csr_current) {
+        *out = new artea_graph{idx->csr};
+        return ARTEA_OK;
+    }
+    HostCsr c;
+    const int rc = graph_from_device(idx, &c);
+    if (rc) return rc;
+    *out = new artea_graph{std::move(c)};
     return ARTEA_OK;
 }
 
@@ -328,7 +342,27 @@ int artea_index_snapshot_hgra(const artea_index* idx, const char* path) {
     clear_error();
     ARTEA_CHECK_ARG(idx && path, "artea_index_snapshot_hgra: null argument");
     ARTEA_CHECK_ARG(idx->dg.attached, "artea_index_snapshot_hgra: no graph attached");
-    return hgra_write(idx->csr, path);
+    return hgra_store_device(const_cast<artea_index*>(idx), path);
+}
+
+int artea_index_layout(const artea_index* idx, uint32_t* stride0, uint32_t* stride_upper, uint64_t* rows_upper) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx, "artea_index_layout: null index");
+    ARTEA_CHECK_ARG(idx->dg.attached, "artea_index_layout: no graph attached");
+    if (stride0) *stride0 = idx->dg.stride0;
+    if (stride_upper) *stride_upper = idx->dg.strideU;
+    if (rows_upper) *rows_upper = idx->dg.rowsU;
+    return ARTEA_OK;
+}
+
+int artea_index_hgra_seconds(const artea_index* idx, double* load3, double* store3) {
+    clear_error();
+    ARTEA_CHECK_ARG(idx, "artea_index_hgra_seconds: null index");
+    for (int i = 0; i < 3; ++i) {
+        if (load3) load3[i] = idx->hgra_load_s[i];
+        if (store3) store3[i] = idx->hgra_store_s[i];
+    }
+    return ARTEA_OK;
 }
 
 int artea_index_info(const artea_index* idx, uint32_t* n, uint32_t* dim_pad, uint32_t* top, uint32_t* entry,

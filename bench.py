@@ -442,6 +442,130 @@ def workload_rnet(args):
     print(json.dumps(out))
 
 
+def workload_hgra(args):
+    """HGRA v1 snapshot and restore of the search workload's graph (1M x 128
+    L2, built on the GPU with the C0 parameters) through the device path
+    (record layout / segmented parse, checks and ELL fill in HBM), against the
+    host path (hgra.cpp: parse into the grouped CSR, ELL conversion, copy) and
+    the oracle's reader.  One step = one restore of the file into the device
+    layout.  `value` is the device-resident part (the parse, its walk, the id
+    checks and the ELL fill, file bytes over their host-clock span); the file
+    read and the copy to the device are reported beside it."""
+    import ctypes as C
+    import tempfile
+
+    import artea_amd as aa
+    from artea_amd._lib import lib
+
+    n, seed = args.n, 20261203 + 1
+    base = aa.generate(0, seed, 0, n, 128)
+    idx = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN)
+    t0 = time.time()
+    gpu_build(idx, args, seed)
+    log("graph built in %.1fs" % (time.time() - t0))
+    tmp = tempfile.mkdtemp(prefix="hgra_", dir=os.environ.get("TMPDIR", "/tmp"))
+    path = os.path.join(tmp, "c1.graph")
+    host_path = os.path.join(tmp, "c1_host.graph")
+    # snapshot: device writer vs the host writer over the same graph
+    dev_w = []
+    for _ in range(max(1, args.warmup) + args.steps):
+        t0 = time.time()
+        idx.snapshot_hgra(path)
+        dev_w.append((time.time() - t0, idx.hgra_seconds()[1]))
+    dev_w = dev_w[max(1, args.warmup):]
+    h = C.c_void_p()
+    check_rc(lib().artea_index_get_graph(idx.h, C.byref(h)))
+    t0 = time.time()
+    check_rc(lib().artea_graph_snapshot(h, host_path.encode()))
+    host_w = time.time() - t0
+    lib().artea_graph_destroy(h)
+    data = open(path, "rb").read()
+    size = len(data)
+    same_w = data == open(host_path, "rb").read()
+    # restore: device loader (timed steps) vs host reader + attach
+    idx2 = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN)
+    for _ in range(args.warmup):
+        idx2.load_hgra(path)
+    loads = []
+    for _ in range(args.steps):
+        t0 = time.time()
+        idx2.load_hgra(path)
+        loads.append((time.time() - t0, idx2.hgra_seconds()[0]))
+    idx2.snapshot_hgra(host_path)
+    same_r = open(host_path, "rb").read() == data
+    t0 = time.time()
+    h = C.c_void_p()
+    check_rc(lib().artea_graph_restore(path.encode(), C.byref(h)))
+    host_parse = time.time() - t0
+    idx3 = aa.DeviceIndex(base, aa.DistanceMetrics.EUCLIDEAN)
+    t0 = time.time()
+    check_rc(lib().artea_index_set_graph(idx3.h, h))
+    host_attach = time.time() - t0
+    lib().artea_graph_destroy(h)
+    med = lambda xs: float(np.median(xs))
+    total = med([x[0] for x in loads])
+    read_copy = med([x[1][0] for x in loads])
+    parse = med([x[1][1] for x in loads])
+    fill = med([x[1][2] for x in loads])
+    dev_part = parse + fill
+    w_total = med([x[0] for x in dev_w])
+    w_layout = med([x[1][0] for x in dev_w])
+    # algorithmic bytes of the device part: the body read by the checks and by
+    # the fill, and the ELL arrays written
+    lay = idx2.layout()
+    ell_bytes = (n * lay["stride0"] + lay["rows_upper"] * lay["stride_upper"]) * 4
+    alg = 2 * (size - 28) + ell_bytes
+    achieved = alg / dev_part / 1e9
+    out = {"metric": "HGRA restore into the device layout, GB/s of file (1M x 128 L2 graph)",
+           "value": round(size / dev_part / 1e9, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(dev_part * 1e3, 2), "higher_is_better": True,
+           "scaling": "strong", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (the search workload's graph, built on the GPU)",
+           "config": {"workload": "HGRA v1 restore / snapshot, %d-vertex graph, %.1f MB file" % (n, size / 1e6),
+                      "n": n, "file_bytes": size},
+           "restore_seconds": {"device_total": round(total, 4), "file_read_and_copy": round(read_copy, 4),
+                               "parse_and_walk": round(parse, 4), "checks_and_fill": round(fill, 4),
+                               "host_parse": round(host_parse, 4), "host_ell_and_copy": round(host_attach, 4)},
+           "snapshot_seconds": {"device_total": round(w_total, 4), "device_layout": round(w_layout, 4),
+                                "host_writer": round(host_w, 4)},
+           "device_file_equals_host_file": same_w, "restored_snapshot_equals_file": same_r,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "span": "host clock over the parse, its walk, the checks and the fill (with their syncs)"}}
+    log("HGRA %.1f MB: device restore %.3fs (read+copy %.3f, parse %.4f, checks+fill %.4f) vs host %.3f + %.3f; "
+        "device snapshot %.3fs (layout %.4f) vs host writer %.3f; files equal %s/%s" % (
+            size / 1e6, total, read_copy, parse, fill, host_parse, host_attach, w_total, w_layout, host_w, same_w,
+            same_r))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        orc.build()
+        t0 = time.time()
+        g = orc.Graph.restore(path)
+        cpu_s = time.time() - t0
+        q = os.path.join(tmp, "c1_oracle.graph")
+        g.snapshot(q)
+        same_o = open(q, "rb").read() == data
+        del g
+        out["cpu_baseline"] = {"value": round(size / cpu_s / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                               "cpu_model": host_cpu_model(), "numa": host_numa_policy(),
+                               "sample": "the whole file restored by the oracle's reader (one thread)",
+                               "oracle_snapshot_equals_file": same_o}
+        log("oracle reader: %.3fs (%.3f GB/s), its snapshot equals the file %s" % (cpu_s, size / cpu_s / 1e9, same_o))
+    for f in (path, host_path, os.path.join(tmp, "c1_oracle.graph")):
+        if os.path.exists(f):
+            os.remove(f)
+    os.rmdir(tmp)
+    print(json.dumps(out))
+
+
+def check_rc(rc):
+    from artea_amd._lib import check
+
+    check(rc)
+
+
 def workload_routing(args):
     """One routing loop of refine_layer (RoutingUpdater topk 64 / queue 96,
     merge, PruningUpdater scale 1.10) on the L0 layer of the 1,000,000 x 128
@@ -701,7 +825,7 @@ def main():
                     help="a value, or 'probe' for the reference's auto-probe (quantile 0.9, 500 samples)")
     ap.add_argument("--level-fraction", type=float, default=None,
                     help="hierarchy level fraction (default: 1 for search, 1/32 for cosine and build)")
-    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing", "rnet"),
+    ap.add_argument("--workload", choices=("search", "cosine", "build", "c0", "probe", "routing", "rnet", "hgra"),
                     default="search",
                     help="search: 1M x 128 L2 batched vs plain (the headline); cosine: 1M x 100 cosine sweep; "
                          "build: 10M x 96 hierarchy build (every level) FP32 vs BF16 join, then serving on that index; "
@@ -747,6 +871,8 @@ def main():
         return workload_routing(args)
     if args.workload == "rnet":
         return workload_rnet(args)
+    if args.workload == "hgra":
+        return workload_hgra(args)
     if args.workload == "probe":
         if args.n == 1_000_000:
             args.n = 10_000_000

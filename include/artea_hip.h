@@ -100,12 +100,33 @@ int artea_index_set_graph_csr(artea_index* idx, uint32_t top, uint32_t ul_max, u
 int artea_index_set_graph(artea_index* idx, const artea_graph* g);
 /* HierarchicalGraphFileManager::restore (persistence/
  * hierarchical_graph_file_manager.hpp:152-324) straight into the device
- * layout, with the same validation and error texts. */
+ * layout, with the same validation and error texts: the file's body is
+ * copied to the device and parsed there (records found by a segmented parse
+ * that the host verifies segment by segment, ids checked and rows written
+ * into the ELL arrays by one wavefront per vertex). */
 int artea_index_set_graph_hgra(artea_index* idx, const char* path);
+/* The same with the segmented parse's settings (no counterpart in the
+ * reference): segment_words = words per segment, trial_records = records a
+ * guessed segment start must parse before it is taken (0 and 0 = the
+ * defaults, 4096 and 4).  Every guess is verified against the previous
+ * segment's end, so the result does not depend on them. */
+int artea_index_set_graph_hgra_segmented(artea_index* idx, const char* path, uint32_t segment_words,
+                                         uint32_t trial_records);
 /* Copy of the attached graph as a host graph (snapshot, export). */
 int artea_index_get_graph(const artea_index* idx, artea_graph** out);
-/* HierarchicalGraphFileManager::snapshot (:88-145) of the attached graph. */
+/* HierarchicalGraphFileManager::snapshot (:88-145) of the attached graph,
+ * laid out on the device (record lengths, their prefix sum, one wavefront
+ * per record) and streamed to the file. */
 int artea_index_snapshot_hgra(const artea_index* idx, const char* path);
+/* The attached graph's device layout: ELL row strides (words) of level 0
+ * and of the upper levels, and the number of upper-level rows.  No
+ * counterpart in the reference (its layout is the grouped CSR). */
+int artea_index_layout(const artea_index* idx, uint32_t* stride0, uint32_t* stride_upper, uint64_t* rows_upper);
+/* Host-clock phases of the index's last device HGRA restore (load3: file
+ * read and copy to the device, segmented parse and its walk, id checks and
+ * ELL fill) and snapshot (store3: record layout on the device, copy back
+ * and file write, 0), in seconds.  No counterpart in the reference. */
+int artea_index_hgra_seconds(const artea_index* idx, double* load3, double* store3);
 /* top_occupied_level_id / entry_point_vid / get_num_vertices (:83-132). */
 int artea_index_info(const artea_index* idx, uint32_t* n, uint32_t* dim_pad, uint32_t* top,
                      uint32_t* entry_vid, uint32_t* ul_max, uint32_t* bl_max);
