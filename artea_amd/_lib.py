@@ -62,6 +62,8 @@ SIGNATURES = {
     "artea_launch_graph_launch": (i32, [P, P]),
     "artea_launch_graph_destroy": (i32, [P]),
     "artea_server_create": (i32, [P, u32, u32, u32, u32, PP]),
+    "artea_server_create_budget": (i32, [P, u32, u32, u32, u64, u32, PP]),
+    "artea_server_admission": (i32, [P, P, P, C.POINTER(u32), C.POINTER(u64)]),
     "artea_server_destroy": (i32, [P]),
     "artea_server_submit": (i32, [P, P, u32, P, P, C.POINTER(u64)]),
     "artea_server_poll": (i32, [P, u32, P, P, P, C.POINTER(u32)]),

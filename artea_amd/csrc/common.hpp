@@ -58,5 +58,8 @@ int graph_from_device(const artea_index* idx, HostCsr* out);
 // list, and an asynchronous copy of that list (count, then batch indices)
 // into host memory, ordered after the router's latest launch on `stream`.
 uint32_t rk_overflow_list();
+// Wavefronts the batched router's persistent kernel keeps resident on the
+// current GPU (its LDS per wave and registers decide it).
+int batched_router_resident_waves(artea_batched_router* r, uint32_t* waves);
 int router_overflow_list_async(artea_batched_router* r, uint32_t* host_dst, hipStream_t stream);
 }  // namespace artea

@@ -147,6 +147,25 @@ int launch_body(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, u
                 const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds,
                 uint32_t* d_ov, float* d_od, hipStream_t s, bool persistent, int thr_mode, float* d_thr);
 
+// The kernel a launch of queue size `queue` runs (the pinned or default slot
+// of the batched router, or the plain kernel) and its LDS bytes.
+int select_kernel(const artea_router* r, uint32_t queue, bool persistent, KernelFn* fn, size_t* lds) {
+    const artea_index* idx = r->idx;
+    KernelSet ks;
+    uint32_t K;
+    int rc = pick_kernels(queue, idx->dim_pad / 16, idx->metric, &ks, &K);
+    if (rc) return rc;
+    const bool has_u8 = idx->d_u8 != nullptr;
+    int slot = r->variant >= 0 ? r->variant : default_variant(K, has_u8);
+    if (!has_u8 && slot >= kNumVariants) slot -= kNumVariants;  // the one-byte rows were dropped
+    const bool u8 = slot >= kNumVariants;
+    const int variant = slot % kNumVariants;
+    *fn = persistent ? (u8 ? ks.persistent_u8[variant] : ks.persistent[variant]) : (has_u8 ? ks.plain_u8 : ks.plain);
+    const bool fn_u8 = (persistent ? u8 : has_u8) && ks.u8_kernels;  // the kernel's LDS layout
+    *lds = lds_bytes(idx->dim_pad, K, fn_u8);
+    return ARTEA_OK;
+}
+
 int launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue, const uint32_t* d_topk,
            const uint32_t* d_queue, uint32_t out_stride, uint32_t flags, const float* thresholds, uint32_t* d_ov,
            float* d_od, hipStream_t s, bool persistent, int thr_mode = 0, float* d_thr = nullptr) {
@@ -230,19 +249,10 @@ int launch_body(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, u
         a.hmask = uint32_t((uint64_t(1) << B) - 1);
         a.vis_mode = B <= 20 ? 0u : (B <= 27 ? 1u : 2u);
     }
-    KernelSet ks;
-    uint32_t K;
-    int rc = pick_kernels(queue, a.C, idx->metric, &ks, &K);
+    KernelFn fn = nullptr;
+    size_t lds = 0;
+    int rc = select_kernel(r, queue, persistent, &fn, &lds);
     if (rc) return rc;
-    const bool has_u8 = idx->d_u8 != nullptr;
-    int slot = r->variant >= 0 ? r->variant : default_variant(K, has_u8);
-    if (!has_u8 && slot >= kNumVariants) slot -= kNumVariants;  // the one-byte rows were dropped
-    const bool u8 = slot >= kNumVariants;
-    const int variant = slot % kNumVariants;
-    const KernelFn fn = persistent ? (u8 ? ks.persistent_u8[variant] : ks.persistent[variant])
-                                   : (has_u8 ? ks.plain_u8 : ks.plain);
-    const bool fn_u8 = (persistent ? u8 : has_u8) && ks.u8_kernels;  // the kernel's LDS layout
-    const size_t lds = lds_bytes(a.D, K, fn_u8);
     if (lds > 65536) {
         // above the default dynamic-LDS limit (large queues, wide rows)
         ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -295,6 +305,22 @@ int read_overflows(artea_router* r, uint32_t max, uint32_t* qidx, uint32_t* coun
 }  // namespace
 
 namespace artea {
+
+int batched_router_resident_waves(artea_batched_router* br, uint32_t* waves) {
+    KernelFn fn = nullptr;
+    size_t lds = 0;
+    int rc = select_kernel(&br->base, br->base.queue, true, &fn, &lds);
+    if (rc) return rc;
+    if (lds > 65536)
+        ARTEA_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          int(lds)));
+    int dev = 0, cus = 0, per_cu = 0;
+    ARTEA_HIP_TRY(hipGetDevice(&dev));
+    ARTEA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    ARTEA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWavesPerBlock * 64, lds));
+    *waves = uint32_t(std::max(1, cus * std::max(per_cu, 1)) * kWavesPerBlock);
+    return ARTEA_OK;
+}
 
 int router_launch(artea_router* r, const float* d_q, uint32_t nq, uint32_t topk, uint32_t queue,
                   const uint32_t* d_topk, const uint32_t* d_queue, uint32_t out_stride, uint32_t flags,

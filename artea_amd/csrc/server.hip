@@ -3,12 +3,24 @@
 // their results are collected by polling; no call blocks on the GPU.
 //
 // Requests wait in one FIFO per queue-size class (the router's register
-// layouts).  `lanes` streams each hold at most `budget` requests in flight.
-// A pump step, run by every submit and poll call, (1) harvests the launches
-// whose completion event has fired, and (2) gives every lane with nothing in
-// flight the class of the oldest queued request: up to `budget` of that
-// class's requests, oldest first, staged in pinned memory and run by that
-// class's batched router, followed by the result copies and an event.  Every
+// layouts).  `lanes` streams each hold a launch's requests in flight.  A pump
+// step, run by every submit and poll call, (1) harvests the launches whose
+// completion event has fired, and (2) gives every lane with nothing in
+// flight the class of the oldest queued request: up to that class's
+// admission limit of its requests, oldest first, staged in pinned memory and
+// run by that class's batched router, followed by the result copies and an
+// event.
+//
+// Admission limits.  artea_server_create gives every class the same request
+// count.  artea_server_create_budget derives them from memory: the HBM
+// budget is shared by the lanes' device buffers, so a lane holds
+// hbm_bytes / (lanes * per-request bytes) requests (query row, result row of
+// max_topk ids and distances, topk and queue words); and a class admits at
+// most lds_rounds times the wavefronts its kernel keeps resident, which its
+// LDS per wavefront (candidate-list mirror, visited cache, query row) and
+// registers decide.  A large-queue class, with more LDS per wavefront and
+// fewer resident wavefronts, therefore launches fewer requests at a time,
+// and a launch never holds more requests than lds_rounds passes of the GPU.  Every
 // request is routed by its own wavefront with no state shared with the rest
 // of its launch, so its result is what batch_query returns for it alone.
 //
@@ -92,7 +104,10 @@ struct Lane {
 
 struct artea_server {
     artea_index* idx = nullptr;
-    uint32_t dim = 0, max_topk = 0, max_queue = 0, budget = 0;
+    uint32_t dim = 0, max_topk = 0, max_queue = 0, budget = 0;  // budget: requests a lane's buffers hold
+    uint32_t admit[kNumClasses] = {};                             // requests per launch, per class
+    uint32_t resident[kNumClasses] = {};                          // resident wavefronts per class (budget form)
+    uint64_t hbm_bytes = 0;                                       // device bytes of the lanes' buffers
     int lo_class = 0, hi_class = 0;
     std::vector<Lane> lanes;
     std::vector<Block> blocks;
@@ -261,7 +276,7 @@ int launch(artea_server* s, Lane& ln) {
             c = cc;
         }
     }
-    const uint32_t m = uint32_t(std::min<uint64_t>(s->queued_class[c], s->budget));
+    const uint32_t m = uint32_t(std::min<uint64_t>(s->queued_class[c], s->admit[c]));
     const size_t D = s->dim, K = s->max_topk;
     Block* b = s->free_blocks.back();
     s->free_blocks.pop_back();
@@ -314,24 +329,27 @@ int pump(artea_server* s) {
     return ARTEA_OK;
 }
 
-}  // namespace
+// Device bytes one request holds in a lane: its query row, its result row
+// (max_topk ids and distances) and its topk and queue words.
+uint64_t request_bytes(uint32_t dim_pad, uint32_t max_topk) {
+    return uint64_t(dim_pad) * 4 + uint64_t(max_topk) * 8 + 8;
+}
 
-extern "C" {
-
-int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
-                        artea_server** out) {
-    artea::clear_error();
+// lds_rounds == 0: every class admits `budget` requests per launch; else a
+// class admits min(budget, lds_rounds * its resident wavefronts).
+int server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
+                  uint32_t lds_rounds, artea_server** out) {
     ARTEA_CHECK_ARG(idx && out, "artea_server_create: null argument");
     ARTEA_CHECK_ARG(max_topk >= 1 && max_topk <= max_queue, "artea_server_create: need 1 <= max_topk <= max_queue");
     ARTEA_CHECK_ARG(class_of(max_queue) >= 0, "artea_server_create: max_queue exceeds the router's largest queue");
     ARTEA_CHECK_ARG(lanes >= 1 && lanes <= 16, "artea_server_create: lanes must be in [1, 16]");
-    ARTEA_CHECK_ARG(budget >= 1 && budget <= (1u << 20), "artea_server_create: budget must be in [1, 2^20]");
     artea_server* s = new artea_server();
     s->idx = idx;
     s->dim = idx->dim_pad;
     s->max_topk = max_topk;
     s->max_queue = max_queue;
     s->budget = budget;
+    s->hbm_bytes = uint64_t(lanes) * budget * request_bytes(idx->dim_pad, max_topk);
     s->lo_class = class_of(max_topk);
     s->hi_class = class_of(max_queue);
     s->lanes.resize(lanes);
@@ -347,11 +365,61 @@ int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue,
         b.k.assign(budget, 0);
         s->free_blocks.push_back(&b);
     }
+    for (int c = 0; c < kNumClasses; ++c) s->admit[c] = budget;
+    if (!rc && lds_rounds) {
+        for (int c = s->lo_class; c <= s->hi_class && !rc; ++c) {
+            rc = artea::batched_router_resident_waves(s->lanes[0].routers[c], &s->resident[c]);
+            if (!rc) s->admit[c] = uint32_t(std::min<uint64_t>(budget, uint64_t(lds_rounds) * s->resident[c]));
+        }
+    }
     if (rc) {
         server_free(s);
         return rc;
     }
     *out = s;
+    return ARTEA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
+                        artea_server** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(budget >= 1 && budget <= (1u << 20), "artea_server_create: budget must be in [1, 2^20]");
+    return server_create(idx, max_topk, max_queue, lanes, budget, 0, out);
+}
+
+int artea_server_create_budget(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes,
+                               uint64_t hbm_bytes, uint32_t lds_rounds, artea_server** out) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(idx && out, "artea_server_create_budget: null argument");
+    ARTEA_CHECK_ARG(lanes >= 1 && lanes <= 16, "artea_server_create_budget: lanes must be in [1, 16]");
+    ARTEA_CHECK_ARG(lds_rounds >= 1 && lds_rounds <= 1024, "artea_server_create_budget: lds_rounds must be in [1, 1024]");
+    ARTEA_CHECK_ARG(max_topk >= 1 && max_topk <= max_queue,
+                    "artea_server_create_budget: need 1 <= max_topk <= max_queue");
+    const uint64_t per_request = request_bytes(idx->dim_pad, max_topk);
+    const uint64_t cap = hbm_bytes / (uint64_t(lanes) * per_request);
+    if (cap < 1) {
+        set_error("artea_server_create_budget: " + std::to_string(hbm_bytes) + " bytes hold less than one request per lane (" +
+                  std::to_string(per_request) + " bytes each, " + std::to_string(lanes) + " lanes)");
+        return ARTEA_EINVAL;
+    }
+    return server_create(idx, max_topk, max_queue, lanes, uint32_t(std::min<uint64_t>(cap, 1u << 20)), lds_rounds,
+                         out);
+}
+
+int artea_server_admission(artea_server* s, uint32_t* admit, uint32_t* resident, uint32_t* lane_requests,
+                           uint64_t* hbm_bytes) {
+    artea::clear_error();
+    ARTEA_CHECK_ARG(s, "artea_server_admission: null server");
+    for (int c = 0; c < kNumClasses; ++c) {
+        if (admit) admit[c] = s->admit[c];
+        if (resident) resident[c] = s->resident[c];
+    }
+    if (lane_requests) *lane_requests = s->budget;
+    if (hbm_bytes) *hbm_bytes = s->hbm_bytes;
     return ARTEA_OK;
 }
 

@@ -48,16 +48,27 @@ def queue_classes(q: np.ndarray) -> np.ndarray:
 
 class Server:
     """Open-loop request server.  ``budget`` bounds the requests one lane
-    holds in flight; ``lanes`` bounds how many launches overlap."""
+    holds in flight; ``lanes`` bounds how many launches overlap.  With
+    ``hbm_bytes`` the admission comes from memory instead
+    (artea_server_create_budget): the lanes' device buffers share
+    ``hbm_bytes``, and a launch of a queue-size class takes at most
+    ``lds_rounds`` times the wavefronts that class's kernel keeps resident."""
 
-    def __init__(self, index, max_topk: int = 100, max_queue: int = 800, lanes: int = 4, budget: int = 2048):
+    def __init__(self, index, max_topk: int = 100, max_queue: int = 800, lanes: int = 4, budget: int = 2048,
+                 hbm_bytes: int = 0, lds_rounds: int = 4):
         h = C.c_void_p()
-        check(lib().artea_server_create(index.h, max_topk, max_queue, lanes, budget, C.byref(h)))
+        if hbm_bytes:
+            check(lib().artea_server_create_budget(index.h, max_topk, max_queue, lanes, hbm_bytes, lds_rounds,
+                                                   C.byref(h)))
+        else:
+            check(lib().artea_server_create(index.h, max_topk, max_queue, lanes, budget, C.byref(h)))
         self.h = h
         self.index = index  # the server reads the index's device rows and graph
         self.dim_pad = index.dim_pad
         self.max_topk = max_topk
         self.lanes = lanes
+        if hbm_bytes:
+            budget = self.admission()["lane_requests"]
         self.budget = budget
         self._cap = max(lanes * budget, 1)
         self._t = np.empty(self._cap, np.uint64)
@@ -101,6 +112,17 @@ class Server:
                                            C.c_void_p(table_d.ctypes.data), self._cap,
                                            C.c_void_p(self._t.ctypes.data), C.byref(got)))
         return got.value
+
+    def admission(self):
+        """Requests per launch and resident wavefronts of each queue-size
+        class, the requests one lane holds, and the lanes' device bytes."""
+        admit = np.zeros(6, np.uint32)
+        resident = np.zeros(6, np.uint32)
+        lane, hbm = C.c_uint32(), C.c_uint64()
+        check(lib().artea_server_admission(self.h, C.c_void_p(admit.ctypes.data), C.c_void_p(resident.ctypes.data),
+                                           C.byref(lane), C.byref(hbm)))
+        return {"classes": _CLASSES.tolist(), "admit": admit.tolist(), "resident_waves": resident.tolist(),
+                "lane_requests": lane.value, "hbm_bytes": hbm.value}
 
     def status(self):
         """(queued, in flight, completed but not yet polled) request counts."""

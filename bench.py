@@ -560,6 +560,15 @@ def workload_hgra(args):
     print(json.dumps(out))
 
 
+def make_server(Server, idx, max_queue, args):
+    """The serving leg's server: admission by request count (--serve-budget),
+    or with --serve-hbm-mb by device memory and resident wavefronts."""
+    if args.serve_hbm_mb > 0:
+        return Server(idx, max_topk=100, max_queue=max_queue, lanes=args.serve_lanes,
+                      hbm_bytes=int(args.serve_hbm_mb * 2**20), lds_rounds=args.serve_lds_rounds)
+    return Server(idx, max_topk=100, max_queue=max_queue, lanes=args.serve_lanes, budget=args.serve_budget)
+
+
 def check_rc(rc):
     from artea_amd._lib import check
 
@@ -723,10 +732,10 @@ def workload_build(args):
         stopk = rng.choice(np.array([10, 50, 100], np.uint32), ns)
         squeue = rng.choice(np.array([64, 128, 256, 512, 800], np.uint32), ns)
         squeue = np.maximum(squeue, stopk)
-        srv = Server(idx, max_topk=100, max_queue=800, lanes=args.serve_lanes, budget=args.serve_budget)
+        srv = make_server(Server, idx, 800, args)
         # untimed warm-up (first launches of every class router), then the
         # saturation: one closed batch of the same mix
-        nw = min(ns, args.serve_lanes * args.serve_budget)
+        nw = min(ns, args.serve_lanes * srv.budget)
         srv.run(queries[pick[:nw]], stopk[:nw], squeue[:nw], np.zeros(nw))
         v, d, lat, wall = srv.run(queries[pick], stopk, squeue, np.zeros(ns))
         sat = ns / wall
@@ -744,7 +753,8 @@ def workload_build(args):
             log("serving at %.0f q/s offered: %.0f achieved, p50 %.2f ms p99 %.2f ms (queue 64: %.2f, queue 800: "
                 "%.2f)" % (rate, ns / wall, loads[-1]["p50_ms"], loads[-1]["p99_ms"], loads[-1]["p99_ms_queue64"],
                            loads[-1]["p99_ms_queue800"]))
-        out["serving"] = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": args.serve_budget,
+        out["serving"] = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": srv.budget,
+                          "admission": srv.admission(),
                           "saturation_qps": round(sat), "recall_at_topk": round(rec, 4), "loads": loads}
         if not args.no_cpu:
             # CPU closed loop on a sample of the same mix: the oracle router on
@@ -848,6 +858,11 @@ def main():
                     help="serving: overlapping launch lanes (streams); default 3 for search, 4 for build")
     ap.add_argument("--serve-budget", type=int, default=None,
                     help="serving: most requests one lane launches at once; default 8192 for search, 2048 for build")
+    ap.add_argument("--serve-hbm-mb", type=float, default=0.0,
+                    help="serving: admit by device memory instead of --serve-budget: the lanes' buffers share this "
+                         "many MiB, and a launch takes at most --serve-lds-rounds times its kernel's resident waves")
+    ap.add_argument("--serve-lds-rounds", type=int, default=2,
+                    help="serving with --serve-hbm-mb: resident-wavefront passes one launch may take")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after timing, run one plain copy kernel of the base vectors (counter calibration)")
     ap.add_argument("--graph-cache", type=str, default="",
@@ -1205,13 +1220,13 @@ def main():
         # batched peak; continuous batching over overlapping lanes.
         from artea_amd.serving import Server, percentile_nearest_rank, poisson_arrivals
 
-        srv = Server(idx, max_topk=100, max_queue=chosen, lanes=args.serve_lanes, budget=args.serve_budget)
+        srv = make_server(Server, idx, chosen, args)
         ns = args.serve_requests
         pick = np.arange(ns) % nq
         sq, stopk, squeue = queries[pick], topk_q[pick], np.full(ns, chosen, np.uint32)
         # Untimed warm-up: every lane's routers launch once (first launches
         # load kernels and size the persistent grids).
-        nw = min(ns, args.serve_lanes * args.serve_budget)
+        nw = min(ns, args.serve_lanes * srv.budget)
         srv.run(sq[:nw], stopk[:nw], squeue[:nw], np.zeros(nw))
         # Closed-loop saturation: every request present at the start.
         _, _, _, wall = srv.run(sq, stopk, squeue, np.zeros(ns))
@@ -1228,7 +1243,8 @@ def main():
                           "p99_ms": round(percentile_nearest_rank(lat, 99) * 1e3, 3)})
             log("serving at %.0f q/s offered: %.0f q/s achieved, p50 %.2f ms, p99 %.2f ms" % (
                 rate, ns / wall, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
-        serving = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": args.serve_budget,
+        serving = {"requests_per_load": ns, "lanes": args.serve_lanes, "lane_budget": srv.budget,
+                   "admission": srv.admission(),
                    "queue": chosen, "saturation_qps": round(sat), "results_equal_batch": same, "loads": loads}
 
     out = {

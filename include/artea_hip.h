@@ -551,6 +551,23 @@ int artea_distance_rows_device(int metric, const float* d_a, uint32_t na, const 
 typedef struct artea_server artea_server;
 int artea_server_create(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes, uint32_t budget,
                         artea_server** out);
+/* The server with admission by device memory: the lanes' device buffers
+ * take at most hbm_bytes, so a lane holds hbm_bytes / (lanes * (4 dim_pad +
+ * 8 max_topk + 8)) requests (query row, result ids and distances, topk and
+ * queue words); and a launch of one queue-size class takes at most
+ * lds_rounds times the wavefronts that class's kernel keeps resident on the
+ * GPU, which its LDS per wavefront (candidate-list mirror, visited cache,
+ * query row) and registers decide.  Larger queues, more LDS per wavefront,
+ * fewer requests per launch.  EINVAL when hbm_bytes hold less than one
+ * request per lane. */
+int artea_server_create_budget(artea_index* idx, uint32_t max_topk, uint32_t max_queue, uint32_t lanes,
+                               uint64_t hbm_bytes, uint32_t lds_rounds, artea_server** out);
+/* Admission limits: admit[c] requests per launch and resident[c] resident
+ * wavefronts (0 for the count form) of each of the 6 queue-size classes
+ * (candidate lists up to 48, 112, 240, 496, 816, 1008 entries), the requests
+ * one lane holds, and the lanes' device bytes. */
+int artea_server_admission(artea_server* s, uint32_t* admit, uint32_t* resident, uint32_t* lane_requests,
+                           uint64_t* hbm_bytes);
 int artea_server_destroy(artea_server* s);
 int artea_server_submit(artea_server* s, const float* queries, uint32_t n, const uint32_t* topk,
                         const uint32_t* queue, uint64_t* first_ticket);

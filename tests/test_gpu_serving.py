@@ -140,3 +140,46 @@ def test_native_server_submit_poll_through_the_c_abi():
         Server(idx, max_topk=64, max_queue=2000)
     with pytest.raises(aa.ArteaError):
         Server(idx, max_topk=64, max_queue=500, lanes=0)
+
+
+def test_admission_by_device_memory_budget():
+    """artea_server_create_budget: a lane holds hbm_bytes / (lanes x request
+    bytes) requests, a launch of a queue-size class at most lds_rounds times
+    the wavefronts its kernel keeps resident; results equal the plain
+    router's and are the same alone and in the mix; a budget below one
+    request per lane is an argument error."""
+    from artea_amd._lib import ArteaError
+    from artea_amd.serving import QUEUE_CLASSES, Server, poisson_arrivals, queue_class
+
+    vecs = aa.generate(0, 53, 0, 4000, 128)
+    idx = aa.DeviceIndex(vecs, 0)
+    aa.build_full(idx, max_level=2, build_loops=4, triu_iters=2, seed=8)
+    per_request = 128 * 4 + 100 * 8 + 8
+    lanes, rounds = 3, 1
+    hbm = lanes * per_request * 700 + 17
+    srv = Server(idx, max_topk=100, max_queue=800, lanes=lanes, hbm_bytes=hbm, lds_rounds=rounds)
+    a = srv.admission()
+    assert a["lane_requests"] == 700 and a["hbm_bytes"] == lanes * 700 * per_request <= hbm
+    lo, hi = QUEUE_CLASSES.index(queue_class(100)), QUEUE_CLASSES.index(queue_class(800))
+    for c in range(lo, hi + 1):
+        assert a["resident_waves"][c] > 0
+        assert a["admit"][c] == min(700, rounds * a["resident_waves"][c]), a
+    n = 900
+    qs = aa.generate(0, 53, 1, n, 128)
+    rng = np.random.default_rng(11)
+    topk = rng.choice([10, 50, 100], n).astype(np.uint32)
+    queue = np.maximum(topk, rng.choice([64, 128, 256, 512, 800], n)).astype(np.uint32)
+    v, d, lat, _ = srv.run(qs, topk, queue, poisson_arrivals(n, 40000.0, seed=3))
+    assert np.all(lat > 0)
+    for k in (10, 50, 100):
+        for L in np.unique(queue):
+            sel = np.where((topk == k) & (queue == L))[0]
+            if len(sel) == 0:
+                continue
+            pv, pd = aa.HierarchicalGraphRouter(idx, None, k, int(L)).batch_query(qs[sel])
+            assert np.array_equal(pv, v[sel, :k]) and np.array_equal(pd, d[sel, :k]), (k, L)
+    for i in range(0, n, 97):
+        va, da, _, _ = srv.run(qs[i:i + 1], topk[i:i + 1], queue[i:i + 1], np.zeros(1))
+        assert np.array_equal(va[0], v[i]) and np.array_equal(da[0], d[i]), i
+    with pytest.raises(ArteaError, match="less than one request per lane"):
+        Server(idx, max_topk=100, max_queue=800, lanes=2, hbm_bytes=2 * per_request - 1)
