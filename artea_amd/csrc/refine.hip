@@ -212,6 +212,23 @@ __host__ __device__ inline TriLayout tri_layout(uint32_t cap, uint32_t D, bool s
 
 __device__ __forceinline__ uint32_t tri_index(uint32_t i, uint32_t j) { return i * (i - 1) / 2 + j; }  // i > j
 
+// Work item t of the pair triangle's words: entry i (1 <= i < n) owns
+// ceil(i / 32) words of 32 columns j < i; entries 1..32 have one word, 33..64
+// two, 65..96 three, 97..127 four, listed entry by entry.  False once i >= n.
+__device__ __forceinline__ bool pair_word(uint32_t t, uint32_t n, uint32_t& i, uint32_t& w) {
+    uint32_t first = 0;
+#pragma unroll
+    for (uint32_t words = 1; words <= 4; ++words) {
+        if (t < first + 32u * words) {
+            i = 32u * (words - 1) + 1 + (t - first) / words;
+            w = (t - first) % words;
+            return i < n;
+        }
+        first += 32u * words;
+    }
+    return false;
+}
+
 __device__ __forceinline__ uint32_t word_below(uint32_t i, uint32_t w) {  // bits [0, i) of word w
     const uint32_t b = 32u * w;
     return i >= b + 32u ? 0xFFFFFFFFu : (i > b ? (1u << (i - b)) - 1u : 0u);
@@ -601,53 +618,26 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
                     acc_hl = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc_hl, 0, 0, 0);
                     acc_lh = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc_lh, 0, 0, 0);
                 }
+                // The tile only stores its keys; the decision pass after the
+                // last tile takes the decisions.
                 const float nj = j < n ? s_norm[j] : 0.f, rnj = j < n ? s_rn[j] : 0.f;
-                // Per lane, bit q of each word: the decision for its pair q.
-                uint32_t hitb = 0, recb = 0, ambb = 0, nvalid = 0;
+                uint32_t nvalid = 0;
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
                     const uint32_t i = 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
                     const bool valid = (i < n) & (j < i) & (jnew | ((newI >> (i & 31)) & 1u));
                     const float g = acc[q] + (acc_hl[q] + acc_lh[q]);
-                    const float ni = s_norm[i], rni = s_rn[i];
-                    const float4 th = s_thr[i];
                     float key;
                     if (metric == kL2Sqr || metric == kL2)
-                        key = (ni + nj) - 2.0f * g;
+                        key = (s_norm[i] + nj) - 2.0f * g;
                     else if (metric == kDot)
                         key = -g;
                     else
-                        key = 1.0f - g * (rni * rnj);
-                    const float e = key_radius(metric, kb, ni, nj, rni, rnj);
-                    const float lo = key - e, hi = key + e;
-                    const bool dp = hi < th.x, np_ = lo >= th.y, dd = hi < th.z, nd = lo >= th.w;
-                    const bool sure = (dp | np_) & (dd | nd);
+                        key = 1.0f - g * (s_rn[i] * rnj);
                     if (valid) s_P[tri_index(i, j)] = key;
                     nvalid += valid ? 1u : 0u;
-                    hitb |= uint32_t(valid & sure & dp) << q;
-                    recb |= uint32_t(valid & sure & dd) << q;
-                    ambb |= uint32_t(valid & !sure) << q;
                 }
                 approx_pairs += nvalid;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) write_masks(q, (hitb >> q) & 1u, (recb >> q) & 1u);
-                // Ambiguous pairs of the tile: one queue reservation.
-                uint32_t namb = 0;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) namb += uint32_t(__popcll(ballot((ambb >> q) & 1u)));
-                if (namb) {
-                    uint32_t off = 0;
-                    if (lane == 0) off = atomicAdd(&s_nl, namb);
-                    off = readlane_u(off, 0);
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const bool mine = (ambb >> q) & 1u;
-                        const uint64_t m = ballot(mine);
-                        if (mine)
-                            put_recheck(off + lanes_below(m), 32 * I + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3), j);
-                        off += uint32_t(__popcll(m));
-                    }
-                }
             } else {
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
@@ -661,6 +651,40 @@ __global__ __launch_bounds__(256, 2) void triangle_kernel(Ctx c, float inv_scale
         }
         __syncthreads();
         if (MFMA) {
+            // Decisions from the tiles' keys, one thread per (entry i, word w)
+            // of the pair triangle, so every thread of the block takes part
+            // whatever the tile split, each word of H and C has one writer,
+            // and only the pairs that need a decision are visited.  A pair
+            // whose key interval straddles a threshold goes to the FP32
+            // recheck queue; the drain below sets its bits.
+            for (uint32_t t = tid;; t += 256) {
+                uint32_t i, w;
+                if (!pair_word(t, n, i, w)) break;
+                const float ni = s_norm[i], rni = s_rn[i];
+                const float4 th = s_thr[i];
+                const uint32_t j0 = 32 * w, span = min(i - j0, 32u), b0 = tri_index(i, 0);
+                uint32_t todo = span == 32 ? 0xFFFFFFFFu : (1u << span) - 1u;
+                if (!is_new(i)) todo &= w == 0 ? newm[0] : (w == 1 ? newm[1] : (w == 2 ? newm[2] : newm[3]));
+                uint32_t hw = 0, cw = 0, aw = 0;
+                for (uint32_t m = todo; m; m &= m - 1) {
+                    const uint32_t b = uint32_t(__ffs(m) - 1), j = j0 + b;
+                    const float key = s_P[b0 + j];
+                    const float e = key_radius(metric, kb, ni, s_norm[j], rni, s_rn[j]);
+                    const float lo = key - e, hi = key + e;
+                    const bool dp = hi < th.x, np_ = lo >= th.y, dd = hi < th.z, nd = lo >= th.w;
+                    const bool sure = (dp | np_) & (dd | nd);
+                    hw |= uint32_t(sure & dp) << b;
+                    cw |= uint32_t(sure & dd) << b;
+                    aw |= uint32_t(!sure) << b;
+                }
+                s_H[i * 4 + w] = hw;
+                s_C[i * 4 + w] = cw;
+                if (aw) {
+                    uint32_t off = atomicAdd(&s_nl, uint32_t(__popc(aw)));
+                    for (uint32_t m = aw; m; m &= m - 1) put_recheck(off++, i, j0 + uint32_t(__ffs(m) - 1));
+                }
+            }
+            __syncthreads();
             // Ambiguous predicates: FP32 chain (a quad of lanes per pair), bits set here.
             auto settle = [&](uint32_t i, uint32_t j, float te) {
                 const uint32_t p = tri_index(i, j);
