@@ -570,6 +570,33 @@ struct CandList {
         return true;
     }
 
+    // peek() that also returns the entry's position in W.
+    __device__ __forceinline__ uint32_t peek_pos(float* dist, uint32_t* pos) const {
+        uint32_t found = 0xFFFFFFFFu, at = 0;
+        float fd = __int_as_float(0x7F800000);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t b = ballot(v[k] != 0xFFFFFFFFu && (v[k] & kExplored) == 0);
+            if (found == 0xFFFFFFFFu && b) {
+                const int l = __ffsll((unsigned long long)b) - 1;
+                found = readlane_u(v[k], l) & kVidMask;
+                fd = readlane_f(d[k], l);
+                at = uint32_t(k) * 64 + uint32_t(l);
+            }
+        }
+        *dist = fd;
+        *pos = at;
+        return found;
+    }
+
+    // Marks the entry at position pos explored (the pop of a peeked entry).
+    __device__ __forceinline__ void mark_explored(uint32_t pos) {
+        const uint32_t lane = lane_id();
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (pos / 64 == uint32_t(k) && lane == pos % 64) v[k] |= kExplored;
+    }
+
     // The entry the next pop would return if no insertion precedes it
     // (first unexplored entry), or 0xFFFFFFFF.
     __device__ __forceinline__ uint32_t peek(float* dist) const {
@@ -695,7 +722,106 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // The read is issued after this row's ids are consumed and before its
     // distance gathers, whose in-order waits then cover it; issued earlier,
     // the wait on this row's ids would also wait for it.
-    while (W.pop(&pv)) {
+    auto on_first = [&](float d0, uint32_t v0) {
+        if (pre_v == 0xFFFFFFFFu || key_less(d0, v0, pre_d, pre_v)) {
+            pre_v = v0;
+            pre_d = d0;
+            pre_id = lane < a.stride0 ? a.ell0[size_t(v0) * a.stride0 + lane] : 0xFFFFFFFFu;
+        }
+    };
+    auto predict = [&](uint32_t* pos) {
+        pre_v = W.peek_pos(&pre_d, pos);
+        pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
+    };
+    // Rows of at most 64 ids (one chunk): expansions in pairs where the
+    // order allows.  After an expansion's distances, the predicted next pop
+    // e2 (the first unexplored entry) is certainly the next pop when it lies
+    // in T and no candidate of this expansion that passed the prefilter
+    // precedes it: an insertion lands after e2, and the tail trim only
+    // touches positions at or after L.  e2 is then expanded at once and both
+    // expansions' candidates are merged together, this expansion's first,
+    // which is the same sequence of try_push calls (merge_chunk's count test
+    // holds for any run of candidates in order; e2's prefilter uses the
+    // bound before the merge, which only lets more candidates reach the
+    // exact test).  One merge instead of two.  Lists of more than 8 blocks
+    // keep one expansion per merge (the second candidate run would not fit
+    // their register budget).
+    const bool paired = K <= 8 && a.stride0 <= 64;
+    if (paired) {
+        while (W.pop(&pv)) {
+            expansions++;
+            const uint32_t first = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
+                                                                           : 0xFFFFFFFFu);
+            if (ballot(first != 0xFFFFFFFFu) == 0) {
+                pre_v = 0xFFFFFFFFu;  // empty row: no read in flight
+                continue;
+            }
+            const uint32_t cnt = collect<K, (U8R > 0)>(w, first, a.n, a.hmask, a.vis_mode);
+            uint32_t e2pos;
+            predict(&e2pos);
+            if (cnt == 0) continue;
+            const uint32_t e2v = pre_v, e2id = pre_id;
+            const float e2d = pre_d;
+            any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
+            ndc += cnt;
+            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
+            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
+            wave_sync();
+            const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
+            const bool pair = e2v != 0xFFFFFFFFu && e2pos < W.L &&
+                              ballot(((P >> lane) & 1ull) && key_less(myd, myv, e2d, e2v)) == 0;
+            if (!pair) {
+                if (P) W.merge_chunk(myd, myv, P, w, on_first);
+                continue;
+            }
+            // Expand e2 now (its row read was issued by the prediction).
+            W.mark_explored(e2pos);
+            expansions++;
+            uint32_t cnt2 = 0;
+            const bool rows2 = ballot(e2id != 0xFFFFFFFFu) != 0;
+            if (rows2) cnt2 = collect<K, (U8R > 0)>(w, e2id, a.n, a.hmask, a.vis_mode);
+            uint32_t p3;
+            if (rows2) {
+                predict(&p3);
+            } else {
+                pre_v = 0xFFFFFFFFu;
+            }
+            uint64_t P2 = 0;
+            float myd2 = 0.f;
+            uint32_t myv2 = 0u;
+            if (cnt2) {
+                any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt2, u, metric, qn2);
+                ndc += cnt2;
+                myd2 = lane < cnt2 ? w.cdist[lane] : 0.f;
+                myv2 = lane < cnt2 ? w.cand[lane] : 0u;
+                wave_sync();
+                P2 = (cnt2 == 64 ? ~0ull : ((1ull << cnt2) - 1)) & (W.full() ? ballot(myd2 < W.lb) : ~0ull);
+            }
+            const uint32_t m1 = uint32_t(__popcll(P)), m2 = uint32_t(__popcll(P2));
+            if (m1 && m2 && m1 + m2 <= 64) {
+                // Both runs in one chunk, in order: the staging slots of the
+                // candidate rows, read back into lanes 0 .. m1 + m2 - 1.
+                if ((P >> lane) & 1ull) {
+                    w.cdist[lanes_below(P)] = myd;
+                    w.cand[lanes_below(P)] = myv;
+                }
+                if ((P2 >> lane) & 1ull) {
+                    w.cdist[m1 + lanes_below(P2)] = myd2;
+                    w.cand[m1 + lanes_below(P2)] = myv2;
+                }
+                wave_sync();
+                const uint32_t m = m1 + m2;
+                const float cd2 = lane < m ? w.cdist[lane] : 0.f;
+                const uint32_t cv2 = lane < m ? w.cand[lane] : 0u;
+                wave_sync();
+                W.merge_chunk(cd2, cv2, m == 64 ? ~0ull : ((1ull << m) - 1), w, on_first);
+            } else {
+                if (P) W.merge_chunk(myd, myv, P, w, on_first);
+                if (P2) W.merge_chunk(myd2, myv2, P2, w, on_first);
+            }
+        }
+    }
+    while (!paired && W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
         const bool hit = pv == pre_v;
