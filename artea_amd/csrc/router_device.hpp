@@ -743,11 +743,10 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
     // which is the same sequence of try_push calls (merge_chunk's count test
     // holds for any run of candidates in order; e2's prefilter uses the
     // bound before the merge, which only lets more candidates reach the
-    // exact test).  One merge instead of two.  Lists of more than 8 blocks
-    // keep one expansion per merge (the second candidate run would not fit
-    // their register budget).
-    const bool paired = K <= 8 && a.stride0 <= 64;
-    if (paired) {
+    // exact test).  One merge instead of two.  Lists of up to 4 blocks; at 8
+    // blocks the chains below (longer runs measured slower at up to 4 blocks,
+    // faster at 8).
+    if (K <= 4 && a.stride0 <= 64) {
         while (W.pop(&pv)) {
             expansions++;
             const uint32_t first = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
@@ -821,7 +820,89 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
             }
         }
     }
-    while (!paired && W.pop(&pv)) {
+    // Lists of 8 blocks: the same argument for chains.  The next pop e is
+    // certain when it lies in T and none of the prefiltered candidates held
+    // since the last merge precedes it; e is then expanded at once, and the
+    // chain's candidates are merged together in expansion order.  A chain
+    // ends when its next pop is not certain or its candidates would not fit
+    // one merge (64 lanes).  Lists of more than 8 blocks keep one expansion
+    // per merge (the held candidates would not fit their register budget).
+    if (K > 4 && K <= 8 && a.stride0 <= 64) {
+        // Candidates held for the chain's merge, in order: the lanes of
+        // `held` (one expansion's prefiltered lanes as they came, or, once a
+        // second run joins, lanes 0 .. m - 1).
+        float accd = 0.f;
+        uint32_t accv = 0u;
+        uint64_t held = 0;
+        // Adds the lanes of PP (values dd, vv) after the held candidates
+        // through the staging slots; false when they would not fit.
+        auto hold = [&](float dd, uint32_t vv, uint64_t PP) -> bool {
+            if (!held) {
+                accd = dd;
+                accv = vv;
+                held = PP;
+                return true;
+            }
+            const uint32_t m0 = uint32_t(__popcll(held)), m = m0 + uint32_t(__popcll(PP));
+            if (m > 64) return false;
+            if ((held >> lane) & 1ull) {
+                w.cdist[lanes_below(held)] = accd;
+                w.cand[lanes_below(held)] = accv;
+            }
+            if ((PP >> lane) & 1ull) {
+                w.cdist[m0 + lanes_below(PP)] = dd;
+                w.cand[m0 + lanes_below(PP)] = vv;
+            }
+            wave_sync();
+            accd = lane < m ? w.cdist[lane] : 0.f;
+            accv = lane < m ? w.cand[lane] : 0u;
+            held = m == 64 ? ~0ull : ((1ull << m) - 1);
+            wave_sync();
+            return true;
+        };
+        auto merge_held = [&]() {
+            if (held) W.merge_chunk(accd, accv, held, w, on_first);
+            held = 0;
+        };
+        // One expansion of the vertex whose row ids are `ids`: visited
+        // collect, next-pop prediction, distances, prefilter; its candidates
+        // join the held ones (or, when they would not fit, the held ones are
+        // merged first and these after, and the chain ends).
+        auto expand = [&](uint32_t ids, uint32_t* epos) -> bool {
+            expansions++;
+            const bool rows = ballot(ids != 0xFFFFFFFFu) != 0;
+            const uint32_t cnt = rows ? collect<K, (U8R > 0)>(w, ids, a.n, a.hmask, a.vis_mode) : 0u;
+            predict(epos);
+            if (cnt == 0) return true;
+            any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
+            ndc += cnt;
+            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
+            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
+            wave_sync();
+            const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
+            if (!P || hold(myd, myv, P)) return true;
+            merge_held();
+            W.merge_chunk(myd, myv, P, w, on_first);
+            return false;
+        };
+        while (W.pop(&pv)) {
+            const uint32_t first = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
+                                                                           : 0xFFFFFFFFu);
+            uint32_t epos;
+            bool more = expand(first, &epos);
+            while (more) {
+                const uint32_t ev = pre_v, eid = pre_id;
+                const float ed = pre_d;
+                if (ev == 0xFFFFFFFFu || epos >= W.L || ballot(((held >> lane) & 1ull) && key_less(accd, accv, ed, ev)) != 0)
+                    break;
+                W.mark_explored(epos);
+                more = expand(eid, &epos);
+            }
+            merge_held();
+        }
+    }
+    const bool chained = K <= 8 && a.stride0 <= 64;
+    while (!chained && W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
         const bool hit = pv == pre_v;
