@@ -733,175 +733,96 @@ __device__ __forceinline__ QueryStats route_one(const RouterArgs& a, const WaveL
         pre_v = W.peek_pos(&pre_d, pos);
         pre_id = (pre_v != 0xFFFFFFFFu && lane < a.stride0) ? a.ell0[size_t(pre_v) * a.stride0 + lane] : 0xFFFFFFFFu;
     };
-    // Rows of at most 64 ids (one chunk): expansions in pairs where the
-    // order allows.  After an expansion's distances, the predicted next pop
-    // e2 (the first unexplored entry) is certainly the next pop when it lies
-    // in T and no candidate of this expansion that passed the prefilter
-    // precedes it: an insertion lands after e2, and the tail trim only
-    // touches positions at or after L.  e2 is then expanded at once and both
-    // expansions' candidates are merged together, this expansion's first,
-    // which is the same sequence of try_push calls (merge_chunk's count test
-    // holds for any run of candidates in order; e2's prefilter uses the
-    // bound before the merge, which only lets more candidates reach the
-    // exact test).  One merge instead of two.  Lists of up to 4 blocks; at 8
-    // blocks the chains below (longer runs measured slower at up to 4 blocks,
-    // faster at 8).
-    if (K <= 4 && a.stride0 <= 64) {
-        while (W.pop(&pv)) {
-            expansions++;
-            const uint32_t first = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
-                                                                           : 0xFFFFFFFFu);
-            if (ballot(first != 0xFFFFFFFFu) == 0) {
-                pre_v = 0xFFFFFFFFu;  // empty row: no read in flight
-                continue;
-            }
-            const uint32_t cnt = collect<K, (U8R > 0)>(w, first, a.n, a.hmask, a.vis_mode);
-            uint32_t e2pos;
-            predict(&e2pos);
-            if (cnt == 0) continue;
-            const uint32_t e2v = pre_v, e2id = pre_id;
-            const float e2d = pre_d;
-            any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
-            ndc += cnt;
-            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
-            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
-            wave_sync();
-            const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
-            const bool pair = e2v != 0xFFFFFFFFu && e2pos < W.L &&
-                              ballot(((P >> lane) & 1ull) && key_less(myd, myv, e2d, e2v)) == 0;
-            if (!pair) {
-                if (P) W.merge_chunk(myd, myv, P, w, on_first);
-                continue;
-            }
-            // Expand e2 now (its row read was issued by the prediction).
-            W.mark_explored(e2pos);
-            expansions++;
-            uint32_t cnt2 = 0;
-            const bool rows2 = ballot(e2id != 0xFFFFFFFFu) != 0;
-            if (rows2) cnt2 = collect<K, (U8R > 0)>(w, e2id, a.n, a.hmask, a.vis_mode);
-            uint32_t p3;
-            if (rows2) {
-                predict(&p3);
-            } else {
-                pre_v = 0xFFFFFFFFu;
-            }
-            uint64_t P2 = 0;
-            float myd2 = 0.f;
-            uint32_t myv2 = 0u;
-            if (cnt2) {
-                any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt2, u, metric, qn2);
-                ndc += cnt2;
-                myd2 = lane < cnt2 ? w.cdist[lane] : 0.f;
-                myv2 = lane < cnt2 ? w.cand[lane] : 0u;
-                wave_sync();
-                P2 = (cnt2 == 64 ? ~0ull : ((1ull << cnt2) - 1)) & (W.full() ? ballot(myd2 < W.lb) : ~0ull);
-            }
-            const uint32_t m1 = uint32_t(__popcll(P)), m2 = uint32_t(__popcll(P2));
-            if (m1 && m2 && m1 + m2 <= 64) {
-                // Both runs in one chunk, in order: the staging slots of the
-                // candidate rows, read back into lanes 0 .. m1 + m2 - 1.
-                if ((P >> lane) & 1ull) {
-                    w.cdist[lanes_below(P)] = myd;
-                    w.cand[lanes_below(P)] = myv;
-                }
-                if ((P2 >> lane) & 1ull) {
-                    w.cdist[m1 + lanes_below(P2)] = myd2;
-                    w.cand[m1 + lanes_below(P2)] = myv2;
-                }
-                wave_sync();
-                const uint32_t m = m1 + m2;
-                const float cd2 = lane < m ? w.cdist[lane] : 0.f;
-                const uint32_t cv2 = lane < m ? w.cand[lane] : 0u;
-                wave_sync();
-                W.merge_chunk(cd2, cv2, m == 64 ? ~0ull : ((1ull << m) - 1), w, on_first);
-            } else {
-                if (P) W.merge_chunk(myd, myv, P, w, on_first);
-                if (P2) W.merge_chunk(myd2, myv2, P2, w, on_first);
-            }
-        }
-    }
-    // Lists of 8 blocks: the same argument for chains.  The next pop e is
-    // certain when it lies in T and none of the prefiltered candidates held
-    // since the last merge precedes it; e is then expanded at once, and the
-    // chain's candidates are merged together in expansion order.  A chain
-    // ends when its next pop is not certain or its candidates would not fit
-    // one merge (64 lanes).  Lists of more than 8 blocks keep one expansion
-    // per merge (the held candidates would not fit their register budget).
-    if (K > 4 && K <= 8 && a.stride0 <= 64) {
+    // Rows of at most 64 ids (one chunk): chains of expansions where the
+    // order allows.  After an expansion's distances, the predicted next pop e
+    // (the first unexplored entry) is certainly the next pop when it lies in
+    // T and no candidate held since the last merge (prefiltered) precedes it:
+    // an insertion lands after e, and the tail trim only touches positions at
+    // or after L.  e is then expanded at once, and the chain's candidates are
+    // merged together in expansion order, which is the same sequence of
+    // try_push calls (merge_chunk's count test holds for any run of
+    // candidates in order; a later expansion's prefilter uses the bound
+    // before the merge, which only lets more candidates reach the exact
+    // test).  A chain ends when its next pop is not certain or when its
+    // candidates would not fit one merge (64 lanes; the run that did not fit
+    // is merged after the held ones); every link marks an unexplored entry of
+    // T explored and none is added before the merge, so a chain has at most L
+    // links.  The loop has one merge site: each inlined copy of merge_chunk
+    // costs code size and registers (two and three sites measured slower).
+    if (a.stride0 <= 64) {
         // Candidates held for the chain's merge, in order: the lanes of
         // `held` (one expansion's prefiltered lanes as they came, or, once a
-        // second run joins, lanes 0 .. m - 1).
-        float accd = 0.f;
-        uint32_t accv = 0u;
-        uint64_t held = 0;
-        // Adds the lanes of PP (values dd, vv) after the held candidates
-        // through the staging slots; false when they would not fit.
-        auto hold = [&](float dd, uint32_t vv, uint64_t PP) -> bool {
-            if (!held) {
-                accd = dd;
-                accv = vv;
-                held = PP;
-                return true;
+        // second run joins, lanes 0 .. m - 1); `carry` holds a run that did
+        // not fit, merged next.
+        float accd = 0.f, crd = 0.f;
+        uint32_t accv = 0u, crv = 0u;
+        uint64_t held = 0, carry = 0;
+        while (true) {
+            if (carry) {
+                accd = crd;
+                accv = crv;
+                held = carry;
+                carry = 0;
+            } else {
+                if (!W.pop(&pv)) break;
+                uint32_t ids = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
+                                                                        : 0xFFFFFFFFu);
+                held = 0;
+                while (true) {
+                    expansions++;
+                    const bool rows = ballot(ids != 0xFFFFFFFFu) != 0;
+                    const uint32_t cnt = rows ? collect<K, (U8R > 0)>(w, ids, a.n, a.hmask, a.vis_mode) : 0u;
+                    uint32_t epos;
+                    predict(&epos);
+                    if (cnt) {
+                        any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
+                        ndc += cnt;
+                        const float myd = lane < cnt ? w.cdist[lane] : 0.f;
+                        const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
+                        wave_sync();
+                        const uint64_t P =
+                            (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
+                        if (P && !held) {
+                            accd = myd;
+                            accv = myv;
+                            held = P;
+                        } else if (P) {
+                            const uint32_t m0 = uint32_t(__popcll(held)), m = m0 + uint32_t(__popcll(P));
+                            if (m > 64) {
+                                crd = myd;
+                                crv = myv;
+                                carry = P;
+                                break;
+                            }
+                            // The held run, then this one, through the
+                            // staging slots into lanes 0 .. m - 1.
+                            if ((held >> lane) & 1ull) {
+                                w.cdist[lanes_below(held)] = accd;
+                                w.cand[lanes_below(held)] = accv;
+                            }
+                            if ((P >> lane) & 1ull) {
+                                w.cdist[m0 + lanes_below(P)] = myd;
+                                w.cand[m0 + lanes_below(P)] = myv;
+                            }
+                            wave_sync();
+                            accd = lane < m ? w.cdist[lane] : 0.f;
+                            accv = lane < m ? w.cand[lane] : 0u;
+                            held = m == 64 ? ~0ull : ((1ull << m) - 1);
+                            wave_sync();
+                        }
+                    }
+                    if (pre_v == 0xFFFFFFFFu || epos >= W.L ||
+                        ballot(((held >> lane) & 1ull) && key_less(accd, accv, pre_d, pre_v)) != 0)
+                        break;
+                    W.mark_explored(epos);
+                    ids = pre_id;
+                }
+                if (!held) continue;
             }
-            const uint32_t m0 = uint32_t(__popcll(held)), m = m0 + uint32_t(__popcll(PP));
-            if (m > 64) return false;
-            if ((held >> lane) & 1ull) {
-                w.cdist[lanes_below(held)] = accd;
-                w.cand[lanes_below(held)] = accv;
-            }
-            if ((PP >> lane) & 1ull) {
-                w.cdist[m0 + lanes_below(PP)] = dd;
-                w.cand[m0 + lanes_below(PP)] = vv;
-            }
-            wave_sync();
-            accd = lane < m ? w.cdist[lane] : 0.f;
-            accv = lane < m ? w.cand[lane] : 0u;
-            held = m == 64 ? ~0ull : ((1ull << m) - 1);
-            wave_sync();
-            return true;
-        };
-        auto merge_held = [&]() {
-            if (held) W.merge_chunk(accd, accv, held, w, on_first);
-            held = 0;
-        };
-        // One expansion of the vertex whose row ids are `ids`: visited
-        // collect, next-pop prediction, distances, prefilter; its candidates
-        // join the held ones (or, when they would not fit, the held ones are
-        // merged first and these after, and the chain ends).
-        auto expand = [&](uint32_t ids, uint32_t* epos) -> bool {
-            expansions++;
-            const bool rows = ballot(ids != 0xFFFFFFFFu) != 0;
-            const uint32_t cnt = rows ? collect<K, (U8R > 0)>(w, ids, a.n, a.hmask, a.vis_mode) : 0u;
-            predict(epos);
-            if (cnt == 0) return true;
-            any_distances<CT, WIDE, MF, (U8R > 0 ? U8R : 1)>(a, w, cnt, u, metric, qn2);
-            ndc += cnt;
-            const float myd = lane < cnt ? w.cdist[lane] : 0.f;
-            const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
-            wave_sync();
-            const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
-            if (!P || hold(myd, myv, P)) return true;
-            merge_held();
-            W.merge_chunk(myd, myv, P, w, on_first);
-            return false;
-        };
-        while (W.pop(&pv)) {
-            const uint32_t first = pv == pre_v ? pre_id : (lane < a.stride0 ? a.ell0[size_t(pv) * a.stride0 + lane]
-                                                                           : 0xFFFFFFFFu);
-            uint32_t epos;
-            bool more = expand(first, &epos);
-            while (more) {
-                const uint32_t ev = pre_v, eid = pre_id;
-                const float ed = pre_d;
-                if (ev == 0xFFFFFFFFu || epos >= W.L || ballot(((held >> lane) & 1ull) && key_less(accd, accv, ed, ev)) != 0)
-                    break;
-                W.mark_explored(epos);
-                more = expand(eid, &epos);
-            }
-            merge_held();
+            W.merge_chunk(accd, accv, held, w, on_first);
         }
     }
-    const bool chained = K <= 8 && a.stride0 <= 64;
+    const bool chained = a.stride0 <= 64;
     while (!chained && W.pop(&pv)) {
         expansions++;
         const uint32_t* row = a.ell0 + size_t(pv) * a.stride0;
