@@ -35,7 +35,7 @@ using namespace artea::rf;
 
 namespace {
 
-constexpr uint32_t kSortN = 512;       // merge sort width
+constexpr uint32_t kMergeN = 256;      // merge: entries sorted per round by one wavefront
 
 // The same counter-based draw as the oracle: SplitMix64 finaliser chained
 // over (seed, stream, row, j), mapped to [lo, hi) by multiply-high.
@@ -916,98 +916,130 @@ __device__ __forceinline__ unsigned long long sort_key(float d, uint32_t vid, ui
     return (static_cast<unsigned long long>(u) << 32) | (static_cast<unsigned long long>(vid & kMask) << 1) | origin;
 }
 
-// apply_logs merge for rows that received logs; one 256-thread block per row.
+// Bitonic network over 64 * R (key, payload) pairs held four to a lane at
+// most: element r * 64 + lane in k[r], p[r].  Stages with j >= 64 pair
+// registers of one lane, the others pair lanes (lane ^ j).  Equal keys carry
+// equal payloads here (a key holds the vid and its origin), so the network's
+// instability does not show in the result.
+template <int R>
+__device__ __forceinline__ void wave_bitonic(unsigned long long (&k)[4], uint32_t (&p)[4]) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (uint32_t k2 = 2; k2 <= 64u * R; k2 <<= 1) {
+#pragma unroll
+        for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const uint32_t rj = j / 64;
+#pragma unroll
+                for (uint32_t r = 0; r < uint32_t(R); ++r) {
+                    if (r & rj) continue;
+                    const uint32_t r2 = r | rj;
+                    const bool asc = ((r * 64 + lane) & k2) == 0;
+                    if (asc ? (k[r] > k[r2]) : (k[r] < k[r2])) {
+                        const unsigned long long tk = k[r];
+                        k[r] = k[r2];
+                        k[r2] = tk;
+                        const uint32_t tp = p[r];
+                        p[r] = p[r2];
+                        p[r2] = tp;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (uint32_t r = 0; r < uint32_t(R); ++r) {
+                    const uint32_t lo = __shfl_xor(uint32_t(k[r]), int(j)), hi = __shfl_xor(uint32_t(k[r] >> 32), int(j));
+                    const uint32_t op = __shfl_xor(p[r], int(j));
+                    const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
+                    // The lower element of the pair keeps the minimum when the
+                    // k2-block sorts ascending, the upper one otherwise.
+                    const bool take_min = ((lane & j) == 0) == (((r * 64 + lane) & k2) == 0);
+                    if (take_min ? (o < k[r]) : (o > k[r])) {
+                        k[r] = o;
+                        p[r] = op;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// apply_logs merge for rows that received logs (refiner/nbr_log_table.hpp:
+// 127-182): one wavefront per row.  The row's entries and a run of its logs
+// are sorted together by (distance, vid, origin) in registers, over the next
+// power of two >= their count (64, 128 or 256); the first element of each vid
+// is kept (origin 0 = the row's entry sorts first: the row wins ties) and the
+// first cap survivors stay.  Logs beyond kMergeN - (survivors) take further
+// rounds against the survivors, which selects the same entries as one sort of
+// everything: an entry cut in a round has cap smaller unique entries that stay.
 __global__ __launch_bounds__(256) void merge_kernel(Ctx c) {
-    __shared__ unsigned long long key[kSortN];
-    __shared__ uint32_t pay[kSortN];
-    __shared__ uint32_t s_n;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid / 64;
-    for (uint32_t row = blockIdx.x; row < c.n; row += gridDim.x) {
+    __shared__ unsigned long long s_key[4][kMergeN];
+    __shared__ uint32_t s_pay[4][kMergeN];
+    const uint32_t lane = lane_id(), wid = threadIdx.x / 64;
+    unsigned long long* key = s_key[wid];
+    uint32_t* pay = s_pay[wid];
+    for (uint32_t row = blockIdx.x * 4 + wid; row < c.n; row += gridDim.x * 4) {
         const uint32_t m = c.t_cnt[row];
-        if (m == 0) continue;  // uniform across the block
+        if (m == 0) continue;  // uniform across the wavefront
         const uint32_t base = c.t_off[row];
-        const uint32_t nrow = c.cnt[row];
-        for (uint32_t k = tid; k < nrow; k += 256) {
+        uint32_t cur = c.cnt[row];
+        for (uint32_t k = lane; k < cur; k += 64) {
             const uint2 e = c.rows[size_t(row) * c.cap + k];
             key[k] = sort_key(__uint_as_float(e.y), e.x, 0);
             pay[k] = e.x;
         }
-        if (tid == 0) s_n = nrow;
-        __syncthreads();
-        const uint32_t chunk = kSortN - c.cap;
-        for (uint32_t off = 0; off < m; off += chunk) {
-            const uint32_t cur = s_n;
-            const uint32_t nc = min(chunk, m - off);
-            for (uint32_t k = tid; k < kSortN - cur; k += 256) {
-                if (k < nc) {
-                    const uint2 lg = c.seg[base + off + k];
-                    key[cur + k] = sort_key(__uint_as_float(lg.y), lg.x, 1);
-                    pay[cur + k] = (lg.x & kMask) | kNew;
-                } else {
-                    key[cur + k] = ~0ull;
-                    pay[cur + k] = 0xFFFFFFFFu;
-                }
+        for (uint32_t off = 0; off < m;) {
+            const uint32_t nc = min(m - off, kMergeN - cur), total = cur + nc;
+            for (uint32_t k = lane; k < nc; k += 64) {
+                const uint2 lg = c.seg[base + off + k];
+                key[cur + k] = sort_key(__uint_as_float(lg.y), lg.x, 1);
+                pay[cur + k] = (lg.x & kMask) | kNew;
             }
-            __syncthreads();
-            const uint32_t total = cur + nc;
-            for (uint32_t k2 = 2; k2 <= kSortN; k2 <<= 1) {
-                for (uint32_t j = k2 >> 1; j > 0; j >>= 1) {
-                    const uint32_t t = tid;  // 256 threads = kSortN / 2 pairs
-                    const uint32_t i = 2 * j * (t / j) + (t % j);
-                    const uint32_t l = i + j;
-                    const bool up = (i & k2) == 0;
-                    const unsigned long long a = key[i], b = key[l];
-                    if ((a > b) == up) {
-                        key[i] = b;
-                        key[l] = a;
-                        const uint32_t pa = pay[i];
-                        pay[i] = pay[l];
-                        pay[l] = pa;
-                    }
-                    __syncthreads();
-                }
-            }
-            // Adjacent unique by vid, first `cap` survivors, compacted by wave 0.
-            if (wid == 0) {
-                uint32_t outn = 0;
-                uint32_t keep_v[kSortN / 64];
-                unsigned long long keep_k[kSortN / 64];
-                uint32_t dst[kSortN / 64];
+            wave_sync();
+            unsigned long long kr[4];
+            uint32_t pr[4];
 #pragma unroll
-                for (uint32_t b = 0; b < kSortN / 64; ++b) {
-                    const uint32_t i = b * 64 + lane;
-                    bool keep = false;
-                    if (i < total) {
-                        const uint32_t v = pay[i] & kMask;
-                        keep = (i == 0) || ((pay[i - 1] & kMask) != v);
-                    }
-                    const uint64_t km = ballot(keep);
-                    keep_v[b] = keep ? pay[i] : 0xFFFFFFFFu;
-                    keep_k[b] = keep ? key[i] : ~0ull;
-                    dst[b] = keep ? outn + lanes_below(km) : 0xFFFFFFFFu;
-                    outn += uint32_t(__popcll(km));
-                }
-                wave_sync();
-#pragma unroll
-                for (uint32_t b = 0; b < kSortN / 64; ++b) {
-                    if (dst[b] < c.cap) {
-                        key[dst[b]] = keep_k[b];
-                        pay[dst[b]] = keep_v[b];
-                    }
-                }
-                if (lane == 0) s_n = min(outn, c.cap);
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t q = r * 64 + lane;
+                kr[r] = q < total ? key[q] : ~0ull;
+                pr[r] = q < total ? pay[q] : 0xFFFFFFFFu;
             }
-            __syncthreads();
+            wave_sync();
+            if (total <= 64)
+                wave_bitonic<1>(kr, pr);
+            else if (total <= 128)
+                wave_bitonic<2>(kr, pr);
+            else
+                wave_bitonic<4>(kr, pr);
+            // Adjacent unique by vid, the first cap survivors compacted to the
+            // front of the wavefront's arrays.
+            uint32_t outn = 0, last = 0xFFFFFFFFu;  // last: payload of element r * 64 - 1
+#pragma unroll
+            for (uint32_t r = 0; r < 4; ++r) {
+                const uint32_t q = r * 64 + lane;
+                uint32_t prev = __shfl_up(pr[r], 1u);
+                if (lane == 0) prev = last;
+                last = __shfl(pr[r], 63);
+                const bool keep = q < total && (q == 0 || (prev & kMask) != (pr[r] & kMask));
+                const uint64_t km = ballot(keep);
+                const uint32_t dst = outn + lanes_below(km);
+                if (keep && dst < c.cap) {
+                    key[dst] = kr[r];
+                    pay[dst] = pr[r];
+                }
+                outn += uint32_t(__popcll(km));
+            }
+            wave_sync();
+            cur = min(outn, c.cap);
+            off += nc;
         }
-        const uint32_t fin = s_n;
-        for (uint32_t k = tid; k < fin; k += 256) {
+        for (uint32_t k = lane; k < cur; k += 64) {
             const unsigned long long kk = key[k];
             uint32_t u = uint32_t(kk >> 32);
             u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
             c.rows[size_t(row) * c.cap + k] = make_uint2(pay[k], u);
         }
-        if (tid == 0) c.cnt[row] = fin;
-        __syncthreads();
+        if (lane == 0) c.cnt[row] = cur;
+        wave_sync();
     }
 }
 
@@ -1107,7 +1139,7 @@ int merge_logs(artea_refine* r) {
     const size_t slots = size_t(c.n) * c.cap;
     const uint32_t blocks = uint32_t((slots + 255) / 256);
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 1, 8)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 4, 8)), dim3(256), 0, s, c);
     hipLaunchKernelGGL(count_logs_kernel, dim3(1), dim3(64), 0, s, c.t_off, c.n, c.stats);
     ARTEA_HIP_TRY(hipGetLastError());
     ARTEA_HIP_TRY(hipMemsetAsync(c.t_cnt, 0, sizeof(uint32_t) * (c.n + 1), s));
@@ -1222,7 +1254,7 @@ int merge_routing_reverse(artea_refine* r, const RoutingPass& p, uint2* seg) {
     hipStream_t s = r->s;
     ARTEA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(r->scan_tmp, r->scan_bytes, c.t_cnt, c.t_off, c.n + 1, s));
     ARTEA_HIP_TRY(routing_scatter(c, p, seg, s));
-    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 1, 8)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(merge_kernel, dim3(grid_for(r, 4, 8)), dim3(256), 0, s, c);
     hipLaunchKernelGGL(count_logs_kernel, dim3(1), dim3(64), 0, s, c.t_off, c.n, c.stats);
     ARTEA_HIP_TRY(hipGetLastError());
     ARTEA_HIP_TRY(hipMemsetAsync(c.t_cnt, 0, sizeof(uint32_t) * (c.n + 1), s));
