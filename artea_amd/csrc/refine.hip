@@ -916,6 +916,21 @@ __device__ __forceinline__ unsigned long long sort_key(float d, uint32_t vid, ui
     return (static_cast<unsigned long long>(u) << 32) | (static_cast<unsigned long long>(vid & kMask) << 1) | origin;
 }
 
+// The value of lane (lane ^ j), j a power of two below 64 (a constant once
+// the sort's stages are unrolled): DPP quad permutes for 1 and 2, a row
+// rotate by 8 within 16 lanes for 8, ds_swizzle xor masks within 32 lanes for
+// 4 and 16, ds_bpermute for 32.
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x, uint32_t j) {
+    switch (j) {
+    case 1: return uint32_t(__builtin_amdgcn_mov_dpp(int(x), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    case 2: return uint32_t(__builtin_amdgcn_mov_dpp(int(x), 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    case 4: return uint32_t(__builtin_amdgcn_ds_swizzle(int(x), 0x101F));              // and 0x1F, xor 4
+    case 8: return uint32_t(__builtin_amdgcn_mov_dpp(int(x), 0x128, 0xF, 0xF, false)); // row_ror:8
+    case 16: return uint32_t(__builtin_amdgcn_ds_swizzle(int(x), 0x401F));             // and 0x1F, xor 16
+    default: return __shfl_xor(x, int(j));
+    }
+}
+
 // Bitonic network over 64 * R (key, payload) pairs held four to a lane at
 // most: element r * 64 + lane in k[r], p[r].  Stages with j >= 64 pair
 // registers of one lane, the others pair lanes (lane ^ j).  Equal keys carry
@@ -947,8 +962,8 @@ __device__ __forceinline__ void wave_bitonic(unsigned long long (&k)[4], uint32_
             } else {
 #pragma unroll
                 for (uint32_t r = 0; r < uint32_t(R); ++r) {
-                    const uint32_t lo = __shfl_xor(uint32_t(k[r]), int(j)), hi = __shfl_xor(uint32_t(k[r] >> 32), int(j));
-                    const uint32_t op = __shfl_xor(p[r], int(j));
+                    const uint32_t lo = lane_xor(uint32_t(k[r]), j), hi = lane_xor(uint32_t(k[r] >> 32), j);
+                    const uint32_t op = lane_xor(p[r], j);
                     const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
                     // The lower element of the pair keeps the minimum when the
                     // k2-block sorts ascending, the upper one otherwise.

@@ -200,3 +200,39 @@ def test_row_window_passes_equal_oracle(orc, join, sparse):
         assert rows_of(eng) == orc_rows(L), ("reverse", rb, re)
     eng.next_triangle_range(1.10, 1.5, 5, 5)  # empty window: only a merge of nothing
     assert rows_of(eng) == orc_rows(L)
+
+
+@pytest.mark.parametrize("cap", [48, 96, 128])
+def test_merge_many_logs_per_row_equals_oracle(orc, cap):
+    # A row's logs beyond one sort round (256 entries with the row's own)
+    # take further rounds against the survivors.  Every row lists the hubs 0
+    # and 1, so the reverse updater sends each hub n - 1 logs; the other rows
+    # get a few; rows start in (distance, vid) order, as upload expects, with
+    # mixed new/old flags.
+    n, dim = 3000, 24
+    vecs = aa.generate(0, 11, 0, n, dim)
+    rng = np.random.default_rng(5)
+    d2 = lambda u, v: np.float32(((vecs[u] - vecs[v]) ** 2).sum(dtype=np.float32))
+    counts = np.zeros(n, np.uint32)
+    ent = []
+    for u in range(n):
+        ids = [0, 1] + [int(x) for x in rng.choice(n, int(rng.integers(1, 40)), replace=False)]
+        ids = sorted((v for v in dict.fromkeys(ids) if v != u), key=lambda v: (d2(u, v), v))[:cap]
+        counts[u] = len(ids)
+        ent += [(v | (0x80000000 if rng.random() < 0.5 else 0), d2(u, v)) for v in ids]
+    arr = np.array(ent, dtype=[("vid_status", "<u4"), ("dist", "<f4")])
+    idx = aa.DeviceIndex(vecs, aa.DistanceMetrics.EUCLIDEAN_SQR)
+    eng = PropagateEngine(idx, cap=cap, seed=3)
+    eng.upload(counts, np.stack([arr["vid_status"], arr["dist"].view(np.uint32)], axis=1))
+    L = orc.Layer(vecs, aa.DistanceMetrics.EUCLIDEAN_SQR, cap)
+    L.set_rows_raw(counts, arr)
+    assert rows_of(eng) == orc_rows(L), "uploaded rows"
+    # The reference truncates after the reverse updater; the device merge
+    # keeps cap entries itself.
+    eng.next_reverse()
+    eng.next_truncate()
+    L.reverse(nthreads=8)
+    L.truncate(nthreads=8)
+    got, want = rows_of(eng), orc_rows(L)
+    assert len(got[0]) == cap and len(got[1]) == cap
+    assert got == want
