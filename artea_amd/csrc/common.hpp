@@ -54,6 +54,9 @@ int hgra_load_device(artea_index* idx, const char* path, uint32_t segment_words,
 int hgra_store_device(artea_index* idx, const char* path);
 // Host grouped CSR of the device graph (local ids in vid order per group).
 int graph_from_device(const artea_index* idx, HostCsr* out);
+// Checks a grouped CSR, uploads its device form and keeps it as the index's
+// host graph (moved in, no copy).
+int attach_csr(artea_index* idx, HostCsr&& c);
 // Router internals used by the server: the length of a launch's overflow
 // list, and an asynchronous copy of that list (count, then batch indices)
 // into host memory, ordered after the router's latest launch on `stream`.

@@ -302,13 +302,19 @@ int csr_validate(const HostCsr& g) {
         }
         seen[h][g.local[v]] = 1;
     }
-    for (uint32_t x : g.nbrs) {
-        if (x >= g.n) {
-            snprintf(buf, sizeof buf, "invalid neighbor %u", x);
-            set_error(buf);
-            return ARTEA_EINVAL;
+    // Neighbour ids in parallel; the first bad one, in order, is reported.
+    bool bad = false;
+    const int64_t nn = int64_t(g.nbrs.size());
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+    for (int64_t i = 0; i < nn; ++i) bad = bad || g.nbrs[size_t(i)] >= g.n;
+    if (bad)
+        for (uint32_t x : g.nbrs) {
+            if (x >= g.n) {
+                snprintf(buf, sizeof buf, "invalid neighbor %u", x);
+                set_error(buf);
+                return ARTEA_EINVAL;
+            }
         }
-    }
     if (g.entry != kNoVertex && g.entry >= g.n) {
         set_error("entry point vid is out of range");
         return ARTEA_EINVAL;

@@ -1,5 +1,6 @@
 // Index handle: device vectors plus the device form of an attached compact
 // hierarchical graph; error plumbing; the distance-matrix utility.
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
 #include <memory>
@@ -61,19 +62,29 @@ int upload_graph(artea_index* idx, const HostCsr& c) {
     g.stride0 = round_up(max0, 16);
     g.strideU = round_up(maxU, 16);
     g.rowsU = rowsU;
-    std::vector<uint32_t> ell0(size_t(n) * g.stride0, kNoVertex);
+    // Level-0 rows in parallel (the largest array: n x stride0).
+    const size_t n0 = std::max<size_t>(size_t(n) * g.stride0, 1);
+    std::unique_ptr<uint32_t[]> ell0(new uint32_t[n0]);
+    ell0[0] = kNoVertex;
+#pragma omp parallel for schedule(static)
+    for (int64_t vv = 0; vv < int64_t(n); ++vv) {
+        const uint32_t v = uint32_t(vv);
+        uint32_t cnt;
+        const uint32_t* r = c.row(v, 0, &cnt);
+        uint32_t* dst = ell0.get() + size_t(v) * g.stride0;
+        memcpy(dst, r, size_t(cnt) * 4);
+        std::fill(dst + cnt, dst + g.stride0, kNoVertex);
+    }
     std::vector<uint32_t> ellU(std::max<uint64_t>(rowsU, 1) * g.strideU, kNoVertex);
     std::vector<uint2> vinfo(std::max<uint32_t>(n, 1), uint2{kNoVertex, 0});
     uint64_t ub = 0;
     for (uint32_t v = 0; v < n; ++v) {
         const uint32_t h = c.highest[v];
         if (h == kInvalidLevel) continue;
-        uint32_t cnt;
-        const uint32_t* r = c.row(v, 0, &cnt);
-        memcpy(ell0.data() + size_t(v) * g.stride0, r, size_t(cnt) * 4);
         vinfo[v] = uint2{h, uint32_t(ub)};
         for (uint32_t l = h; l >= 1; --l) {
-            r = c.row(v, l, &cnt);
+            uint32_t cnt;
+            const uint32_t* r = c.row(v, l, &cnt);
             memcpy(ellU.data() + size_t(ub + (h - l)) * g.strideU, r, size_t(cnt) * 4);
         }
         ub += h;
@@ -88,10 +99,10 @@ int upload_graph(artea_index* idx, const HostCsr& c) {
         return ARTEA_ENOMEM;
     };
     hipError_t e;
-    if ((e = hipMalloc(&g.ell0, std::max<size_t>(ell0.size(), 1) * 4)) != hipSuccess) return fail(e, "hipMalloc ell0");
+    if ((e = hipMalloc(&g.ell0, n0 * 4)) != hipSuccess) return fail(e, "hipMalloc ell0");
     if ((e = hipMalloc(&g.ellU, ellU.size() * 4)) != hipSuccess) return fail(e, "hipMalloc ellU");
     if ((e = hipMalloc(&g.vinfo, vinfo.size() * sizeof(uint2))) != hipSuccess) return fail(e, "hipMalloc vinfo");
-    if ((e = hipMemcpy(g.ell0, ell0.data(), ell0.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpy(g.ell0, ell0.get(), n0 * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "upload ell0");
     if ((e = hipMemcpy(g.ellU, ellU.data(), ellU.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "upload ellU");
@@ -197,6 +208,16 @@ int set_u8_rows(artea_index* idx, bool enable, int* active) {
 
 }  // namespace
 
+int artea::attach_csr(artea_index* idx, HostCsr&& c) {
+    int rc = csr_validate(c);
+    if (rc) return rc;
+    rc = upload_graph(idx, c);
+    if (rc) return rc;
+    idx->csr = std::move(c);
+    idx->csr_current = true;
+    return ARTEA_OK;
+}
+
 extern "C" {
 
 const char* artea_last_error(void) { return artea::g_last_error.c_str(); }
@@ -285,14 +306,9 @@ int artea_index_set_graph_csr(artea_index* idx, uint32_t top, uint32_t ul_max, u
         }
         c.nbrs.assign(nbrs, nbrs + n_nbrs);
     }
-    int rc = csr_validate(c);
-    if (rc) return rc;
-    rc = upload_graph(idx, c);
-    if (rc) return rc;
-    idx->csr = std::move(c);
-    idx->csr_current = true;
-    return ARTEA_OK;
+    return attach_csr(idx, std::move(c));
 }
+
 
 int artea_index_set_graph_hgra(artea_index* idx, const char* path) {
     clear_error();

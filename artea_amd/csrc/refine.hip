@@ -1466,6 +1466,64 @@ __global__ void centroid_final_kernel(const double* part, uint32_t blocks, uint3
 // its highest level down.  Entry: the top-bucket vertex closest, by
 // `metric`, to the FP64 centroid of the top bucket, computed on the GPU;
 // ties keep the earliest bucket position.
+// The compactor reads the ids of each row's first min(cnt, slot_cap)
+// entries: they are packed slot_cap apart on the device and only they are
+// copied back.  Packed row i is rows[map ? map[i] : i].
+__global__ void pack_ids_kernel(const uint2* rows, const uint32_t* cnt, const uint32_t* map, uint32_t nrows,
+                                uint32_t stride, uint32_t slot_cap, uint32_t* ids, uint32_t* out_cnt) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= size_t(nrows) * slot_cap) return;
+    const uint32_t i = uint32_t(t / slot_cap), k = uint32_t(t % slot_cap);
+    const uint32_t src = map ? map[i] : i;
+    const uint32_t c = min(cnt[src], slot_cap);
+    if (k == 0) out_cnt[i] = c;
+    if (k < c) ids[t] = rows[size_t(src) * stride + k].x & kMask;
+}
+
+// Level l of the compactor's input from nrows device rows (stride entries
+// apart; row of members[i] at map[i], or at i without a map): row_off over
+// all n vertices and the ids back to back in vid order.
+int fetch_level(const uint2* d_rows, const uint32_t* d_cnt, const uint32_t* map, const std::vector<uint32_t>& members,
+                uint32_t n, uint32_t stride, uint32_t slot_cap, hipStream_t s, std::vector<uint64_t>& roff,
+                std::unique_ptr<uint32_t[]>& rids) {
+    const uint32_t nl = uint32_t(members.size());
+    const size_t slots = std::max<size_t>(size_t(nl) * slot_cap, 1);
+    uint32_t *d_ids = nullptr, *d_out = nullptr, *d_map = nullptr;
+    auto release = [&]() {
+        for (void* q : {(void*)d_ids, (void*)d_out, (void*)d_map})
+            if (q) (void)hipFree(q);
+    };
+    hipError_t e = hipMalloc(&d_ids, slots * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_out, std::max<size_t>(nl, 1) * 4);
+    if (e == hipSuccess && map) e = hipMalloc(&d_map, std::max<size_t>(nl, 1) * 4);
+    if (e == hipSuccess && map && nl) e = hipMemcpyAsync(d_map, map, size_t(nl) * 4, hipMemcpyHostToDevice, s);
+    std::unique_ptr<uint32_t[]> ids(new uint32_t[slots]);
+    std::vector<uint32_t> cnt(std::max<uint32_t>(nl, 1));
+    if (e == hipSuccess && nl) {
+        hipLaunchKernelGGL(pack_ids_kernel, dim3(uint32_t((size_t(nl) * slot_cap + 255) / 256)), dim3(256), 0, s,
+                           d_rows, d_cnt, d_map, nl, stride, slot_cap, d_ids, d_out);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(ids.get(), d_ids, size_t(nl) * slot_cap * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_out, size_t(nl) * 4, hipMemcpyDeviceToHost, s);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    release();
+    if (e != hipSuccess) {
+        set_error(std::string("compactor input: ") + hipGetErrorString(e));
+        return ARTEA_EHIP;
+    }
+    roff.assign(size_t(n) + 1, 0);
+    for (uint32_t i = 0; i < nl; ++i) roff[members[i] + 1] = cnt[i];
+    for (uint32_t v = 0; v < n; ++v) roff[v + 1] += roff[v];
+    rids.reset(new uint32_t[std::max<uint64_t>(roff[n], 1)]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < int64_t(nl); ++i) {
+        const uint32_t v = members[size_t(i)];
+        memcpy(rids.get() + roff[v], ids.get() + size_t(i) * slot_cap, size_t(roff[v + 1] - roff[v]) * 4);
+    }
+    return ARTEA_OK;
+}
+
 int compact_and_attach(artea_index* idx, const uint32_t* src_h, const uint32_t* order,
                        const uint64_t* const* row_off, const uint32_t* const* row_ids, uint32_t ul_max,
                        uint32_t bl_max, int metric) {
@@ -1510,6 +1568,8 @@ int compact_and_attach(artea_index* idx, const uint32_t* src_h, const uint32_t* 
             }
         }
     }
+    // Group offsets first (rows of a vertex from its highest level down),
+    // then the ids copied in parallel.
     g.offsets.resize(top + 1);
     uint64_t total = 0;
     for (uint32_t h = 0; h <= top; ++h) {
@@ -1519,13 +1579,25 @@ int compact_and_attach(artea_index* idx, const uint32_t* src_h, const uint32_t* 
             const uint32_t v = bucket[h][loc];
             for (uint32_t l = h;; --l) {
                 o[loc * (h + 1) + (h - l)] = total;
-                const uint64_t a = row_off[l][v], b = row_off[l][v + 1];
-                g.nbrs.insert(g.nbrs.end(), row_ids[l] + a, row_ids[l] + b);
-                total += b - a;
+                total += row_off[l][v + 1] - row_off[l][v];
                 if (l == 0) break;
             }
         }
         o.back() = total;
+    }
+    g.nbrs.resize(total);
+    for (uint32_t h = 0; h <= top; ++h) {
+        const auto& o = g.offsets[h];
+        const std::vector<uint32_t>& bk = bucket[h];
+#pragma omp parallel for schedule(static)
+        for (int64_t loc = 0; loc < int64_t(bk.size()); ++loc) {
+            const uint32_t v = bk[size_t(loc)];
+            for (uint32_t l = h;; --l) {
+                const uint64_t a = row_off[l][v], b = row_off[l][v + 1];
+                memcpy(g.nbrs.data() + o[size_t(loc) * (h + 1) + (h - l)], row_ids[l] + a, size_t(b - a) * 4);
+                if (l == 0) break;
+            }
+        }
     }
     {
         const std::vector<uint32_t>& tb = bucket[top];
@@ -1576,15 +1648,7 @@ int compact_and_attach(artea_index* idx, const uint32_t* src_h, const uint32_t* 
             }
         g.entry = bv;
     }
-    std::vector<uint32_t> vi(std::max<size_t>(size_t(n) * 2, 2));
-    for (uint32_t v = 0; v < n; ++v) {
-        vi[2 * size_t(v)] = g.highest[v];
-        vi[2 * size_t(v) + 1] = g.local[v];
-    }
-    std::vector<const uint64_t*> ptrs;
-    for (auto& x : g.offsets) ptrs.push_back(x.data());
-    return artea_index_set_graph_csr(idx, g.top, g.ul_max, g.bl_max, g.entry, vi.data(), ptrs.data(), g.nbrs.data(),
-                                     g.nbrs.size());
+    return attach_csr(idx, std::move(g));
 }
 }  // namespace
 
@@ -1860,7 +1924,7 @@ int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max,
     }
     // Rows of every refined level, indexed by vid (members only).
     std::vector<std::vector<uint64_t>> roff(top + 1);
-    std::vector<std::vector<uint32_t>> rids(top + 1);
+    std::vector<std::unique_ptr<uint32_t[]>> rids(top + 1);
     for (uint32_t l = 0; l <= top; ++l) {
         const auto t0 = std::chrono::steady_clock::now();
         std::vector<uint32_t> members;
@@ -1887,21 +1951,8 @@ int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max,
         rc = do_routing_loops(r, routing_loops, routing_topk, routing_queue, scale_coeffs);
         if (rc) return rc;
         // writeback_layer_from_single_layer_graph: first slot_cap entries.
-        std::vector<uint2> rows(std::max<size_t>(size_t(nl) * cap, 1));
-        std::vector<uint32_t> cnt(std::max<uint32_t>(nl, 1));
-        ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), r->c.rows, size_t(nl) * cap * sizeof(uint2), hipMemcpyDeviceToHost,
-                                     r->s));
-        ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), r->c.cnt, size_t(nl) * 4, hipMemcpyDeviceToHost, r->s));
-        ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
-        roff[l].assign(size_t(n) + 1, 0);
-        for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[i], slot_cap);
-        for (uint32_t v = 0; v < n; ++v) roff[l][v + 1] += roff[l][v];
-        rids[l].resize(roff[l][n]);
-        for (uint32_t i = 0; i < nl; ++i) {
-            const uint32_t v = members[i];
-            for (uint64_t k = 0; k < roff[l][v + 1] - roff[l][v]; ++k)
-                rids[l][roff[l][v] + k] = rows[size_t(i) * cap + k].x & kMask;
-        }
+        rc = fetch_level(r->c.rows, r->c.cnt, nullptr, members, n, cap, slot_cap, r->s, roff[l], rids[l]);
+        if (rc) return rc;
         if (seconds_per_level)
             seconds_per_level[l] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -1909,7 +1960,7 @@ int artea_build_full_routing(artea_index* idx, uint32_t ul_max, uint32_t bl_max,
     std::vector<const uint32_t*> idp(top + 1);
     for (uint32_t l = 0; l <= top; ++l) {
         offp[l] = roff[l].data();
-        idp[l] = rids[l].data();
+        idp[l] = rids[l].get();
     }
     return compact_and_attach(idx, src_h.data(), nullptr, offp.data(), idp.data(), ul_max, bl_max, bmetric);
 }
@@ -1944,7 +1995,7 @@ extern "C" int artea_build_full_rnet(artea_index* idx, const artea_rnet_params* 
     const uint32_t L = R->L;
     const uint32_t top = R->top == kInvalidLevel ? 0u : R->top;
     std::vector<std::vector<uint64_t>> roff(L + 1);
-    std::vector<std::vector<uint32_t>> rids(L + 1);
+    std::vector<std::unique_ptr<uint32_t[]>> rids(L + 1);
     for (uint32_t l = 0; l <= L; ++l) {
         std::vector<uint32_t> members;
         for (uint32_t v = 0; v < n; ++v)
@@ -1952,9 +2003,6 @@ extern "C" int artea_build_full_rnet(artea_index* idx, const artea_rnet_params* 
         const uint32_t nl = uint32_t(members.size());
         const uint32_t slot_cap = l == 0 ? p->bl_max_nbr_size : p->ul_max_nbr_size;
         roff[l].assign(size_t(n) + 1, 0);
-        std::vector<uint2> rows;
-        std::vector<uint32_t> cnt;
-        uint32_t stride = 0;
         if (l <= top && nl >= 32) {
             const uint32_t cap = uint32_t(float(slot_cap) * 1.5f);
             artea_refine* r = nullptr;
@@ -1978,39 +2026,22 @@ extern "C" int artea_build_full_rnet(artea_index* idx, const artea_rnet_params* 
             if (rc) return rc;
             rc = do_routing_loops(r, routing_loops, routing_topk, routing_queue, p->scale_coeffs);
             if (rc) return rc;
-            rows.resize(std::max<size_t>(size_t(nl) * cap, 1));
-            cnt.resize(std::max<uint32_t>(nl, 1));
-            ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), r->c.rows, size_t(nl) * cap * sizeof(uint2),
-                                         hipMemcpyDeviceToHost, r->s));
-            ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), r->c.cnt, size_t(nl) * 4, hipMemcpyDeviceToHost, r->s));
-            ARTEA_HIP_TRY(hipStreamSynchronize(r->s));
-            stride = cap;
-            for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[i], slot_cap);
+            rc = fetch_level(r->c.rows, r->c.cnt, nullptr, members, n, cap, slot_cap, r->s, roff[l], rids[l]);
+            if (rc) return rc;
         } else if (R->rows[l] && nl > 0) {
-            // not refined: the r-net rows as they are (by vid)
-            rows.resize(size_t(n) * slot_cap);
-            cnt.resize(n);
-            ARTEA_HIP_TRY(hipMemcpyAsync(rows.data(), R->rows[l], size_t(n) * slot_cap * sizeof(uint2),
-                                         hipMemcpyDeviceToHost, R->s));
-            ARTEA_HIP_TRY(hipMemcpyAsync(cnt.data(), R->cnt[l], size_t(n) * 4, hipMemcpyDeviceToHost, R->s));
-            ARTEA_HIP_TRY(hipStreamSynchronize(R->s));
-            stride = slot_cap;
-            for (uint32_t i = 0; i < nl; ++i) roff[l][members[i] + 1] = std::min(cnt[members[i]], slot_cap);
-        }
-        for (uint32_t v = 0; v < n; ++v) roff[l][v + 1] += roff[l][v];
-        rids[l].resize(roff[l][n]);
-        const bool by_vid = !(l <= top && nl >= 32);
-        for (uint32_t i = 0; i < nl; ++i) {
-            const uint32_t v = members[i];
-            const size_t base = size_t(by_vid ? v : i) * stride;
-            for (uint64_t k = 0; k < roff[l][v + 1] - roff[l][v]; ++k) rids[l][roff[l][v] + k] = rows[base + k].x & kMask;
+            // not refined: the r-net rows of the members as they are (by vid)
+            rc = fetch_level(R->rows[l], R->cnt[l], members.data(), members, n, slot_cap, slot_cap, R->s, roff[l],
+                             rids[l]);
+            if (rc) return rc;
+        } else {
+            rids[l].reset(new uint32_t[1]);
         }
     }
     std::vector<const uint64_t*> offp(L + 1);
     std::vector<const uint32_t*> idp(L + 1);
     for (uint32_t l = 0; l <= L; ++l) {
         offp[l] = roff[l].data();
-        idp[l] = rids[l].data();
+        idp[l] = rids[l].get();
     }
     rc = compact_and_attach(idx, R->highest.data(), nullptr, offp.data(), idp.data(), p->ul_max_nbr_size,
                             p->bl_max_nbr_size, bmetric);
