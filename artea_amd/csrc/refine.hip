@@ -1482,44 +1482,80 @@ __global__ void pack_ids_kernel(const uint2* rows, const uint32_t* cnt, const ui
 
 // Level l of the compactor's input from nrows device rows (stride entries
 // apart; row of members[i] at map[i], or at i without a map): row_off over
-// all n vertices and the ids back to back in vid order.
+// all n vertices and the ids back to back in vid order.  The packed ids come
+// back through two pinned staging halves of kStageBytes, each half's rows
+// placed into their final slots while the next half is in flight.
+constexpr size_t kStageBytes = size_t(64) << 20;
+
 int fetch_level(const uint2* d_rows, const uint32_t* d_cnt, const uint32_t* map, const std::vector<uint32_t>& members,
                 uint32_t n, uint32_t stride, uint32_t slot_cap, hipStream_t s, std::vector<uint64_t>& roff,
                 std::unique_ptr<uint32_t[]>& rids) {
     const uint32_t nl = uint32_t(members.size());
-    const size_t slots = std::max<size_t>(size_t(nl) * slot_cap, 1);
-    uint32_t *d_ids = nullptr, *d_out = nullptr, *d_map = nullptr;
+    roff.assign(size_t(n) + 1, 0);
+    if (nl == 0) {
+        rids.reset(new uint32_t[1]);
+        return ARTEA_OK;
+    }
+    const uint32_t chunk_rows = uint32_t(std::max<size_t>(1, kStageBytes / (size_t(slot_cap) * 4)));
+    const size_t chunk_elems = size_t(chunk_rows) * slot_cap;
+    uint32_t *d_ids = nullptr, *d_out = nullptr, *d_map = nullptr, *stage = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
     auto release = [&]() {
+        (void)hipStreamSynchronize(s);  // no copy may still target the staging halves
         for (void* q : {(void*)d_ids, (void*)d_out, (void*)d_map})
             if (q) (void)hipFree(q);
+        if (stage) (void)hipHostFree(stage);
+        for (hipEvent_t x : ev)
+            if (x) (void)hipEventDestroy(x);
     };
-    hipError_t e = hipMalloc(&d_ids, slots * 4);
-    if (e == hipSuccess) e = hipMalloc(&d_out, std::max<size_t>(nl, 1) * 4);
-    if (e == hipSuccess && map) e = hipMalloc(&d_map, std::max<size_t>(nl, 1) * 4);
-    if (e == hipSuccess && map && nl) e = hipMemcpyAsync(d_map, map, size_t(nl) * 4, hipMemcpyHostToDevice, s);
-    std::unique_ptr<uint32_t[]> ids(new uint32_t[slots]);
-    std::vector<uint32_t> cnt(std::max<uint32_t>(nl, 1));
-    if (e == hipSuccess && nl) {
+    std::vector<uint32_t> cnt(nl);
+    hipError_t e = hipMalloc(&d_ids, size_t(nl) * slot_cap * 4);
+    if (e == hipSuccess) e = hipMalloc(&d_out, size_t(nl) * 4);
+    if (e == hipSuccess && map) e = hipMalloc(&d_map, size_t(nl) * 4);
+    if (e == hipSuccess && map) e = hipMemcpyAsync(d_map, map, size_t(nl) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&stage), 2 * chunk_elems * 4, hipHostMallocDefault);
+    for (int h = 0; h < 2 && e == hipSuccess; ++h) e = hipEventCreateWithFlags(&ev[h], hipEventDisableTiming);
+    if (e == hipSuccess) {
         hipLaunchKernelGGL(pack_ids_kernel, dim3(uint32_t((size_t(nl) * slot_cap + 255) / 256)), dim3(256), 0, s,
                            d_rows, d_cnt, d_map, nl, stride, slot_cap, d_ids, d_out);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(ids.get(), d_ids, size_t(nl) * slot_cap * 4, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_out, size_t(nl) * 4, hipMemcpyDeviceToHost, s);
     }
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_out, size_t(nl) * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) {
+        for (uint32_t i = 0; i < nl; ++i) roff[members[i] + 1] = cnt[i];
+        for (uint32_t v = 0; v < n; ++v) roff[v + 1] += roff[v];
+        rids.reset(new uint32_t[std::max<uint64_t>(roff[n], 1)]);
+        const uint32_t chunks = (nl + chunk_rows - 1) / chunk_rows;
+        auto issue = [&](uint32_t c) {
+            const uint32_t r0 = c * chunk_rows, r1 = std::min(nl, r0 + chunk_rows);
+            hipError_t x = hipMemcpyAsync(stage + (c & 1u) * chunk_elems, d_ids + size_t(r0) * slot_cap,
+                                          size_t(r1 - r0) * slot_cap * 4, hipMemcpyDeviceToHost, s);
+            return x == hipSuccess ? hipEventRecord(ev[c & 1u], s) : x;
+        };
+        auto place = [&](uint32_t c) {
+            hipError_t x = hipEventSynchronize(ev[c & 1u]);
+            if (x != hipSuccess) return x;
+            const uint32_t r0 = c * chunk_rows, r1 = std::min(nl, r0 + chunk_rows);
+            const uint32_t* src = stage + (c & 1u) * chunk_elems;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = r0; i < int64_t(r1); ++i) {
+                const uint32_t v = members[size_t(i)];
+                memcpy(rids.get() + roff[v], src + size_t(i - r0) * slot_cap, size_t(roff[v + 1] - roff[v]) * 4);
+            }
+            return hipSuccess;
+        };
+        e = issue(0);
+        for (uint32_t c = 1; c < chunks && e == hipSuccess; ++c) {
+            e = issue(c);
+            if (e == hipSuccess) e = place(c - 1);
+        }
+        if (e == hipSuccess) e = place(chunks - 1);
+    }
     release();
     if (e != hipSuccess) {
         set_error(std::string("compactor input: ") + hipGetErrorString(e));
         return ARTEA_EHIP;
-    }
-    roff.assign(size_t(n) + 1, 0);
-    for (uint32_t i = 0; i < nl; ++i) roff[members[i] + 1] = cnt[i];
-    for (uint32_t v = 0; v < n; ++v) roff[v + 1] += roff[v];
-    rids.reset(new uint32_t[std::max<uint64_t>(roff[n], 1)]);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < int64_t(nl); ++i) {
-        const uint32_t v = members[size_t(i)];
-        memcpy(rids.get() + roff[v], ids.get() + size_t(i) * slot_cap, size_t(roff[v + 1] - roff[v]) * 4);
     }
     return ARTEA_OK;
 }
