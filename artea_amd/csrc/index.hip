@@ -37,6 +37,45 @@ void artea::replace_device_graph(artea_index* idx, const DeviceGraph& g) {
 
 namespace {
 
+// Level-0 ELL rows (n x stride0, the largest array) into g.ell0: built in
+// parallel into two pinned staging halves of 64 MB, each uploaded while the
+// other is filled.
+hipError_t upload_level0(const HostCsr& c, const DeviceGraph& g, hipStream_t s) {
+    const uint32_t n = c.n, stride = g.stride0;
+    if (n == 0) return hipMemcpy(g.ell0, &kNoVertex, 4, hipMemcpyHostToDevice);
+    const uint32_t chunk_rows = uint32_t(std::max<size_t>(1, (size_t(64) << 20) / (size_t(stride) * 4)));
+    const size_t chunk_elems = size_t(chunk_rows) * stride;
+    uint32_t* stage = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&stage), 2 * chunk_elems * 4, hipHostMallocDefault);
+    for (int h = 0; h < 2 && e == hipSuccess; ++h) e = hipEventCreateWithFlags(&ev[h], hipEventDisableTiming);
+    const uint32_t chunks = (n + chunk_rows - 1) / chunk_rows;
+    for (uint32_t k = 0; k < chunks && e == hipSuccess; ++k) {
+        // the half this chunk fills was last uploaded two chunks ago
+        if (k >= 2) e = hipEventSynchronize(ev[k & 1u]);
+        if (e != hipSuccess) break;
+        const uint32_t v0 = k * chunk_rows, v1 = std::min(n, v0 + chunk_rows);
+        uint32_t* half = stage + (k & 1u) * chunk_elems;
+#pragma omp parallel for schedule(static)
+        for (int64_t vv = v0; vv < int64_t(v1); ++vv) {
+            const uint32_t v = uint32_t(vv);
+            uint32_t cnt;
+            const uint32_t* r = c.row(v, 0, &cnt);
+            uint32_t* dst = half + size_t(v - v0) * stride;
+            memcpy(dst, r, size_t(cnt) * 4);
+            std::fill(dst + cnt, dst + stride, kNoVertex);
+        }
+        e = hipMemcpyAsync(g.ell0 + size_t(v0) * stride, half, size_t(v1 - v0) * stride * 4, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[k & 1u], s);
+    }
+    const hipError_t f = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = f;
+    if (stage) (void)hipHostFree(stage);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return e;
+}
+
 // Converts the grouped CSR into the device ELL layout and uploads it.
 int upload_graph(artea_index* idx, const HostCsr& c) {
     DeviceGraph g;
@@ -62,19 +101,7 @@ int upload_graph(artea_index* idx, const HostCsr& c) {
     g.stride0 = round_up(max0, 16);
     g.strideU = round_up(maxU, 16);
     g.rowsU = rowsU;
-    // Level-0 rows in parallel (the largest array: n x stride0).
     const size_t n0 = std::max<size_t>(size_t(n) * g.stride0, 1);
-    std::unique_ptr<uint32_t[]> ell0(new uint32_t[n0]);
-    ell0[0] = kNoVertex;
-#pragma omp parallel for schedule(static)
-    for (int64_t vv = 0; vv < int64_t(n); ++vv) {
-        const uint32_t v = uint32_t(vv);
-        uint32_t cnt;
-        const uint32_t* r = c.row(v, 0, &cnt);
-        uint32_t* dst = ell0.get() + size_t(v) * g.stride0;
-        memcpy(dst, r, size_t(cnt) * 4);
-        std::fill(dst + cnt, dst + g.stride0, kNoVertex);
-    }
     std::vector<uint32_t> ellU(std::max<uint64_t>(rowsU, 1) * g.strideU, kNoVertex);
     std::vector<uint2> vinfo(std::max<uint32_t>(n, 1), uint2{kNoVertex, 0});
     uint64_t ub = 0;
@@ -102,8 +129,7 @@ int upload_graph(artea_index* idx, const HostCsr& c) {
     if ((e = hipMalloc(&g.ell0, n0 * 4)) != hipSuccess) return fail(e, "hipMalloc ell0");
     if ((e = hipMalloc(&g.ellU, ellU.size() * 4)) != hipSuccess) return fail(e, "hipMalloc ellU");
     if ((e = hipMalloc(&g.vinfo, vinfo.size() * sizeof(uint2))) != hipSuccess) return fail(e, "hipMalloc vinfo");
-    if ((e = hipMemcpy(g.ell0, ell0.get(), n0 * 4, hipMemcpyHostToDevice)) != hipSuccess)
-        return fail(e, "upload ell0");
+    if ((e = upload_level0(c, g, idx->stream)) != hipSuccess) return fail(e, "upload ell0");
     if ((e = hipMemcpy(g.ellU, ellU.data(), ellU.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return fail(e, "upload ellU");
     if ((e = hipMemcpy(g.vinfo, vinfo.data(), vinfo.size() * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess)
