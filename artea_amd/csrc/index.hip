@@ -43,7 +43,7 @@ namespace {
 hipError_t upload_level0(const HostCsr& c, const DeviceGraph& g, hipStream_t s) {
     const uint32_t n = c.n, stride = g.stride0;
     if (n == 0) return hipMemcpy(g.ell0, &kNoVertex, 4, hipMemcpyHostToDevice);
-    const uint32_t chunk_rows = uint32_t(std::max<size_t>(1, (size_t(64) << 20) / (size_t(stride) * 4)));
+    const uint32_t chunk_rows = std::min(n, uint32_t(std::max<size_t>(1, (size_t(64) << 20) / (size_t(stride) * 4))));
     const size_t chunk_elems = size_t(chunk_rows) * stride;
     uint32_t* stage = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};

@@ -1496,7 +1496,8 @@ int fetch_level(const uint2* d_rows, const uint32_t* d_cnt, const uint32_t* map,
         rids.reset(new uint32_t[1]);
         return ARTEA_OK;
     }
-    const uint32_t chunk_rows = uint32_t(std::max<size_t>(1, kStageBytes / (size_t(slot_cap) * 4)));
+    const uint32_t chunk_rows =
+        std::min(nl, uint32_t(std::max<size_t>(1, kStageBytes / (size_t(std::max(slot_cap, 1u)) * 4))));
     const size_t chunk_elems = size_t(chunk_rows) * slot_cap;
     uint32_t *d_ids = nullptr, *d_out = nullptr, *d_map = nullptr, *stage = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
