@@ -855,9 +855,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=100000,
                     help="requests per offered-load level of the serving leg (0 = skip)")
     ap.add_argument("--serve-lanes", type=int, default=None,
-                    help="serving: overlapping launch lanes (streams); default 3 for search, 4 for build")
+                    help="serving: overlapping launch lanes (streams); default 2 for search, 4 for build")
     ap.add_argument("--serve-budget", type=int, default=None,
-                    help="serving: most requests one lane launches at once; default 8192 for search, 2048 for build")
+                    help="serving: most requests one lane launches at once; default 6144 for search, 2048 for build")
     ap.add_argument("--serve-hbm-mb", type=float, default=0.0,
                     help="serving: admit by device memory instead of --serve-budget: the lanes' buffers share this "
                          "many MiB, and a launch takes at most --serve-lds-rounds times its kernel's resident waves")
@@ -875,9 +875,9 @@ def main():
     if args.level_fraction is None:
         args.level_fraction = 1.0 if args.workload == "search" else 1.0 / 32
     if args.serve_lanes is None:
-        args.serve_lanes = 3 if args.workload == "search" else 4
+        args.serve_lanes = 2 if args.workload == "search" else 4
     if args.serve_budget is None:
-        args.serve_budget = 8192 if args.workload == "search" else 2048
+        args.serve_budget = 6144 if args.workload == "search" else 2048
     if args.workload == "cosine":
         return workload_cosine(args)
     if args.workload == "c0":
