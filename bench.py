@@ -74,7 +74,7 @@ def gpu_build(idx, args, seed, join=0):
     l0 = resolve_l0(idx, args, seed)
     if args.hierarchy == "rnet":
         rs, total = build_full_rnet(idx, rnet_params(seed=seed, l0_min_distance=l0), build_loops=15, triu_iters=4,
-                                    prefill=0.4, shift=1.5, join=join)
+                                    prefill=0.4, shift=1.5, join=join, routing_loops=args.routing_loops)
         return np.array([rs, total - rs])
     return build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
                       l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction, join=join)
@@ -106,7 +106,8 @@ def cpu_build(orc, base, args, seed, cores, l0):
     GPU build used)."""
     if args.hierarchy == "rnet":
         return orc.build_full_rnet(base, orc.rnet_params(seed=seed, l0_min_distance=l0), metric=orc.L2,
-                                   build_loops=15, triu_iters=4, prefill=0.4, shift=1.5, nthreads=cores)
+                                   build_loops=15, triu_iters=4, prefill=0.4, shift=1.5,
+                                   routing_loops=args.routing_loops, nthreads=cores)
     return orc.build_full(base, metric=orc.L2, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4,
                           scale=1.10, shift=1.5, l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction,
                           nthreads=cores)
@@ -170,7 +171,8 @@ def workload_cosine(args):
     out = {"metric": "QPS @ recall>=0.95 (1M x 100 cosine)", "unit": "queries/s", "n_gpus": 1,
            "higher_is_better": True, "dtype": "f32", "data": "synthetic (power-law-spectrum Gaussian, unit-norm)",
            "config": {"workload": "cosine search, 1,000,000 x 100 (pad 112), %d queries" % nq, "n": n, "nq": nq},
-           "build_seconds": round(build_s, 2), "hierarchy": args.hierarchy, "level_fraction": args.level_fraction}
+           "build_seconds": round(build_s, 2), "hierarchy": args.hierarchy,
+           "routing_loops": args.routing_loops, "level_fraction": args.level_fraction}
     for topk in (10, 100):
         queues = [L for L in (64, 100, 200, 400, 800) if L >= topk]
         rows, best = sweep_queues(aa, idx, q_d, nq, gt_i[:, :topk], topk, queues, stream, args.target_recall)
@@ -272,7 +274,8 @@ def workload_c0(args):
            "dtype": "u8" if idx.u8_rows else "f32",
            "data": "synthetic (seeded 256-component integer Gaussian mixture)",
            "config": {"workload": "C0: 100,000 x 128 L2, 1,000 queries, k 100, queue 200", "n": n, "nq": nq},
-           "gpu_build_s": round(gpu_build_s, 3), "hierarchy": args.hierarchy, "recall@100": round(rec, 4),
+           "gpu_build_s": round(gpu_build_s, 3), "hierarchy": args.hierarchy,
+           "routing_loops": args.routing_loops, "recall@100": round(rec, 4),
            "plain_qps": round(nq / t_plain, 1),
            "batched_qps": round(nq / t_batched, 1), "batched_equals_plain": same_plain}
     if not args.no_cpu:
@@ -720,6 +723,7 @@ def workload_build(args):
            "config": {"workload": "hierarchy build (every level) + serving, %d x 96 L2" % n, "n": n, "nq": nq},
            "builds": builds, "bf16_speedup": round(builds["fp32"]["seconds"] / builds["bf16"]["seconds"], 3),
            "edge_sets_identical": same, "top": idx.info()["top"], "hierarchy": args.hierarchy,
+           "routing_loops": args.routing_loops,
            "l0_min_distance": resolve_l0(idx, args, seed),
            "level_fraction": args.level_fraction,
            "ground_truth_s": round(gt_s, 2), "search_sweep": rows, "search_chosen": best}
@@ -831,6 +835,9 @@ def main():
     ap.add_argument("--hierarchy", choices=("rnet", "geometric"), default="rnet",
                     help="levels and upper-level rows from r-net insertion (the reference's pipeline) or from the "
                          "seeded geometric draw with a random prefill")
+    ap.add_argument("--routing-loops", type=int, default=0,
+                    help="r-net builds: routing loops (RoutingUpdater topk 64, queue 96) after each level's build "
+                         "loops; the configs name none (PropagateConfig's default 0), the reference recommends 1")
     ap.add_argument("--l0-min-distance", default="1.0",
                     help="a value, or 'probe' for the reference's auto-probe (quantile 0.9, 500 samples)")
     ap.add_argument("--level-fraction", type=float, default=None,
