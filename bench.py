@@ -77,7 +77,8 @@ def gpu_build(idx, args, seed, join=0):
                                     prefill=0.4, shift=1.5, join=join, routing_loops=args.routing_loops)
         return np.array([rs, total - rs])
     return build_full(idx, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4, scale=1.10, shift=1.5,
-                      l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction, join=join)
+                      l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction, join=join,
+                      routing_loops=args.routing_loops)
 
 
 def resolve_l0(idx, args, seed):
@@ -110,7 +111,7 @@ def cpu_build(orc, base, args, seed, cores, l0):
                                    routing_loops=args.routing_loops, nthreads=cores)
     return orc.build_full(base, metric=orc.L2, ul_max=32, bl_max=64, build_loops=15, triu_iters=4, prefill=0.4,
                           scale=1.10, shift=1.5, l0_min_distance=l0, seed=seed, level_fraction=args.level_fraction,
-                          nthreads=cores)
+                          routing_loops=args.routing_loops, nthreads=cores)
 
 
 def sweep_queues(aa, idx, q_d, nq, gt_i, topk, queues, stream, target):
@@ -836,8 +837,8 @@ def main():
                     help="levels and upper-level rows from r-net insertion (the reference's pipeline) or from the "
                          "seeded geometric draw with a random prefill")
     ap.add_argument("--routing-loops", type=int, default=0,
-                    help="r-net builds: routing loops (RoutingUpdater topk 64, queue 96) after each level's build "
-                         "loops; the configs name none (PropagateConfig's default 0), the reference recommends 1")
+                    help="routing loops (RoutingUpdater topk 64, queue 96) after each level's build loops; the "
+                         "configs name none (PropagateConfig's default 0), the reference recommends 1")
     ap.add_argument("--l0-min-distance", default="1.0",
                     help="a value, or 'probe' for the reference's auto-probe (quantile 0.9, 500 samples)")
     ap.add_argument("--level-fraction", type=float, default=None,
