@@ -136,16 +136,55 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
         if (P) W.merge_chunk(myd, id, P, w, [](float, uint32_t) {});
     }
 
-    // beam_loop_body over the layer's rows (DynamicFlatRange).
+    // beam_loop_body over the layer's rows (DynamicFlatRange).  The row of
+    // the entry the next pop will most likely return (its count and first 64
+    // ids) is read after this row's first ids are consumed and before their
+    // distance gathers, whose waits then cover it; a merge whose smallest
+    // accepted key precedes that entry reads the new first entry's row
+    // instead.  The rows do not change during a routing pass, so a read row
+    // is the row the pop finds.
+    uint32_t pre_v = 0xFFFFFFFFu, pre_n = 0, pre_id = 0xFFFFFFFFu;
+    float pre_d = 0.f;
+    const uint32_t head = min(c.cap, 64u);  // the row slots a first chunk may read
+    auto read_row = [&](uint32_t v, uint32_t* nr_out, uint32_t* id_out) {
+        const uint32_t lr = local_of(c, v);
+        const bool ok = lr != 0xFFFFFFFFu;
+        const uint32_t nr = ok ? c.cnt[lr] : 0u;
+        const uint32_t x = ok && lane < head ? c.rows[size_t(lr) * c.cap + lane].x : 0u;
+        *nr_out = nr;
+        *id_out = lane < min(nr, head) ? (x & kMask) : 0xFFFFFFFFu;
+    };
+    auto on_first = [&](float d0, uint32_t v0) {
+        if (pre_v == 0xFFFFFFFFu || key_less(d0, v0, pre_d, pre_v)) {
+            pre_v = v0;
+            pre_d = d0;
+            read_row(v0, &pre_n, &pre_id);
+        }
+    };
     uint32_t cur;
     while (W.pop(&cur)) {
         exps++;
-        const uint32_t lr = local_of(c, cur);
-        const uint32_t nr = lr != 0xFFFFFFFFu ? c.cnt[lr] : 0u;
-        const uint2* nrow = c.rows + size_t(lr) * c.cap;
+        uint32_t nr, first;
+        if (cur == pre_v) {
+            nr = pre_n;
+            first = pre_id;
+        } else {
+            read_row(cur, &nr, &first);
+        }
+        pre_v = 0xFFFFFFFFu;
+        bool predicted = false;
         for (uint32_t base = 0; base < nr; base += 64) {
-            const uint32_t id = base + lane < nr ? (nrow[base + lane].x & kMask) : 0xFFFFFFFFu;
+            uint32_t id = first;
+            if (base > 0) {
+                const uint2* nrow = c.rows + size_t(local_of(c, cur)) * c.cap;
+                id = base + lane < nr ? (nrow[base + lane].x & kMask) : 0xFFFFFFFFu;
+            }
             const uint32_t cnt = collect<K, false>(w, id, a.n, a.hmask, a.vis_mode);
+            if (!predicted) {
+                predicted = true;
+                pre_v = W.peek(&pre_d);
+                if (pre_v != 0xFFFFFFFFu) read_row(pre_v, &pre_n, &pre_id);
+            }
             if (cnt == 0) continue;
             any_distances<0, false, MF, 1>(a, w, cnt, u, metric, qn2);
             ndc += cnt;
@@ -154,7 +193,7 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
             wave_sync();
             const uint64_t P = (cnt == 64 ? ~0ull : ((1ull << cnt) - 1)) & (W.full() ? ballot(myd < W.lb) : ~0ull);
             if (!P) continue;
-            W.merge_chunk(myd, myv, P, w, [](float, uint32_t) {});
+            W.merge_chunk(myd, myv, P, w, on_first);
         }
     }
 
