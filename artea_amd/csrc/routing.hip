@@ -98,7 +98,9 @@ __device__ uint32_t prune_list(const Ctx& c, const uint32_t* av, const float* ad
 }
 
 // One pivot: warm-start beam search, filter, merge with the row, prune.
-template <int K, int MF>
+// CT > 0: the distance chains unrolled for rows of CT 16-float chunks, and
+// 32 rows per distance pass.
+template <int K, int MF, int CT>
 __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& p, const WaveLds& w, RouteLds& R,
                           uint32_t row) {
     const uint32_t lane = lane_id();
@@ -109,7 +111,7 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
     for (uint32_t i = lane; i < vis_bytes(K, false) / 4; i += 64) reinterpret_cast<uint32_t*>(w.vis)[i] = 0u;
     wave_sync();
     const float qn2 = MF == 2 ? quad_norm2(w.q, int(c.C), lane & 3u) : 0.f;
-    const U8Query<0> u{};
+    const U8Query<CT> u{};
     const uint32_t n0 = c.cnt[row];
     const uint2* rp = c.rows + size_t(row) * c.cap;
     CandList<K> W;
@@ -186,7 +188,7 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
                 if (pre_v != 0xFFFFFFFFu) read_row(pre_v, &pre_n, &pre_id);
             }
             if (cnt == 0) continue;
-            any_distances<0, false, MF, 1>(a, w, cnt, u, metric, qn2);
+            any_distances<CT, (CT > 0), MF, 1>(a, w, cnt, u, metric, qn2);
             ndc += cnt;
             const float myd = lane < cnt ? w.cdist[lane] : 0.f;
             const uint32_t myv = lane < cnt ? w.cand[lane] : 0u;
@@ -274,7 +276,7 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
 }
 
 // Persistent pass over rows (or the listed rows), a wavefront per pivot.
-template <int K, int MF>
+template <int K, int MF, int CT>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void routing_kernel(Ctx c, RouterArgs a, RoutingPass p,
                                                                       const uint32_t* list, uint32_t n_rows,
                                                                       uint32_t* work) {
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void routing_kernel(Ctx c, Rou
         if (lane_id() == 0) i = atomicAdd(work, 1u);
         i = readfirst_u(i);
         if (i >= n_rows) break;
-        route_row<K, MF>(c, a, p, w, R, list ? list[i] : i);
+        route_row<K, MF, CT>(c, a, p, w, R, list ? list[i] : i);
     }
 }
 
@@ -346,16 +348,26 @@ __global__ __launch_bounds__(256) void prune_kernel(Ctx c, float inv_scale, uint
 
 using RoutingFn = void (*)(Ctx, RouterArgs, RoutingPass, const uint32_t*, uint32_t, uint32_t*);
 
+// L2 rows of 96, 112 or 128 floats (6, 7 or 8 chunks) get the unrolled
+// chains; other widths and the DOT / cosine families the generic ones.
 template <int K>
-RoutingFn routing_fn(int metric) {
-    return metric == kDot ? routing_kernel<K, 1> : (metric == kCosine ? routing_kernel<K, 2> : routing_kernel<K, 0>);
+RoutingFn routing_fn(int metric, uint32_t chunks) {
+    if (metric == kDot) return routing_kernel<K, 1, 0>;
+    if (metric == kCosine) return routing_kernel<K, 2, 0>;
+    switch (chunks) {
+    case 6: return routing_kernel<K, 0, 6>;
+    case 7: return routing_kernel<K, 0, 7>;
+    case 8: return routing_kernel<K, 0, 8>;
+    default: return routing_kernel<K, 0, 0>;
+    }
 }
 
 size_t routing_lds(uint32_t D, int K) { return kWavesPerBlock * (wave_lds_bytes(D, K, false) + sizeof(RouteLds)); }
 
 hipError_t launch_routing(const Ctx& c, const RouterArgs& a, const RoutingPass& p, int K, const uint32_t* list,
                           uint32_t n_rows, uint32_t* work, hipStream_t s, int dev_cus) {
-    const RoutingFn fn = K == 2 ? routing_fn<2>(c.metric) : (K == 4 ? routing_fn<4>(c.metric) : routing_fn<8>(c.metric));
+    const RoutingFn fn = K == 2 ? routing_fn<2>(c.metric, c.C)
+                                : (K == 4 ? routing_fn<4>(c.metric, c.C) : routing_fn<8>(c.metric, c.C));
     const size_t lds = routing_lds(c.D, K);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        int(lds));
