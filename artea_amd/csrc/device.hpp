@@ -352,13 +352,17 @@ __device__ __forceinline__ float reduce16_lane(const float (&a)[16]) {
 
 // dist_func_t(a, b) (utils/simd_distance.hpp:64-79, 241-275) by one lane:
 // the 16 accumulator chains over chunks 0..C-1 in order, then the reduce
-// tree; the same bits as quad_distance.
+// tree; the same bits as quad_distance.  CT > 0: C == CT, the chunk loop
+// unrolled so every load of the row is in flight at once.
+template <int CT = 0>
 __device__ __forceinline__ float lane_distance(const float* __restrict__ a, const float* __restrict__ b, uint32_t C,
                                                int metric) {
     float acc[16], na[16], nb[16];
 #pragma unroll
     for (int l = 0; l < 16; ++l) acc[l] = na[l] = nb[l] = 0.f;
-    for (uint32_t c = 0; c < C; ++c) {
+    const uint32_t nc = CT > 0 ? uint32_t(CT) : C;
+#pragma unroll(CT > 0 ? CT : 1)
+    for (uint32_t c = 0; c < nc; ++c) {
         float x[16], y[16];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

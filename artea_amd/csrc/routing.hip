@@ -55,7 +55,9 @@ __device__ __forceinline__ uint32_t rank_in(const float* kd, const uint32_t* kv,
 // retained entry (pairs of two old entries skipped) lies closer to it than
 // its distance / scale; the retained entries go to b, marked old.  The
 // reference stops at the first such entry; the decision does not depend on
-// the order, so the retained entries are tested together, one per lane.
+// the order, so the retained entries are tested together, one per lane
+// (CT > 0: rows of CT chunks, the lane's chain unrolled).
+template <int CT>
 __device__ uint32_t prune_list(const Ctx& c, const uint32_t* av, const float* ad, uint32_t m, uint32_t* bv,
                                float* bd, float inv_scale, unsigned long long* pairs) {
     const uint32_t lane = lane_id();
@@ -75,7 +77,7 @@ __device__ uint32_t prune_list(const Ctx& c, const uint32_t* av, const float* ad
                 if (j < nr) {
                     const uint32_t rj = bv[j];
                     need = !(old_i && (rj & kNew) == 0);
-                    if (need) hit = lane_distance(cv, c.vecs + size_t(rj & kMask) * c.D, c.C, c.metric) < thr;
+                    if (need) hit = lane_distance<CT>(cv, c.vecs + size_t(rj & kMask) * c.D, c.C, c.metric) < thr;
                 }
                 np += uint64_t(__popcll(ballot(need)));
                 pruned = ballot(hit) != 0;
@@ -259,7 +261,7 @@ __device__ void route_row(const Ctx& c, const RouterArgs& a, const RoutingPass& 
     }
     wave_sync();
     unsigned long long pairs = 0;
-    const uint32_t nr = prune_list(c, R.av, R.ad, m, R.bv, R.bd, p.inv_scale, &pairs);
+    const uint32_t nr = prune_list<CT>(c, R.av, R.ad, m, R.bv, R.bd, p.inv_scale, &pairs);
     uint2* out = p.ext + size_t(row) * p.ext_stride;
     for (uint32_t j = lane; j < nr; j += 64) out[j] = make_uint2(R.bv[j], __float_as_uint(R.bd[j]));
     if (lane == 0) {
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(256) void prune_kernel(Ctx c, float inv_scale, uint
             s_ad[wid][j] = __uint_as_float(e.y);
         }
         wave_sync();
-        const uint32_t nr = prune_list(c, s_av[wid], s_ad[wid], n0, s_bv[wid], s_bd[wid], inv_scale, &pairs);
+        const uint32_t nr = prune_list<0>(c, s_av[wid], s_ad[wid], n0, s_bv[wid], s_bd[wid], inv_scale, &pairs);
         for (uint32_t j = lane; j < nr; j += 64) rp[j] = make_uint2(s_bv[wid][j], __float_as_uint(s_bd[wid][j]));
         if (lane == 0) c.cnt[row] = nr;
         wave_sync();
