@@ -111,6 +111,7 @@ struct RnetLds {
 // no kept entry (pairs of two old entries skipped) is closer to it than its
 // distance * inv_scale.  Kept entries go to bv/bd, marked old.  The kept
 // entries are tested together, one per lane.
+template <int CT = 0>
 __device__ uint32_t hier_prune(const float* vecs, uint32_t D, uint32_t C, int metric, const uint32_t* av,
                                const float* ad, uint32_t m, uint32_t* bv, float* bd, uint32_t max_nbr,
                                float inv_scale, unsigned long long* pairs) {
@@ -137,7 +138,7 @@ __device__ uint32_t hier_prune(const float* vecs, uint32_t D, uint32_t C, int me
                 if (j < nk) {
                     const uint32_t rj = bv[j];
                     need = !(old_i && (rj & kNewR) == 0);
-                    if (need) hit = lane_distance(cvec, vecs + size_t(rj & kMaskR) * D, C, metric) < thr;
+                    if (need) hit = lane_distance<CT>(cvec, vecs + size_t(rj & kMaskR) * D, C, metric) < thr;
                 }
                 np += uint64_t(__popcll(ballot(need)));
                 pruned = ballot(hit) != 0;
@@ -203,14 +204,15 @@ __device__ __forceinline__ void reset_exploration(CandList<K>& W) {
     W.n = keep;
 }
 
-// Beam loop over level l's rows from W (already reset and marked).
-template <int K, int MF, bool EXACT>
+// Beam loop over level l's rows from W (already reset and marked).  CT > 0:
+// rows of CT 16-float chunks (unrolled distance chains, 32 rows per pass).
+template <int K, int MF, bool EXACT, int CT>
 __device__ void beam_level(const RnetArgs& r, const RouterArgs& a, const WaveLds& w, CandList<K>& W, uint32_t l,
                            uint32_t* bm, uint32_t* touched, uint32_t& tcount, float qn2, unsigned long long& ndc,
                            unsigned long long& exps) {
     const uint32_t lane = lane_id();
     const int metric = family_metric<MF>(a.metric);
-    const U8Query<0> u{};
+    const U8Query<CT> u{};
     const uint32_t capl = cap_of(r, l);
     const uint2* rows = r.rows[l];
     const uint32_t* cnt = r.cnt[l];
@@ -224,7 +226,7 @@ __device__ void beam_level(const RnetArgs& r, const RouterArgs& a, const WaveLds
             const uint32_t c = EXACT ? exact_collect(w, id, a.n, bm, touched, tcount)
                                      : collect<K, false>(w, id, a.n, a.hmask, a.vis_mode);
             if (c == 0) continue;
-            any_distances<0, false, MF, 1>(a, w, c, u, metric, qn2);
+            any_distances<CT, (CT > 0), MF, 1>(a, w, c, u, metric, qn2);
             ndc += c;
             const float myd = lane < c ? w.cdist[lane] : 0.f;
             const uint32_t myv = lane < c ? w.cand[lane] : 0u;
@@ -259,7 +261,7 @@ __device__ void seed_list(const RouterArgs& a, const WaveLds& w, CandList<K>& W,
     }
 }
 
-template <int K, int MF>
+template <int K, int MF, int CT>
 __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLds& w, RnetLds& S, uint32_t slot,
                             uint32_t i) {
     const uint32_t lane = lane_id();
@@ -270,7 +272,7 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
     for (uint32_t l = lane; l <= r.L; l += 64) S.min_d[l] = FLT_MAX;
     wave_sync();
     const float qn2 = MF == 2 ? quad_norm2(w.q, int(a.C), lane & 3u) : 0.f;
-    const U8Query<0> u{};
+    const U8Query<CT> u{};
     uint2* stash = r.stash + size_t(slot) * (r.L + 1) * r.search_qs;
     uint32_t* stash_n = r.stash_n + size_t(slot) * (r.L + 1);
     for (uint32_t l = lane; l <= r.L; l += 64) stash_n[l] = 0;
@@ -288,7 +290,7 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
         const uint32_t e = r.top_bucket[draw(r.seed, kRnetStream, q, 0, 0, r.top_size)];
         if (lane == 0) w.cand[0] = e;
         wave_sync();
-        any_distances<0, false, MF, 1>(a, w, 1, u, family_metric<MF>(a.metric), qn2);
+        any_distances<CT, false, MF, 1>(a, w, 1, u, family_metric<MF>(a.metric), qn2);
         const float ed = w.cdist[0];
         wave_sync();
         ndc++;
@@ -303,7 +305,7 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
                     const uint32_t id = idx < W.n ? (W.v[k] & kVidMask) : 0xFFFFFFFFu;
                     exact_collect(w, id, a.n, bm, touched, tcount);
                 }
-                beam_level<K, MF, true>(r, a, w, W, l, bm, touched, tcount, qn2, ndc, exps);
+                beam_level<K, MF, true, CT>(r, a, w, W, l, bm, touched, tcount, qn2, ndc, exps);
             }
             const uint32_t nt = min(W.n, W.L);
             const float best = W.get_d(0);
@@ -352,7 +354,7 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
         for (uint32_t j = lane; j < vis_bytes(K, false) / 4; j += 64) reinterpret_cast<uint32_t*>(w.vis)[j] = 0u;
         wave_sync();
         seed_list<K>(a, w, W, stash + size_t(l) * r.search_qs, sm);
-        beam_level<K, MF, false>(r, a, w, W, l, nullptr, nullptr, tcount, qn2, ndc, exps);
+        beam_level<K, MF, false, CT>(r, a, w, W, l, nullptr, nullptr, tcount, qn2, ndc, exps);
         overflow += W.overflow;
         const uint32_t nt = min(W.n, W.L);
         uint32_t m = 0;
@@ -369,8 +371,8 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
             m += uint32_t(__popcll(km));
         }
         wave_sync();
-        const uint32_t nk = hier_prune(a.vecs, a.D, a.C, a.metric, S.cv, S.cd, m, S.kv, S.kd, cap_of(r, l), r.inv_scale,
-                                       &pairs);
+        const uint32_t nk = hier_prune<CT>(a.vecs, a.D, a.C, a.metric, S.cv, S.cd, m, S.kv, S.kd, cap_of(r, l),
+                                           r.inv_scale, &pairs);
         uint2* out = r.fwd + size_t(oi) * r.fstride + fwd_off(r, l);
         for (uint32_t j = lane; j < nk; j += 64) out[j] = make_uint2(S.kv[j], __float_as_uint(S.kd[j]));
         if (lane == 0) fc[l] = nk;
@@ -397,7 +399,7 @@ __device__ void plan_vertex(const RnetArgs& r, const RouterArgs& a, const WaveLd
     wave_sync();
 }
 
-template <int K, int MF>
+template <int K, int MF, int CT>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void rnet_plan_kernel(RnetArgs r, RouterArgs a) {
     const WaveLds w = wave_lds<K, false>(a);
     extern __shared__ __align__(16) unsigned char smem[];
@@ -409,7 +411,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void rnet_plan_kernel(RnetArgs
         if (lane_id() == 0) i = atomicAdd(r.work, 1u);
         i = readfirst_u(i);
         if (i >= r.count) break;
-        plan_vertex<K, MF>(r, a, w, S, slot, i);
+        plan_vertex<K, MF, CT>(r, a, w, S, slot, i);
     }
 }
 
@@ -477,6 +479,7 @@ __global__ void seg_head_kernel(const unsigned long long* keys, uint32_t m, uint
 // goes iff the new one stayed and is closer to it than its own distance /
 // scale; the first cap survivors are kept.  The new entry's distances to
 // the row are computed in one pass, one lane per entry.
+template <int CT>
 __global__ __launch_bounds__(256) void reverse_apply_kernel(RnetArgs r, const unsigned long long* keys,
                                                             const uint32_t* vals, const uint32_t* heads,
                                                             const uint32_t* n_heads, uint32_t m, uint32_t* seg_work) {
@@ -536,7 +539,8 @@ __global__ __launch_bounds__(256) void reverse_apply_kernel(RnetArgs r, const un
                     const uint32_t j = s * 64 + lane;
                     dn[s] = 0.f;
                     if (j < cap) {
-                        dn[s] = lane_distance(qvec, r.vecs + size_t(rv[j] & kMaskR) * r.D, r.C, r.metric);
+                        dn[s] =
+                            lane_distance<CT>(qvec, r.vecs + size_t(rv[j] & kMaskR) * r.D, r.C, r.metric);
                         hit_before |= j < pn && dn[s] < thr_new;
                     }
                 }
@@ -580,7 +584,7 @@ __global__ __launch_bounds__(256) void reverse_apply_kernel(RnetArgs r, const un
                 td[rank] = dp;
             }
             wave_sync();
-            n = hier_prune(r.vecs, r.D, r.C, r.metric, tv, td, cap + 1, rv, rd, cap, r.inv_scale, &pairs);
+            n = hier_prune<CT>(r.vecs, r.D, r.C, r.metric, tv, td, cap + 1, rv, rd, cap, r.inv_scale, &pairs);
         }
         for (uint32_t j = lane; j < n; j += 64) row[j] = make_uint2(rv[j], __float_as_uint(rd[j]));
         if (lane == 0) r.cnt[l][t] = n;
@@ -595,12 +599,32 @@ namespace {
 
 using PlanFn = void (*)(RnetArgs, RouterArgs);
 
+// L2 rows of 96, 112 or 128 floats (6, 7 or 8 chunks) get the unrolled
+// chains; other widths and the DOT / cosine families the generic ones.
 template <int K>
-PlanFn plan_fn(int metric) {
-    return metric == kDot ? rnet_plan_kernel<K, 1> : (metric == kCosine ? rnet_plan_kernel<K, 2> : rnet_plan_kernel<K, 0>);
+PlanFn plan_fn(int metric, uint32_t chunks) {
+    if (metric == kDot) return rnet_plan_kernel<K, 1, 0>;
+    if (metric == kCosine) return rnet_plan_kernel<K, 2, 0>;
+    switch (chunks) {
+    case 6: return rnet_plan_kernel<K, 0, 6>;
+    case 7: return rnet_plan_kernel<K, 0, 7>;
+    case 8: return rnet_plan_kernel<K, 0, 8>;
+    default: return rnet_plan_kernel<K, 0, 0>;
+    }
 }
-PlanFn plan_fn_k(int K, int metric) {
-    return K == 2 ? plan_fn<2>(metric) : (K == 4 ? plan_fn<4>(metric) : plan_fn<8>(metric));
+PlanFn plan_fn_k(int K, int metric, uint32_t chunks) {
+    return K == 2 ? plan_fn<2>(metric, chunks) : (K == 4 ? plan_fn<4>(metric, chunks) : plan_fn<8>(metric, chunks));
+}
+using ReverseFn = void (*)(RnetArgs, const unsigned long long*, const uint32_t*, const uint32_t*, const uint32_t*,
+                           uint32_t, uint32_t*);
+ReverseFn reverse_fn(int metric, uint32_t chunks) {
+    if (metric == kDot || metric == kCosine) return reverse_apply_kernel<0>;
+    switch (chunks) {
+    case 6: return reverse_apply_kernel<6>;
+    case 7: return reverse_apply_kernel<7>;
+    case 8: return reverse_apply_kernel<8>;
+    default: return reverse_apply_kernel<0>;
+    }
 }
 size_t plan_lds(uint32_t D, int K) { return kWavesPerBlock * (wave_lds_bytes(D, K, false) + sizeof(RnetLds)); }
 
@@ -715,7 +739,7 @@ int rnet_run(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric, cons
     // Wave slots: enough blocks for the occupancy of the widest kernel used.
     uint32_t blocks = 0;
     for (int K : {K0, 8}) {
-        const PlanFn fn = plan_fn_k(K, metric);
+        const PlanFn fn = plan_fn_k(K, metric, dim_pad / 16);
         const size_t lds = plan_lds(dim_pad, K);
         RNET_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      int(lds)));
@@ -803,7 +827,7 @@ int rnet_run(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric, cons
         a.count = count;
         hipError_t e = hipMemsetAsync(a.work, 0, 4, st);
         if (e != hipSuccess) return e;
-        const PlanFn fn = plan_fn_k(K, metric);
+        const PlanFn fn = plan_fn_k(K, metric, dim_pad / 16);
         const uint32_t nb = std::max(1u, std::min(blocks, (count + kWavesPerBlock - 1) / kWavesPerBlock));
         hipLaunchKernelGGL(fn, dim3(nb), dim3(kWavesPerBlock * 64), plan_lds(dim_pad, K), st, a, ra);
         return hipGetLastError();
@@ -872,7 +896,8 @@ int rnet_run(const float* d_vecs, uint32_t n, uint32_t dim_pad, int metric, cons
             tb = tmp_bytes;
             RNET_TRY(hipcub::DeviceSelect::Flagged(tmp, tb, iota, flags, heads, n_heads, int(m), st));
             RNET_TRY(hipMemsetAsync(seg_work, 0, 4, st));
-            hipLaunchKernelGGL(reverse_apply_kernel, dim3(std::max(1u, std::min((m + 3) / 4, uint32_t(cus) * 8u))),
+            hipLaunchKernelGGL(reverse_fn(metric, dim_pad / 16),
+                               dim3(std::max(1u, std::min((m + 3) / 4, uint32_t(cus) * 8u))),
                                dim3(256), 0, st, a, keys2, vals2, heads, n_heads, m, seg_work);
             RNET_TRY(hipGetLastError());
         }
