@@ -831,8 +831,9 @@ def workload_build(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--join", choices=("fp32", "bf16"), default="fp32",
-                    help="triangle-join pair distances of the search workload's build (the graph is the same)")
+    ap.add_argument("--join", choices=("fp32", "bf16"), default="bf16",
+                    help="triangle-join pair distances of the search workload's build: BF16 MFMA tiles with FP32 "
+                         "rechecks (default) or the FP32 chain; the graph is the same")
     ap.add_argument("--hierarchy", choices=("rnet", "geometric"), default="rnet",
                     help="levels and upper-level rows from r-net insertion (the reference's pipeline) or from the "
                          "seeded geometric draw with a random prefill")
