@@ -1278,7 +1278,7 @@ def main():
         "plain_qps": round(qps_plain, 1),
         "batched_over_plain": round(qps / qps_plain, 3),
         "recall@100": next(s["recall@100"] for s in sweep if s["queue"] == chosen),
-        "build_seconds": round(build_s, 2),
+        "build_seconds": round(build_s, 2), "build_join": args.join,
         "sweep": sweep,
         "ground_truth": {"fp64_kernel_s": round(gt_fp64_s, 3), "bf16_mfma_s": round(gt_mfma_s, 3),
                          "identical": gt_same, "kept_pairs_per_query": round(gst.get("kept_pairs", 0) / nq, 1)},
